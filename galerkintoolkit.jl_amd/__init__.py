@@ -1,0 +1,21 @@
+"""MI355X-native finite-element assembly for GalerkinToolkit-style problems.
+
+Host-side mirror of the reference's interface for the assembly path
+(GT.cartesian_mesh, GT.interior/boundary, GT.lagrange_space, GT.measure,
+GT.assemble_matrix/assemble_vector, GT.update_matrix!) whose numeric work runs
+in libgtk_hip.so (HIP kernels for gfx950 behind the C ABI of include/gtk_hip.h).
+"""
+from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, Laplace, Load, Mass, Measure,
+                       SparseMatrixCSC, assemble_matrix, assemble_matrix_with_free_and_dirichlet_columns,
+                       assemble_vector, face_constant_field, measure, update_matrix)
+from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
+from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space
+from . import inputs, refcell
+
+__all__ = [
+    "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "Laplace", "Load",
+    "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
+    "assemble_vector", "face_constant_field", "measure", "update_matrix", "Mesh", "cartesian_mesh",
+    "perturb_interior_nodes", "Domain", "LagrangeSpace", "boundary", "interior", "lagrange_space", "inputs",
+    "refcell",
+]

@@ -1,0 +1,168 @@
+"""ctypes binding of libgtk_hip.so (the C ABI declared in include/gtk_hip.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the shared library is missing or a GPU is absent, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libgtk_hip.so")
+
+GTK_FREE, GTK_DIRICHLET = 1, 2
+GTK_LAPLACE, GTK_MASS, GTK_ELASTICITY = 0, 1, 2
+GTK_FP64, GTK_MIXED = 0, 1
+GTK_F_CONST, GTK_F_CELL, GTK_F_NODAL = 0, 1, 2
+
+# every symbol the header declares
+EXPORTS = [
+    "gtk_create", "gtk_destroy", "gtk_last_error", "gtk_version", "gtk_set_mesh", "gtk_update_coordinates",
+    "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_get_pattern", "gtk_set_cell_coefficients",
+    "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
+    "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix",
+]
+
+
+class GtkStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("cells", C.c_int64), ("nnz", C.c_int64),
+                ("bytes_alg", C.c_int64), ("flops_alg", C.c_int64)]
+
+
+class GtkError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libgtk_hip.so and declare the prototypes (no GPU call)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise GtkError(f"{LIBPATH} is missing: build it with `make -C {os.path.join(HERE, 'csrc')}`")
+    L = C.CDLL(LIBPATH)
+    vp = C.c_void_p
+    f64 = np.ctypeslib.ndpointer(np.float64, flags="C")
+    i32 = np.ctypeslib.ndpointer(np.int32, flags="C")
+    i = C.c_int
+    i64 = C.c_int64
+    L.gtk_create.argtypes = [i, C.POINTER(vp)]
+    L.gtk_destroy.argtypes = [vp]
+    L.gtk_last_error.argtypes = [vp]
+    L.gtk_last_error.restype = C.c_char_p
+    L.gtk_version.restype = C.c_char_p
+    L.gtk_set_mesh.argtypes = [vp, i, i64, f64, i64, i, i32]
+    L.gtk_update_coordinates.argtypes = [vp, f64]
+    L.gtk_set_space.argtypes = [vp, i, i, i64, i64, i32]
+    L.gtk_set_reference.argtypes = [vp, i, f64, i, f64, f64, f64]
+    L.gtk_build_pattern.argtypes = [vp, i, i, C.POINTER(i64), C.POINTER(i64)]
+    L.gtk_get_pattern.argtypes = [vp, i, i, i32, i32]
+    L.gtk_set_cell_coefficients.argtypes = [vp, i, C.c_void_p]
+    L.gtk_assemble_matrix.argtypes = [vp, i, i, i, i, C.c_void_p, C.POINTER(GtkStats)]
+    L.gtk_get_values.argtypes = [vp, i, i, f64]
+    L.gtk_values_device_ptr.argtypes = [vp, i, i, C.POINTER(C.c_void_p)]
+    L.gtk_assemble_vector.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
+    L.gtk_assemble_matrix_async.argtypes = [vp, i, i, i, i]
+    L.gtk_sync.argtypes = [vp]
+    L.gtk_time_matrix.argtypes = [vp, i, i, i, i, i, C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+class Context:
+    """Owner of one gtk_ctx (device buffers, stream, plans)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        self.h = C.c_void_p()
+        rc = self.L.gtk_create(device, C.byref(self.h))
+        if rc != 0:
+            raise GtkError(f"gtk_create failed ({rc}): {self.L.gtk_last_error(None).decode()}")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise GtkError(f"{what} failed ({rc}): {self.L.gtk_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.L.gtk_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # thin wrappers ------------------------------------------------------------
+    def set_mesh(self, dim, xyz, cell_nodes_1b):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        cn = np.ascontiguousarray(cell_nodes_1b, np.int32)
+        self.check(self.L.gtk_set_mesh(self.h, dim, xyz.shape[0], xyz, cn.shape[0], cn.shape[1], cn), "gtk_set_mesh")
+
+    def update_coordinates(self, xyz):
+        self.check(self.L.gtk_update_coordinates(self.h, np.ascontiguousarray(xyz, np.float64)), "gtk_update_coordinates")
+
+    def set_space(self, ncomp, cell_dofs, nfree, ndiri):
+        cd = np.ascontiguousarray(cell_dofs, np.int32)
+        self.check(self.L.gtk_set_space(self.h, ncomp, cd.shape[1], nfree, ndiri, cd), "gtk_set_space")
+
+    def set_reference(self, w, phi, grad_phi, grad_geo):
+        w = np.ascontiguousarray(w, np.float64)
+        phi = np.ascontiguousarray(phi, np.float64)
+        self.check(self.L.gtk_set_reference(self.h, len(w), w, phi.shape[1], phi,
+                                            np.ascontiguousarray(grad_phi, np.float64),
+                                            np.ascontiguousarray(grad_geo, np.float64)), "gtk_set_reference")
+
+    def build_pattern(self, rows, cols):
+        nnz, ncols = C.c_int64(), C.c_int64()
+        self.check(self.L.gtk_build_pattern(self.h, rows, cols, C.byref(nnz), C.byref(ncols)), "gtk_build_pattern")
+        return nnz.value, ncols.value
+
+    def get_pattern(self, rows, cols, nnz, ncols):
+        colptr = np.zeros(ncols + 1, np.int32)
+        rowval = np.zeros(max(nnz, 1), np.int32)
+        self.check(self.L.gtk_get_pattern(self.h, rows, cols, colptr, rowval), "gtk_get_pattern")
+        return colptr, rowval[:nnz]
+
+    def set_cell_coefficients(self, coef, ncoef):
+        if coef is None:
+            self.check(self.L.gtk_set_cell_coefficients(self.h, 0, None), "gtk_set_cell_coefficients")
+            return
+        self._coef = np.ascontiguousarray(coef, np.float64)
+        self.check(self.L.gtk_set_cell_coefficients(self.h, ncoef, self._coef.ctypes.data_as(C.c_void_p)),
+                   "gtk_set_cell_coefficients")
+
+    def assemble_matrix(self, form, precision, rows, cols, nnz):
+        out = np.zeros(max(nnz, 1), np.float64)
+        st = GtkStats()
+        self.check(self.L.gtk_assemble_matrix(self.h, form, precision, rows, cols, out.ctypes.data_as(C.c_void_p),
+                                              C.byref(st)), "gtk_assemble_matrix")
+        return out[:nnz], st
+
+    def assemble_vector(self, fkind, f, rows, n):
+        out = np.zeros(max(n, 1), np.float64)
+        st = GtkStats()
+        self.check(self.L.gtk_assemble_vector(self.h, fkind, np.ascontiguousarray(f, np.float64), rows,
+                                              out.ctypes.data_as(C.c_void_p), C.byref(st)), "gtk_assemble_vector")
+        return out[:n], st
+
+    def assemble_matrix_async(self, form, precision, rows, cols):
+        self.check(self.L.gtk_assemble_matrix_async(self.h, form, precision, rows, cols), "gtk_assemble_matrix_async")
+
+    def sync(self):
+        self.check(self.L.gtk_sync(self.h), "gtk_sync")
+
+    def time_matrix(self, form, precision, rows, cols, launches):
+        ms = C.c_double()
+        self.check(self.L.gtk_time_matrix(self.h, form, precision, rows, cols, launches, C.byref(ms)), "gtk_time_matrix")
+        return ms.value
+
+    def values_device_ptr(self, rows, cols):
+        p = C.c_void_p()
+        self.check(self.L.gtk_values_device_ptr(self.h, rows, cols, C.byref(p)), "gtk_values_device_ptr")
+        return p.value

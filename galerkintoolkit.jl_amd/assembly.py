@@ -1,0 +1,220 @@
+"""GT-style assembly entry points backed by libgtk_hip.
+
+Mirrors GT.measure (GalerkinToolkit.jl src/problems.jl:4-25),
+GT.assemble_matrix / GT.assemble_vector (src/problems.jl:244-274, :319-350),
+GT.update_matrix! (src/problems.jl:352-361) and
+assemble_matrix_with_free_and_dirichlet_columns (src/problems.jl:363-380).
+The reference takes a Julia lambda and compiles it; here the weak form is
+one of the integrands the library implements (Laplace with a per-cell
+coefficient, mass, isotropic elasticity, and the load f . v), passed as a
+form object.  The result is the reference's SparseMatrixCSC{Float64,Int32}
+layout (1-based colptr/rowval) and a Float64 vector.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from . import refcell as R
+from .space import DIRICHLET, FREE
+
+FP64, MIXED = "fp64", "mixed"
+
+
+class Measure:
+    """GT.measure(domain, degree): the reference rule of every cell."""
+
+    def __init__(self, domain, degree):
+        self.domain = domain
+        self.degree = int(degree)
+        self.x, self.w = R.quadrature(domain.mesh.cellkind, self.degree)
+
+
+def measure(domain, degree):
+    return Measure(domain, degree)
+
+
+class FaceConstantField:
+    """GT.face_constant_field(values, domain): one value (or tuple) per cell."""
+
+    def __init__(self, values, domain):
+        self.values = np.ascontiguousarray(values, np.float64)
+        self.domain = domain
+
+
+def face_constant_field(values, domain):
+    return FaceConstantField(values, domain)
+
+
+class DiscreteField:
+    """A field of a space given by its free then Dirichlet values."""
+
+    def __init__(self, space, free_values, dirichlet_values=None):
+        self.space = space
+        fv = np.asarray(free_values, np.float64)
+        dv = np.zeros(space.ndiri) if dirichlet_values is None else np.asarray(dirichlet_values, np.float64)
+        self.values = np.ascontiguousarray(np.concatenate([fv, dv]))
+
+
+@dataclass
+class Laplace:
+    """int kappa grad(u) . grad(v) dOmega."""
+    dOmega: Measure
+    kappa: object = None
+
+
+@dataclass
+class Mass:
+    """int kappa u v dOmega."""
+    dOmega: Measure
+    kappa: object = None
+
+
+@dataclass
+class Elasticity:
+    """int sigma(eps(u)) : eps(v) dOmega, sigma = lambda tr(eps) I + 2 mu eps."""
+    dOmega: Measure
+    lam: object
+    mu: object
+
+
+@dataclass
+class Load:
+    """int f . v dOmega; f constant (scalar/tuple), per cell, or discrete."""
+    dOmega: Measure
+    f: object = 1.0
+
+
+class SparseMatrixCSC:
+    """Julia's SparseMatrixCSC{Float64,Int32} layout: 1-based colptr/rowval."""
+
+    def __init__(self, m, n, colptr, rowval, nzval):
+        self.m, self.n = int(m), int(n)
+        self.colptr, self.rowval, self.nzval = colptr, rowval, nzval
+
+    @property
+    def shape(self):
+        return (self.m, self.n)
+
+    @property
+    def nnz(self):
+        return len(self.rowval)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval, self.rowval - 1, self.colptr - 1), shape=(self.m, self.n))
+
+
+class _Assembler:
+    """The device context of one (space, measure) pair."""
+
+    def __init__(self, space, dOmega, device=0):
+        mesh = space.mesh
+        self.space = space
+        self.dOmega = dOmega
+        self.ctx = L.Context(device)
+        self.ctx.set_mesh(mesh.D, mesh.coords, mesh.cell_nodes_1b())
+        self.ctx.set_space(space.ncomp, space.cell_dofs, space.nfree, space.ndiri)
+        phi, gphi = space.fe.tabulate(dOmega.x)
+        _, gN = R.Lagrange(mesh.cellkind, 1).tabulate(dOmega.x)
+        self.ctx.set_reference(dOmega.w, phi, gphi, gN)
+        self.patterns = {}
+
+    def pattern(self, rows, cols):
+        key = (rows, cols)
+        if key not in self.patterns:
+            nnz, ncols = self.ctx.build_pattern(rows, cols)
+            colptr, rowval = self.ctx.get_pattern(rows, cols, nnz, ncols)
+            nrows = self.space.nfree if rows == FREE else self.space.ndiri
+            self.patterns[key] = (nnz, ncols, nrows, colptr, rowval)
+        return self.patterns[key]
+
+
+def _assembler(space, dOmega, device):
+    key = (id(dOmega), device)
+    cache = space.__dict__.setdefault("_assemblers", {})
+    a = cache.get(key)
+    if a is None or a.dOmega is not dOmega:
+        a = _Assembler(space, dOmega, device)
+        cache[key] = a
+    return a
+
+
+def _form_code(form):
+    if isinstance(form, Laplace):
+        return L.GTK_LAPLACE
+    if isinstance(form, Mass):
+        return L.GTK_MASS
+    if isinstance(form, Elasticity):
+        return L.GTK_ELASTICITY
+    raise TypeError("unsupported bilinear form")
+
+
+def _coefficients(form, ncells):
+    if isinstance(form, Elasticity):
+        lam = form.lam.values if isinstance(form.lam, FaceConstantField) else np.full(ncells, float(form.lam))
+        mu = form.mu.values if isinstance(form.mu, FaceConstantField) else np.full(ncells, float(form.mu))
+        return np.ascontiguousarray(np.stack([lam, mu], axis=1).ravel()), 2
+    k = form.kappa
+    if k is None:
+        return None, 0
+    if isinstance(k, FaceConstantField):
+        return k.values, 1
+    return np.full(ncells, float(k)), 1
+
+
+def assemble_matrix(form, T, trial_space, test_space, free_or_dirichlet=(FREE, FREE), precision=FP64, reuse=False,
+                    device=0):
+    """GT.assemble_matrix(a, Float64, U, V; free_or_dirichlet) on the GPU."""
+    if T not in (float, np.float64):
+        raise TypeError("values are Float64")
+    if trial_space is not test_space:
+        raise NotImplementedError("trial and test spaces must be the same space")
+    space = test_space
+    a = _assembler(space, form.dOmega, device)
+    rows, cols = free_or_dirichlet
+    nnz, ncols, nrows, colptr, rowval = a.pattern(rows, cols)
+    coef, ncoef = _coefficients(form, space.mesh.cells.shape[0])
+    a.ctx.set_cell_coefficients(coef, ncoef)
+    prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+    nzval, stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, nnz)
+    A = SparseMatrixCSC(nrows, ncols, colptr, rowval, nzval)
+    A.stats = stats
+    if reuse:
+        return A, (a, form, rows, cols, prec)
+    return A
+
+
+def update_matrix(A, cache, form=None):
+    """GT.update_matrix!(A, cache): refill values with the same pattern
+    (new coefficients or coordinates); the pattern is not rebuilt."""
+    a, form0, rows, cols, prec = cache
+    form = form0 if form is None else form
+    coef, ncoef = _coefficients(form, a.space.mesh.cells.shape[0])
+    a.ctx.set_cell_coefficients(coef, ncoef)
+    A.nzval, A.stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, A.nnz)
+    return A
+
+
+def assemble_vector(form, T, space, free_or_dirichlet=FREE, device=0):
+    """GT.assemble_vector(l, Float64, V; free_or_dirichlet) on the GPU."""
+    if not isinstance(form, Load):
+        raise TypeError("unsupported linear form")
+    a = _assembler(space, form.dOmega, device)
+    f = form.f
+    nc = space.ncomp
+    if isinstance(f, DiscreteField):
+        kind, data = L.GTK_F_NODAL, f.values
+    elif isinstance(f, FaceConstantField):
+        kind, data = L.GTK_F_CELL, f.values.reshape(-1)
+    else:
+        kind, data = L.GTK_F_CONST, np.broadcast_to(np.asarray(f, np.float64), (nc,)).copy()
+    n = space.nfree if free_or_dirichlet == FREE else space.ndiri
+    b, stats = a.ctx.assemble_vector(kind, data, free_or_dirichlet, n)
+    return b
+
+
+def assemble_matrix_with_free_and_dirichlet_columns(form, T, U, V, precision=FP64, device=0):
+    A = assemble_matrix(form, T, U, V, (FREE, FREE), precision=precision, device=device)
+    Ad = assemble_matrix(form, T, U, V, (FREE, DIRICHLET), precision=precision, device=device)
+    return A, Ad

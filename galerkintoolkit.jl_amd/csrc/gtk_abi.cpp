@@ -1,0 +1,620 @@
+// gtk_abi.cpp -- C ABI of libgtk_hip (declared in include/gtk_hip.h).
+//
+// A context owns the device copies of the mesh, the cell -> DOF map, the
+// reference tables, the per-cell coefficients, and, per (rows, cols) block,
+// the CSC pattern, the column-owner plan and the value array.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gtk_hip.h"
+#include "gtk_kernels.h"
+#include "gtk_plan.h"
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+struct MatrixPlan {
+  gtk::Pattern pat;
+  gtk::TilePlan tiles;
+  gtk::OffsetPlan offs;
+  DevBuf colptr, tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, ctr_off, nzval;
+  bool valid = false;
+};
+
+struct VectorPlan {
+  gtk::TilePlan tiles;
+  DevBuf tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, out, f;
+  int64_t nrows = 0;
+  bool valid = false;
+};
+
+int nthreads_host() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 1;
+  return (int)std::min(16u, n);
+}
+
+}  // namespace
+
+struct gtk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  // host copies
+  int dim = 0, nv = 0;
+  int64_t n_nodes = 0, n_cells = 0;
+  std::vector<double> xyz;
+  std::vector<int32_t> cell_nodes;  // 0-based
+  int ncomp = 0, dofs_per_cell = 0;
+  int64_t n_free = 0, n_diri = 0;
+  std::vector<int32_t> cell_dofs;
+  int nq = 0, nshape = 0;
+  std::vector<double> w, phi, gphi_t, gN;  // gphi_t: [q][d][s]
+  int ncoef = 0;
+  // device copies
+  DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef;
+  std::map<int, std::unique_ptr<MatrixPlan>> mplans;
+  std::map<int, std::unique_ptr<VectorPlan>> vplans;
+  bool has_mesh = false, has_space = false, has_ref = false;
+};
+
+static std::string g_create_err;
+
+#define FAIL(ctx, code, msg)  \
+  do {                        \
+    (ctx)->err = (msg);       \
+    return (code);            \
+  } while (0)
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+      return GTK_EDEVICE;                                                                   \
+    }                                                                                       \
+  } while (0)
+
+template <typename T>
+static int upload(gtk_ctx *ctx, DevBuf &b, const T *src, size_t count) {
+  size_t bytes = count * sizeof(T);
+  if (b.n < bytes || !b.p) {
+    b.release();
+    if (bytes) {
+      hipError_t e = hipMalloc(&b.p, bytes);
+      if (e != hipSuccess) {
+        b.p = nullptr;
+        ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return GTK_ENOMEM;
+      }
+    }
+    b.n = bytes;
+  }
+  if (bytes) HIPCHK(ctx, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return GTK_OK;
+}
+
+static int alloc(gtk_ctx *ctx, DevBuf &b, size_t bytes) {
+  if (b.p && b.n >= bytes) return GTK_OK;
+  b.release();
+  if (!bytes) return GTK_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return GTK_ENOMEM;
+  }
+  b.n = bytes;
+  return GTK_OK;
+}
+
+static gtk::MeshView view(const gtk_ctx *ctx) {
+  gtk::MeshView m;
+  m.dim = ctx->dim;
+  m.n_nodes = ctx->n_nodes;
+  m.n_cells = ctx->n_cells;
+  m.nodes_per_cell = ctx->nv;
+  m.xyz = ctx->xyz.data();
+  m.cell_nodes = ctx->cell_nodes.data();
+  m.cell_dofs = ctx->cell_dofs.data();
+  m.dofs_per_cell = ctx->dofs_per_cell;
+  m.n_free = ctx->n_free;
+  m.n_diri = ctx->n_diri;
+  return m;
+}
+
+static gtk::ElemKey elem_key(const gtk_ctx *ctx) {
+  return gtk::ElemKey{ctx->dim, ctx->nv, ctx->nshape, ctx->nq, ctx->ncomp};
+}
+
+// Largest geometry footprint per cell among the forms of this element, so
+// one plan serves every form and precision.
+static void tile_limits(const gtk_ctx *ctx, int &max_cells, int &chunk) {
+  gtk::ElemKey key = elem_key(ctx);
+  size_t geo = 0;
+  for (int form = 0; form < 3; form++) {
+    gtk::KernelInfo k;
+    if (gtk::find_matrix_kernel(key, form, GTK_FP64, k)) geo = std::max(geo, k.geo_bytes_per_cell);
+  }
+  if (!geo) geo = (size_t)ctx->nq * 8 * (ctx->dim * ctx->dim + 3);
+  const size_t budget = 48 * 1024;
+  max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, budget / geo));
+  // chunk: cells whose owned DOFs form a tile; (c^(1/D)+1)^D cells are needed
+  chunk = 1;
+  for (int c = 2; c <= 4096; c *= 2) {
+    double side = std::pow((double)c, 1.0 / ctx->dim);
+    double need = std::pow(side + 1.0, ctx->dim);
+    if (ctx->nv == ctx->dim + 1) need *= 2.5;  // simplices: larger neighbourhoods
+    if (need <= max_cells) chunk = c;
+  }
+}
+
+extern "C" {
+
+const char *gtk_version(void) { return "libgtk_hip gfx950 " __DATE__ " " __TIME__; }
+
+int gtk_create(int device, gtk_ctx **out) {
+  if (!out) {
+    g_create_err = "out is NULL";
+    return GTK_EINVAL;
+  }
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    g_create_err = std::string("no HIP device: ") + hipGetErrorString(e);
+    return GTK_EDEVICE;
+  }
+  if (device < 0 || device >= n) {
+    g_create_err = "device ordinal out of range";
+    return GTK_EINVAL;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return GTK_EDEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      g_create_err = std::string("libgtk_hip is built for gfx950, device is ") + prop.gcnArchName;
+      return GTK_EUNSUP;
+    }
+  }
+  gtk_ctx *ctx = new (std::nothrow) gtk_ctx();
+  if (!ctx) {
+    g_create_err = "out of host memory";
+    return GTK_ENOMEM;
+  }
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    g_create_err = "stream/event creation failed";
+    delete ctx;
+    return GTK_EDEVICE;
+  }
+  *out = ctx;
+  return GTK_OK;
+}
+
+int gtk_destroy(gtk_ctx *ctx) {
+  if (!ctx) return GTK_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->mplans.clear();
+  ctx->vplans.clear();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GTK_OK;
+}
+
+const char *gtk_last_error(const gtk_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int gtk_set_mesh(gtk_ctx *ctx, int dim, int64_t n_nodes, const double *xyz, int64_t n_cells, int nodes_per_cell,
+                 const int32_t *cell_nodes) {
+  if (!ctx) return GTK_EINVAL;
+  if (dim < 2 || dim > 3) FAIL(ctx, GTK_EINVAL, "dim must be 2 or 3");
+  if (n_nodes <= 0 || n_cells <= 0 || !xyz || !cell_nodes) FAIL(ctx, GTK_EINVAL, "empty mesh or NULL array");
+  if (nodes_per_cell < dim + 1 || nodes_per_cell > 8) FAIL(ctx, GTK_EINVAL, "nodes_per_cell out of range");
+  if (n_nodes > INT32_MAX || n_cells > INT32_MAX) FAIL(ctx, GTK_EINVAL, "mesh too large for Int32 ids");
+  std::vector<int32_t> cn((size_t)n_cells * nodes_per_cell);
+  for (size_t i = 0; i < cn.size(); i++) {
+    int32_t v = cell_nodes[i];
+    if (v < 1 || v > n_nodes) FAIL(ctx, GTK_EINVAL, "cell node id out of range (ids are 1-based)");
+    cn[i] = v - 1;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ctx->dim = dim;
+  ctx->nv = nodes_per_cell;
+  ctx->n_nodes = n_nodes;
+  ctx->n_cells = n_cells;
+  ctx->xyz.assign(xyz, xyz + n_nodes * dim);
+  ctx->cell_nodes.swap(cn);
+  int rc;
+  if ((rc = upload(ctx, ctx->d_xyz, ctx->xyz.data(), ctx->xyz.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_cell_nodes, ctx->cell_nodes.data(), ctx->cell_nodes.size()))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->has_mesh = true;
+  ctx->has_space = false;
+  ctx->mplans.clear();
+  ctx->vplans.clear();
+  return GTK_OK;
+}
+
+int gtk_update_coordinates(gtk_ctx *ctx, const double *xyz) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_mesh) FAIL(ctx, GTK_ESTATE, "gtk_set_mesh first");
+  if (!xyz) FAIL(ctx, GTK_EINVAL, "xyz is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ctx->xyz.assign(xyz, xyz + ctx->n_nodes * ctx->dim);
+  int rc = upload(ctx, ctx->d_xyz, ctx->xyz.data(), ctx->xyz.size());
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return GTK_OK;
+}
+
+int gtk_set_space(gtk_ctx *ctx, int ncomp, int dofs_per_cell, int64_t n_free, int64_t n_diri,
+                  const int32_t *cell_dofs) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_mesh) FAIL(ctx, GTK_ESTATE, "gtk_set_mesh first");
+  if (ncomp < 1 || ncomp > 3 || dofs_per_cell < 1 || dofs_per_cell % ncomp || dofs_per_cell > 64)
+    FAIL(ctx, GTK_EINVAL, "ncomp/dofs_per_cell out of range");
+  if (n_free < 0 || n_diri < 0 || n_free + n_diri <= 0 || !cell_dofs) FAIL(ctx, GTK_EINVAL, "invalid dof counts");
+  if (n_free > INT32_MAX || n_diri > INT32_MAX) FAIL(ctx, GTK_EINVAL, "dof count exceeds Int32");
+  size_t n = (size_t)ctx->n_cells * dofs_per_cell;
+  for (size_t i = 0; i < n; i++) {
+    int32_t d = cell_dofs[i];
+    if (d == 0 || d > n_free || -d > n_diri) FAIL(ctx, GTK_EINVAL, "cell dof out of range (GT signed encoding)");
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ctx->ncomp = ncomp;
+  ctx->dofs_per_cell = dofs_per_cell;
+  ctx->n_free = n_free;
+  ctx->n_diri = n_diri;
+  ctx->cell_dofs.assign(cell_dofs, cell_dofs + n);
+  int rc = upload(ctx, ctx->d_cell_dofs, ctx->cell_dofs.data(), n);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->has_space = true;
+  ctx->mplans.clear();
+  ctx->vplans.clear();
+  return GTK_OK;
+}
+
+int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const double *phi, const double *grad_phi,
+                      const double *grad_geo) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_mesh) FAIL(ctx, GTK_ESTATE, "gtk_set_mesh first");
+  if (nq < 1 || nq > 128 || nshape < 1 || nshape > 64 || !w || !phi || !grad_phi || !grad_geo)
+    FAIL(ctx, GTK_EINVAL, "invalid reference tables");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int D = ctx->dim;
+  ctx->nq = nq;
+  ctx->nshape = nshape;
+  ctx->w.assign(w, w + nq);
+  ctx->phi.assign(phi, phi + (size_t)nq * nshape);
+  ctx->gphi_t.assign((size_t)nq * nshape * D, 0.0);
+  for (int q = 0; q < nq; q++)
+    for (int s = 0; s < nshape; s++)
+      for (int d = 0; d < D; d++) ctx->gphi_t[((size_t)q * D + d) * nshape + s] = grad_phi[((size_t)q * nshape + s) * D + d];
+  ctx->gN.assign(grad_geo, grad_geo + (size_t)nq * ctx->nv * D);
+  int rc;
+  if ((rc = upload(ctx, ctx->d_w, ctx->w.data(), ctx->w.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_phi, ctx->phi.data(), ctx->phi.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_gphi, ctx->gphi_t.data(), ctx->gphi_t.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_gN, ctx->gN.data(), ctx->gN.size()))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->has_ref = true;
+  return GTK_OK;
+}
+
+int gtk_set_cell_coefficients(gtk_ctx *ctx, int ncoef, const double *coef) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_mesh) FAIL(ctx, GTK_ESTATE, "gtk_set_mesh first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!coef) {
+    ctx->ncoef = 0;
+    return GTK_OK;
+  }
+  if (ncoef < 1 || ncoef > 2) FAIL(ctx, GTK_EINVAL, "ncoef must be 1 or 2");
+  int rc = upload(ctx, ctx->d_coef, coef, (size_t)ctx->n_cells * ncoef);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->ncoef = ncoef;
+  return GTK_OK;
+}
+
+static int key_of(int rows, int cols) { return rows * 4 + cols; }
+
+static int upload_tiles(gtk_ctx *ctx, const gtk::TilePlan &t, DevBuf &tcp, DevBuf &tc, DevBuf &tcol, DevBuf &cid,
+                        DevBuf &ccp, DevBuf &cs, DevBuf &cb) {
+  int rc;
+  if ((rc = upload(ctx, tcp, t.tile_cell_ptr.data(), t.tile_cell_ptr.size()))) return rc;
+  if ((rc = upload(ctx, tc, t.tile_cells.data(), t.tile_cells.size()))) return rc;
+  if ((rc = upload(ctx, tcol, t.tile_col_ptr.data(), t.tile_col_ptr.size()))) return rc;
+  if ((rc = upload(ctx, cid, t.col_id.data(), t.col_id.size()))) return rc;
+  if ((rc = upload(ctx, ccp, t.col_ctr_ptr.data(), t.col_ctr_ptr.size()))) return rc;
+  if ((rc = upload(ctx, cs, t.ctr_slot.data(), t.ctr_slot.size()))) return rc;
+  if ((rc = upload(ctx, cb, t.ctr_b.data(), t.ctr_b.size()))) return rc;
+  return GTK_OK;
+}
+
+int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *ncols) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_space) FAIL(ctx, GTK_ESTATE, "gtk_set_space first");
+  if ((rows != GTK_FREE && rows != GTK_DIRICHLET) || (cols != GTK_FREE && cols != GTK_DIRICHLET))
+    FAIL(ctx, GTK_EINVAL, "rows/cols must be GTK_FREE or GTK_DIRICHLET");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto plan = std::make_unique<MatrixPlan>();
+  gtk::MeshView m = view(ctx);
+  int nt = nthreads_host();
+  std::string e = gtk::build_pattern(m, rows, cols, plan->pat, nt);
+  if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+  int max_cells, chunk;
+  tile_limits(ctx, max_cells, chunk);
+  e = gtk::build_tiles(m, cols, max_cells, 1 << 20, chunk, plan->tiles, nt);
+  if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+  e = gtk::build_offsets(m, plan->pat, plan->tiles, plan->offs, nt);
+  if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+  int rc;
+  if ((rc = upload(ctx, plan->colptr, plan->pat.colptr.data(), plan->pat.colptr.size()))) return rc;
+  if ((rc = upload_tiles(ctx, plan->tiles, plan->tile_cell_ptr, plan->tile_cells, plan->tile_col_ptr, plan->col_id,
+                         plan->col_ctr_ptr, plan->ctr_slot, plan->ctr_b)))
+    return rc;
+  if ((rc = upload(ctx, plan->ctr_off, plan->offs.off.data(), plan->offs.off.size()))) return rc;
+  if ((rc = alloc(ctx, plan->nzval, (size_t)std::max<int64_t>(1, plan->pat.nnz) * sizeof(double)))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  plan->valid = true;
+  if (nnz) *nnz = plan->pat.nnz;
+  if (ncols) *ncols = plan->pat.ncols;
+  ctx->mplans[key_of(rows, cols)] = std::move(plan);
+  return GTK_OK;
+}
+
+int gtk_get_pattern(gtk_ctx *ctx, int rows, int cols, int32_t *colptr, int32_t *rowval) {
+  if (!ctx) return GTK_EINVAL;
+  auto it = ctx->mplans.find(key_of(rows, cols));
+  if (it == ctx->mplans.end()) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  const gtk::Pattern &p = it->second->pat;
+  if (colptr) std::memcpy(colptr, p.colptr.data(), p.colptr.size() * sizeof(int32_t));
+  if (rowval) std::memcpy(rowval, p.rowval.data(), p.rowval.size() * sizeof(int32_t));
+  return GTK_OK;
+}
+
+static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision) {
+  gtk::KernelInfo k;
+  if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
+  if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
+  if (!gtk::find_matrix_kernel(elem_key(ctx), form, precision, k))
+    FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
+  if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
+  if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
+  if (plan.tiles.ntiles == 0) return GTK_OK;
+  size_t lds = k.table_bytes + (size_t)plan.tiles.max_tile_cells * k.geo_bytes_per_cell;
+  lds = (lds + 15) & ~(size_t)15;
+  size_t acc_off = lds;
+  lds += (size_t)(gtk::kBlock / 64) * k.cols_per_wave * std::max(1, plan.offs.max_seg) * sizeof(double);
+  if (lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
+  if (lds > 64 * 1024) HIPCHK(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  gtk::MatrixArgs a;
+  a.xyz = ctx->d_xyz.as<double>();
+  a.cell_nodes = ctx->d_cell_nodes.as<int32_t>();
+  a.coef = ctx->ncoef ? ctx->d_coef.as<double>() : nullptr;
+  a.w = ctx->d_w.as<double>();
+  a.phi = ctx->d_phi.as<double>();
+  a.gphi = ctx->d_gphi.as<double>();
+  a.gN = ctx->d_gN.as<double>();
+  a.tile_cell_ptr = plan.tile_cell_ptr.as<int32_t>();
+  a.tile_cells = plan.tile_cells.as<int32_t>();
+  a.tile_col_ptr = plan.tile_col_ptr.as<int32_t>();
+  a.col_id = plan.col_id.as<int32_t>();
+  a.col_ctr_ptr = plan.col_ctr_ptr.as<int32_t>();
+  a.ctr_slot = plan.ctr_slot.as<uint16_t>();
+  a.ctr_b = plan.ctr_b.as<uint16_t>();
+  a.ctr_off = plan.ctr_off.as<uint16_t>();
+  a.colptr = plan.colptr.as<int32_t>();
+  a.nzval = plan.nzval.as<double>();
+  a.max_seg = std::max(1, plan.offs.max_seg);
+  a.acc_offset = (int)acc_off;
+  void *args[] = {&a};
+  HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(plan.tiles.ntiles), dim3(gtk::kBlock), args, lds, ctx->stream));
+  return GTK_OK;
+}
+
+static MatrixPlan *find_plan(gtk_ctx *ctx, int rows, int cols) {
+  auto it = ctx->mplans.find(key_of(rows, cols));
+  return it == ctx->mplans.end() ? nullptr : it->second.get();
+}
+
+static void fill_stats(const gtk_ctx *ctx, const MatrixPlan &p, gtk_stats *st) {
+  st->cells = ctx->n_cells;
+  st->nnz = p.pat.nnz;
+  // algorithmic bytes: coordinates, cell->node map, coefficients, the plan's
+  // index streams read once, and the values written once
+  int64_t b = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 +
+              (int64_t)ctx->n_cells * ctx->ncoef * 8 + (int64_t)p.offs.off.size() * 2 +
+              (int64_t)p.tiles.ctr_slot.size() * 4 + (int64_t)p.tiles.col_id.size() * 8 + p.pat.nnz * 8;
+  st->bytes_alg = b;
+  const int64_t D = ctx->dim, nd = ctx->dofs_per_cell;
+  st->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * D * nd * nd + 2 * D * D * nd + 2 * ctx->nv * D * D);
+}
+
+int gtk_assemble_matrix_async(gtk_ctx *ctx, int form, int precision, int rows, int cols) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return launch_matrix(ctx, *p, form, precision);
+}
+
+int gtk_sync(gtk_ctx *ctx) {
+  if (!ctx) return GTK_EINVAL;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return GTK_OK;
+}
+
+int gtk_assemble_matrix(gtk_ctx *ctx, int form, int precision, int rows, int cols, double *nzval_out,
+                        gtk_stats *stats) {
+  if (!ctx) return GTK_EINVAL;
+  auto t0 = std::chrono::steady_clock::now();
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  int rc = launch_matrix(ctx, *p, form, precision);
+  if (rc) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  if (nzval_out && p->pat.nnz)
+    HIPCHK(ctx, hipMemcpyAsync(nzval_out, p->nzval.p, p->pat.nnz * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->kernel_ms = ms;
+    fill_stats(ctx, *p, stats);
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return GTK_OK;
+}
+
+int gtk_time_matrix(gtk_ctx *ctx, int form, int precision, int rows, int cols, int launches, double *ms_per_launch) {
+  if (!ctx) return GTK_EINVAL;
+  if (launches < 1 || !ms_per_launch) FAIL(ctx, GTK_EINVAL, "launches must be >= 1");
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int i = 0; i < launches; i++) {
+    int rc = launch_matrix(ctx, *p, form, precision);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *ms_per_launch = ms / launches;
+  return GTK_OK;
+}
+
+int gtk_get_values(gtk_ctx *ctx, int rows, int cols, double *nzval_out) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  if (!nzval_out) FAIL(ctx, GTK_EINVAL, "nzval_out is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (p->pat.nnz)
+    HIPCHK(ctx, hipMemcpyAsync(nzval_out, p->nzval.p, p->pat.nnz * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return GTK_OK;
+}
+
+int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  if (!dptr) FAIL(ctx, GTK_EINVAL, "dptr is NULL");
+  *dptr = p->nzval.as<double>();
+  return GTK_OK;
+}
+
+int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, double *b_out, gtk_stats *stats) {
+  if (!ctx) return GTK_EINVAL;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!ctx->has_space) FAIL(ctx, GTK_ESTATE, "gtk_set_space first");
+  if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
+  if (rows != GTK_FREE && rows != GTK_DIRICHLET) FAIL(ctx, GTK_EINVAL, "rows must be GTK_FREE or GTK_DIRICHLET");
+  if (fkind < GTK_F_CONST || fkind > GTK_F_NODAL || !f) FAIL(ctx, GTK_EINVAL, "invalid load");
+  gtk::KernelInfo k;
+  if (!gtk::find_vector_kernel(elem_key(ctx), fkind, k)) FAIL(ctx, GTK_EUNSUP, "no vector kernel for this element");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto &slot = ctx->vplans[rows];
+  if (!slot) {
+    auto vp = std::make_unique<VectorPlan>();
+    int max_cells, chunk;
+    tile_limits(ctx, max_cells, chunk);
+    std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, chunk, vp->tiles, nthreads_host());
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    int rc = upload_tiles(ctx, vp->tiles, vp->tile_cell_ptr, vp->tile_cells, vp->tile_col_ptr, vp->col_id,
+                          vp->col_ctr_ptr, vp->ctr_slot, vp->ctr_b);
+    if (rc) return rc;
+    vp->nrows = rows == GTK_FREE ? ctx->n_free : ctx->n_diri;
+    if ((rc = alloc(ctx, vp->out, (size_t)std::max<int64_t>(1, vp->nrows) * sizeof(double)))) return rc;
+    vp->valid = true;
+    slot = std::move(vp);
+  }
+  VectorPlan &vp = *slot;
+  size_t nf = fkind == GTK_F_CONST ? (size_t)ctx->ncomp
+                                   : (fkind == GTK_F_CELL ? (size_t)ctx->n_cells * ctx->ncomp : (size_t)(ctx->n_free + ctx->n_diri));
+  int rc = upload(ctx, vp.f, f, nf);
+  if (rc) return rc;
+  size_t lds = k.table_bytes + (size_t)vp.tiles.max_tile_cells * k.geo_bytes_per_cell;
+  if (lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
+  if (lds > 64 * 1024) HIPCHK(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  gtk::VectorArgs a;
+  a.xyz = ctx->d_xyz.as<double>();
+  a.cell_nodes = ctx->d_cell_nodes.as<int32_t>();
+  a.cell_dofs = ctx->d_cell_dofs.as<int32_t>();
+  a.f = vp.f.as<double>();
+  a.n_free = ctx->n_free;
+  a.w = ctx->d_w.as<double>();
+  a.phi = ctx->d_phi.as<double>();
+  a.gN = ctx->d_gN.as<double>();
+  a.tile_cell_ptr = vp.tile_cell_ptr.as<int32_t>();
+  a.tile_cells = vp.tile_cells.as<int32_t>();
+  a.tile_col_ptr = vp.tile_col_ptr.as<int32_t>();
+  a.col_id = vp.col_id.as<int32_t>();
+  a.col_ctr_ptr = vp.col_ctr_ptr.as<int32_t>();
+  a.ctr_slot = vp.ctr_slot.as<uint16_t>();
+  a.ctr_b = vp.ctr_b.as<uint16_t>();
+  a.out = vp.out.as<double>();
+  void *args[] = {&a};
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (vp.tiles.ntiles)
+    HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(vp.tiles.ntiles), dim3(gtk::kBlock), args, lds, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  if (b_out && vp.nrows)
+    HIPCHK(ctx, hipMemcpyAsync(b_out, vp.out.p, vp.nrows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->kernel_ms = ms;
+    stats->cells = ctx->n_cells;
+    stats->nnz = vp.nrows;
+    stats->bytes_alg = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 + vp.nrows * 8;
+    stats->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * ctx->dofs_per_cell + 2 * ctx->nv * ctx->dim * ctx->dim);
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return GTK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+// gtk_plan.cpp -- symbolic phase of libgtk_hip (host C++).
+//
+// build_pattern restates what the reference's symbolic pass plus compress
+// produce (GalerkinToolkit.jl src/assembly.jl:75-157 counts the COO triplets
+// cell by cell with trial DOFs outer and test DOFs inner, skipping DOFs
+// outside the FREE/DIRICHLET selection; :571-575 compresses them into a
+// SparseMatrixCSC{Float64,Int32}): column j holds the sorted union of the row
+// DOFs of every cell that contains column DOF j, structural zeros included.
+// The cached-CSC back end (src/assembly.jl:694-771) builds the same pattern.
+//
+// build_tiles / build_offsets are this library's own work decomposition:
+// every column is owned by one tile, so the kernels write each nonzero once
+// from one workgroup in a fixed contributor order (cells ascending, the
+// order in which the reference's COO triplets are summed).
+#include "gtk_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+
+namespace gtk {
+
+namespace {
+
+inline int32_t map_dof(int32_t dof, int set) {
+  if (set == 1) return dof > 0 ? dof : 0;  // GTK_FREE
+  return dof < 0 ? -dof : 0;               // GTK_DIRICHLET
+}
+
+void parallel_for(int64_t n, int nthreads, const std::function<void(int, int64_t, int64_t)> &f) {
+  if (nthreads <= 1 || n < 4096) {
+    f(0, 0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; t++) {
+    int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back(f, t, b, e);
+  }
+  for (auto &x : th) x.join();
+}
+
+// (cell, local dof) incidence of every DOF of set S, cells ascending.
+struct Incidence {
+  std::vector<int64_t> ptr;  // nset+1
+  std::vector<int32_t> cell;
+  std::vector<uint16_t> b;
+};
+
+Incidence incidence(const MeshView &m, int set, int64_t nset) {
+  Incidence inc;
+  inc.ptr.assign(nset + 2, 0);
+  const int nd = m.dofs_per_cell;
+  for (int64_t c = 0; c < m.n_cells; c++)
+    for (int a = 0; a < nd; a++) {
+      int32_t j = map_dof(m.cell_dofs[c * nd + a], set);
+      if (j) inc.ptr[j]++;
+    }
+  // ptr[j] currently counts DOF j (1-based) -> exclusive prefix over 0-based
+  std::vector<int64_t> start(nset + 1, 0);
+  for (int64_t j = 0; j < nset; j++) start[j + 1] = start[j] + inc.ptr[j + 1];
+  inc.ptr.assign(start.begin(), start.end());
+  inc.cell.resize(start[nset]);
+  inc.b.resize(start[nset]);
+  std::vector<int64_t> fill(start.begin(), start.end() - 1);
+  for (int64_t c = 0; c < m.n_cells; c++)
+    for (int a = 0; a < nd; a++) {
+      int32_t j = map_dof(m.cell_dofs[c * nd + a], set);
+      if (!j) continue;
+      int64_t p = fill[j - 1]++;
+      inc.cell[p] = (int32_t)c;
+      inc.b[p] = (uint16_t)a;
+    }
+  return inc;
+}
+
+uint64_t spread3(uint64_t x) {
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
+uint64_t spread2(uint64_t x) {
+  x &= 0xffffffff;
+  x = (x | x << 16) & 0x0000ffff0000ffffULL;
+  x = (x | x << 8) & 0x00ff00ff00ff00ffULL;
+  x = (x | x << 4) & 0x0f0f0f0f0f0f0f0fULL;
+  x = (x | x << 2) & 0x3333333333333333ULL;
+  x = (x | x << 1) & 0x5555555555555555ULL;
+  return x;
+}
+
+}  // namespace
+
+std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, int nthreads) {
+  if (rows != 1 && rows != 2) return "rows must be GTK_FREE or GTK_DIRICHLET";
+  if (cols != 1 && cols != 2) return "cols must be GTK_FREE or GTK_DIRICHLET";
+  out.rows = rows;
+  out.cols = cols;
+  out.nrows = rows == 1 ? m.n_free : m.n_diri;
+  out.ncols = cols == 1 ? m.n_free : m.n_diri;
+  const int64_t ncols = out.ncols;
+  const int nd = m.dofs_per_cell;
+  Incidence inc = incidence(m, cols, ncols);
+  std::vector<int64_t> len(ncols, 0);
+  std::vector<std::vector<int32_t>> part(std::max(1, nthreads));
+  std::vector<int64_t> part_begin(std::max(1, nthreads), 0), part_end(std::max(1, nthreads), 0);
+  parallel_for(ncols, nthreads, [&](int t, int64_t b, int64_t e) {
+    std::vector<int32_t> &buf = part[t];
+    part_begin[t] = b;
+    part_end[t] = e;
+    std::vector<int32_t> tmp;
+    for (int64_t j = b; j < e; j++) {
+      tmp.clear();
+      for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++) {
+        const int32_t *dofs = m.cell_dofs + (int64_t)inc.cell[k] * nd;
+        for (int a = 0; a < nd; a++) {
+          int32_t r = map_dof(dofs[a], rows);
+          if (r) tmp.push_back(r);
+        }
+      }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      len[j] = (int64_t)tmp.size();
+      buf.insert(buf.end(), tmp.begin(), tmp.end());
+    }
+  });
+  out.colptr.assign(ncols + 1, 0);
+  int64_t acc = 0;
+  out.colptr[0] = 1;
+  for (int64_t j = 0; j < ncols; j++) {
+    acc += len[j];
+    if (acc + 1 > INT32_MAX) return "nnz exceeds the Int32 range of SparseMatrixCSC{Float64,Int32}";
+    out.colptr[j + 1] = (int32_t)(acc + 1);
+  }
+  out.nnz = acc;
+  out.rowval.resize(acc);
+  for (size_t t = 0; t < part.size(); t++) {
+    if (part[t].empty()) continue;
+    int64_t dst = out.colptr[part_begin[t]] - 1;
+    std::memcpy(out.rowval.data() + dst, part[t].data(), part[t].size() * sizeof(int32_t));
+  }
+  return "";
+}
+
+std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int chunk, TilePlan &out,
+                        int nthreads) {
+  (void)nthreads;
+  if (max_cells < 1 || max_cells > 65535) return "max_cells out of range";
+  out = TilePlan();
+  out.set = set;
+  out.ndofs_cell = m.dofs_per_cell;
+  const int64_t nset = set == 1 ? m.n_free : m.n_diri;
+  out.nset = nset;
+  Incidence inc = incidence(m, set, nset);
+  // cells in Morton order of their centroids
+  const int D = m.dim, nv = m.nodes_per_cell;
+  std::vector<double> cen((size_t)m.n_cells * D, 0.0);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t c = 0; c < m.n_cells; c++)
+    for (int d = 0; d < D; d++) {
+      double s = 0;
+      for (int n = 0; n < nv; n++) s += m.xyz[(int64_t)m.cell_nodes[c * nv + n] * D + d];
+      s /= nv;
+      cen[c * D + d] = s;
+      lo[d] = std::min(lo[d], s);
+      hi[d] = std::max(hi[d], s);
+    }
+  std::vector<uint64_t> key(m.n_cells);
+  const double grid = D == 3 ? 2097151.0 : 4294967295.0;
+  for (int64_t c = 0; c < m.n_cells; c++) {
+    uint64_t q[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+      double ext = hi[d] - lo[d];
+      double t = ext > 0 ? (cen[c * D + d] - lo[d]) / ext : 0.0;
+      q[d] = (uint64_t)std::llround(t * grid);
+    }
+    key[c] = D == 3 ? (spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2) : (spread2(q[0]) | spread2(q[1]) << 1);
+  }
+  std::vector<int32_t> order(m.n_cells);
+  for (int64_t c = 0; c < m.n_cells; c++) order[c] = (int32_t)c;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  std::vector<int32_t> rank(m.n_cells);
+  for (int64_t i = 0; i < m.n_cells; i++) rank[order[i]] = (int32_t)i;
+  // owner rank of each dof = smallest Morton rank among its cells
+  std::vector<int32_t> owner(nset, INT32_MAX);
+  for (int64_t j = 0; j < nset; j++)
+    for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++) owner[j] = std::min(owner[j], rank[inc.cell[k]]);
+  // dofs ordered by (owner rank, dof id): counting sort over ranks
+  std::vector<int64_t> rp(m.n_cells + 1, 0);
+  for (int64_t j = 0; j < nset; j++)
+    if (owner[j] != INT32_MAX) rp[owner[j] + 1]++;
+  for (int64_t i = 0; i < m.n_cells; i++) rp[i + 1] += rp[i];
+  std::vector<int32_t> cols_sorted(rp[m.n_cells]);
+  {
+    std::vector<int64_t> f(rp.begin(), rp.end() - 1);
+    for (int64_t j = 0; j < nset; j++)
+      if (owner[j] != INT32_MAX) cols_sorted[f[owner[j]]++] = (int32_t)j;
+  }
+  // greedy tiles inside each chunk of `chunk` consecutive owner ranks
+  std::vector<int32_t> mark(m.n_cells, -1);
+  std::vector<int32_t> cur_cells, cur_cols;
+  auto flush = [&]() {
+    if (cur_cols.empty()) return;
+    int tile = out.ntiles++;
+    std::sort(cur_cells.begin(), cur_cells.end());
+    out.tile_cell_ptr.push_back((int32_t)out.tile_cells.size());
+    out.tile_col_ptr.push_back((int32_t)out.col_id.size());
+    for (size_t s = 0; s < cur_cells.size(); s++) mark[cur_cells[s]] = (int32_t)s;  // slot
+    out.max_tile_cells = std::max<int>(out.max_tile_cells, (int)cur_cells.size());
+    out.max_tile_cols = std::max<int>(out.max_tile_cols, (int)cur_cols.size());
+    out.tile_cells.insert(out.tile_cells.end(), cur_cells.begin(), cur_cells.end());
+    for (int32_t j : cur_cols) {
+      out.col_id.push_back(j);
+      out.col_ctr_ptr.push_back((int32_t)out.ctr_slot.size());
+      int n = 0;
+      for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++) {
+        out.ctr_slot.push_back((uint16_t)mark[inc.cell[k]]);
+        out.ctr_b.push_back(inc.b[k]);
+        n++;
+      }
+      out.max_ctr = std::max(out.max_ctr, n);
+    }
+    for (int32_t c : cur_cells) mark[c] = -1;
+    (void)tile;
+    cur_cells.clear();
+    cur_cols.clear();
+  };
+  std::vector<int32_t> in_tile(m.n_cells, 0);
+  int32_t stamp = 0;
+  int64_t chunk_id = -1;
+  for (size_t i = 0; i < cols_sorted.size(); i++) {
+    int32_t j = cols_sorted[i];
+    int64_t cid = owner[j] / std::max(1, chunk);
+    if (cid != chunk_id) {
+      flush();
+      chunk_id = cid;
+      stamp++;
+    }
+    // cells this column would add
+    int add = 0;
+    for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++)
+      if (in_tile[inc.cell[k]] != stamp) add++;
+    if (!cur_cols.empty() && ((int)cur_cells.size() + add > max_cells || (int)cur_cols.size() + 1 > max_cols)) {
+      flush();
+      stamp++;
+    }
+    if (add > max_cells) return "a single DOF touches more cells than a tile may hold";
+    for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++) {
+      int32_t c = inc.cell[k];
+      if (in_tile[c] != stamp) {
+        in_tile[c] = stamp;
+        cur_cells.push_back(c);
+      }
+    }
+    cur_cols.push_back(j);
+  }
+  flush();
+  out.tile_cell_ptr.push_back((int32_t)out.tile_cells.size());
+  out.tile_col_ptr.push_back((int32_t)out.col_id.size());
+  out.col_ctr_ptr.push_back((int32_t)out.ctr_slot.size());
+  if ((int64_t)out.col_id.size() != nset) return "tile plan does not cover every DOF of the set";
+  return "";
+}
+
+std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads) {
+  if (p.cols != t.set) return "tile plan set differs from the pattern columns";
+  const int nd = m.dofs_per_cell;
+  const int64_t ncol_plan = (int64_t)t.col_id.size();
+  out.off.assign(t.ctr_slot.size() * (size_t)nd, kNoRow);
+  int max_seg = 0;
+  for (int64_t j = 0; j < p.ncols; j++) max_seg = std::max(max_seg, p.colptr[j + 1] - p.colptr[j]);
+  if (max_seg >= kNoRow) return "column segment too long for 16-bit offsets";
+  out.max_seg = max_seg;
+  // tile of each planned column (to read tile cells)
+  std::vector<int32_t> col_tile(ncol_plan);
+  for (int tl = 0; tl < t.ntiles; tl++)
+    for (int32_t i = t.tile_col_ptr[tl]; i < t.tile_col_ptr[tl + 1]; i++) col_tile[i] = tl;
+  std::string err;
+  parallel_for(ncol_plan, nthreads, [&](int, int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      int32_t j = t.col_id[i];
+      const int32_t *seg = p.rowval.data() + (p.colptr[j] - 1);
+      const int32_t len = p.colptr[j + 1] - p.colptr[j];
+      const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[col_tile[i]];
+      for (int32_t k = t.col_ctr_ptr[i]; k < t.col_ctr_ptr[i + 1]; k++) {
+        int32_t c = cells[t.ctr_slot[k]];
+        const int32_t *dofs = m.cell_dofs + (int64_t)c * nd;
+        for (int a = 0; a < nd; a++) {
+          int32_t r = map_dof(dofs[a], p.rows);
+          if (!r) continue;
+          const int32_t *it = std::lower_bound(seg, seg + len, r);
+          out.off[(size_t)k * nd + a] = (uint16_t)(it - seg);
+        }
+      }
+    }
+  });
+  return err;
+}
+
+}  // namespace gtk

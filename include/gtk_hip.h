@@ -1,0 +1,148 @@
+/*
+ * gtk_hip.h -- C ABI of libgtk_hip.so, the MI355X (gfx950) finite-element
+ * assembly library.
+ *
+ * The library replaces the per-cell loop that the reference generates for
+ * GT.assemble_matrix / GT.assemble_vector over GT.measure on a
+ * GT.lagrange_space (GalerkinToolkit.jl: src/problems.jl:244-274 and
+ * :319-350 call generate_assemble_vector / generate_assemble_matrix,
+ * src/compiler.jl:1097-1155, whose loop templates are
+ * src/compiler.jl:1826-1997), together with the allocation protocol that
+ * turns the contributions into a SparseMatrixCSC{Float64,Int32} and a
+ * Vector{Float64} (src/assembly.jl:75-208, :428-588, cached CSC :592-893).
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a negative GTK_E* code; nothing
+ *    throws across the ABI.  gtk_last_error() returns the message of the
+ *    last failure on the context (or of the last failed gtk_create).
+ *  - The caller owns every host array; set_* calls copy them to device
+ *    memory owned by the context until gtk_destroy.
+ *  - Indices follow the reference: 1-based Int32 node ids, DOF ids in the
+ *    reference's signed encoding (>0 free, <0 Dirichlet), CSC colptr/rowval
+ *    1-based Int32 exactly as SparseMatrixCSC{Float64,Int32} stores them.
+ *  - A context is not thread-safe.  Calls are synchronous unless named
+ *    *_async; distinct contexts may be used from distinct threads.
+ */
+#ifndef GTK_HIP_H
+#define GTK_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gtk_ctx gtk_ctx;
+
+/* status codes */
+#define GTK_OK 0
+#define GTK_EINVAL (-1)   /* invalid argument or inconsistent sizes */
+#define GTK_ESTATE (-2)   /* call out of order (mesh/space/pattern missing) */
+#define GTK_EDEVICE (-3)  /* HIP runtime error */
+#define GTK_ENOMEM (-4)   /* host or device allocation failed */
+#define GTK_EUNSUP (-5)   /* element/form/precision combination not built */
+
+/* row / column selectors, as the reference's FREE / DIRICHLET
+ * (src/field.jl:136-142, skip_dof src/assembly.jl:155-157) */
+#define GTK_FREE 1
+#define GTK_DIRICHLET 2
+
+/* bilinear forms recognised by the library (the integrands of
+ * src/compiler.jl:1544-1606 that the supported weak forms lower to) */
+#define GTK_LAPLACE 0     /* int kappa grad(u).grad(v), kappa per cell (or 1) */
+#define GTK_MASS 1        /* int kappa u v */
+#define GTK_ELASTICITY 2  /* int sigma(eps(u)) : eps(v), (lambda, mu) per cell */
+
+/* arithmetic of the per-cell contraction */
+#define GTK_FP64 0   /* everything in FP64: the plain drop-in */
+#define GTK_MIXED 1  /* FP64 geometry (J, J^-1, det J) and FP64 cross-cell
+                        accumulation; per-cell contraction in FP32 */
+
+/* load-vector source kinds */
+#define GTK_F_CONST 0  /* one value per component */
+#define GTK_F_CELL 1   /* one value per cell and component */
+#define GTK_F_NODAL 2  /* discrete field: values at free dofs then Dirichlet dofs */
+
+typedef struct gtk_stats {
+  double kernel_ms;     /* device time of the assembly launch(es), HIP events */
+  double total_ms;      /* wall time of the call on the host */
+  int64_t cells;        /* cells processed */
+  int64_t nnz;          /* nonzeros written (matrix) or rows written (vector) */
+  int64_t bytes_alg;    /* algorithmic HBM bytes of the launch (see DESIGN) */
+  int64_t flops_alg;    /* algorithmic flops of the launch */
+} gtk_stats;
+
+/* Context life cycle.  device: HIP device ordinal. */
+int gtk_create(int device, gtk_ctx **out);
+int gtk_destroy(gtk_ctx *ctx);
+const char *gtk_last_error(const gtk_ctx *ctx);
+/* Library build identifier (compile date, target). */
+const char *gtk_version(void);
+
+/* Mesh: node coordinates (n_nodes*dim, node-major, as
+ * GT.node_coordinates), cells of one reference type with nodes_per_cell
+ * nodes of an order-1 geometry (GT.face_nodes(mesh, D), 1-based).
+ * Replaces: the node_coordinates / face_nodes reads of the generated loop
+ * (src/accessors.jl:638-662, :941-968). */
+int gtk_set_mesh(gtk_ctx *ctx, int dim, int64_t n_nodes, const double *xyz, int64_t n_cells,
+                 int nodes_per_cell, const int32_t *cell_nodes);
+/* Replace node coordinates only (same topology). */
+int gtk_update_coordinates(gtk_ctx *ctx, const double *xyz);
+
+/* Space: cell -> DOF map in GT's signed encoding (GT.face_dofs of the
+ * space on the cells, src/space.jl:299-535), with ncomp components per
+ * node stored component-major.  dofs_per_cell = nshape*ncomp. */
+int gtk_set_space(gtk_ctx *ctx, int ncomp, int dofs_per_cell, int64_t n_free, int64_t n_diri,
+                  const int32_t *cell_dofs);
+
+/* Reference tables at the quadrature points of GT.measure(Omega, degree)
+ * (src/problems.jl:4-25, src/quadrature.jl:36-157): weights w[nq],
+ * shape values phi[nq*nshape], reference gradients grad_phi[nq*nshape*dim],
+ * order-1 geometry gradients grad_geo[nq*nodes_per_cell*dim]
+ * (tabulator, src/space.jl:960-970; src/accessors.jl:486-496). */
+int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const double *phi,
+                      const double *grad_phi, const double *grad_geo);
+
+/* Symbolic phase for a (rows, cols) block: builds the CSC pattern and the
+ * column-owner work plan.  Equals the pattern compress() produces for the
+ * COO triplets of allocate_matrix (src/assembly.jl:75-157, :571-575);
+ * structural zeros are kept, rows sorted within columns.
+ * Returns nnz and the column count. */
+int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *ncols);
+int gtk_get_pattern(gtk_ctx *ctx, int rows, int cols, int32_t *colptr, int32_t *rowval);
+
+/* Per-cell coefficients resident on the device: kappa (ncoef=1) or
+ * (lambda, mu) (ncoef=2), n_cells*ncoef values, cell-major
+ * (GT.face_constant_field, src/field.jl:60-70).  NULL resets to 1. */
+int gtk_set_cell_coefficients(gtk_ctx *ctx, int ncoef, const double *coef);
+
+/* Numeric assembly of the (rows, cols) block into the device-resident
+ * value array of that pattern; nzval_out (host, nnz) may be NULL to keep
+ * the values on the device.  Replaces the generated loop of
+ * src/compiler.jl:1865-1917 plus contribute!/compress
+ * (src/assembly.jl:189-208, :545-575). */
+int gtk_assemble_matrix(gtk_ctx *ctx, int form, int precision, int rows, int cols, double *nzval_out,
+                        gtk_stats *stats);
+/* Copy the device values of a (rows, cols) block to the host. */
+int gtk_get_values(gtk_ctx *ctx, int rows, int cols, double *nzval_out);
+/* Device address of the value array of a (rows, cols) block (for callers
+ * that keep the matrix on the GPU). */
+int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr);
+
+/* Load vector b_i = int f . v_i over the rows set (src/problems.jl:244-274,
+ * template src/compiler.jl:1927-1997, dense_vector src/assembly.jl:558-569).
+ * f: GTK_F_CONST ncomp values, GTK_F_CELL n_cells*ncomp, GTK_F_NODAL
+ * (n_free+n_diri) values.  b_out (host) may be NULL. */
+int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, double *b_out, gtk_stats *stats);
+
+/* Enqueue the matrix assembly of a (rows, cols) block on the context's
+ * stream without synchronising (for timing loops); gtk_sync waits. */
+int gtk_assemble_matrix_async(gtk_ctx *ctx, int form, int precision, int rows, int cols);
+int gtk_sync(gtk_ctx *ctx);
+/* Average device time per launch of the last timed run (HIP events on the
+ * context stream around n launches). */
+int gtk_time_matrix(gtk_ctx *ctx, int form, int precision, int rows, int cols, int launches, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,35 @@
+import importlib.util
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "galerkintoolkit.jl_amd")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+def load_package():
+    if "gtk_amd" in sys.modules:
+        return sys.modules["gtk_amd"]
+    spec = importlib.util.spec_from_file_location("gtk_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gtk_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def GT():
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def O():
+    import oracle
+    return oracle

@@ -1,0 +1,92 @@
+"""Host logic of the symbolic phase: CSC pattern, column-owner tiles and
+row offsets, replayed on the CPU with exact integer-valued entries and
+compared bit for bit with a COO assembly in the reference's contribution
+order (cells ascending, trial outer, test inner; src/assembly.jl:189-208)."""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "galerkintoolkit.jl_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    out = tmp_path_factory.mktemp("plan") / "libplan_harness.so"
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-shared", "-fPIC", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "host", "plan_harness.cpp"), os.path.join(CSRC, "gtk_plan.cpp"),
+                    "-o", str(out), "-lpthread"], check=True)
+    L = C.CDLL(str(out))
+    f64 = np.ctypeslib.ndpointer(np.float64, flags="C")
+    i32 = np.ctypeslib.ndpointer(np.int32, flags="C")
+    L.harness_run.argtypes = [C.c_int, C.c_int64, f64, C.c_int64, C.c_int, i32, C.c_int, C.c_int64, C.c_int64, i32,
+                              C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), i32, i32, f64, C.c_int64,
+                              C.POINTER(C.c_int), C.c_char_p, C.c_int]
+    return L
+
+
+def coo_reference(cell_dofs, rows, cols, nrows, ncols):
+    """COO triplets in the reference order, duplicates summed, CSC."""
+    import scipy.sparse as sp
+    I, J, V = [], [], []
+    nc, nd = cell_dofs.shape
+    sel = (lambda d: d > 0) if True else None
+    for c in range(nc):
+        for b in range(nd):
+            dj = cell_dofs[c, b]
+            if not ((dj > 0) if cols == 1 else (dj < 0)):
+                continue
+            for a in range(nd):
+                di = cell_dofs[c, a]
+                if not ((di > 0) if rows == 1 else (di < 0)):
+                    continue
+                I.append(abs(di) - 1)
+                J.append(abs(dj) - 1)
+                V.append(1.0 + c * 7.0 + a * 131.0 + b * 977.0)
+    A = sp.coo_matrix((V, (I, J)), shape=(nrows, ncols)).tocsc()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def run(harness, GT, cells, order, simplex=False, diri=0, rows=1, cols=1, max_cells=64, chunk=8):
+    D = len(cells)
+    m = GT.cartesian_mesh(tuple([0.0, 1.0] * D), cells, simplexify=simplex)
+    db = GT.boundary(m) if diri else None
+    V = GT.lagrange_space(GT.interior(m), order, dirichlet_boundary=db)
+    cd = np.ascontiguousarray(V.cell_dofs, np.int32)
+    cn = np.ascontiguousarray(m.cells, np.int32)
+    cap = 10_000_000
+    colptr = np.zeros((V.nfree if cols == 1 else V.ndiri) + 1, np.int32)
+    rowval = np.zeros(cap, np.int32)
+    nz = np.zeros(cap)
+    nnz = C.c_int64()
+    nt = C.c_int()
+    err = C.create_string_buffer(256)
+    rc = harness.harness_run(D, m.coords.shape[0], np.ascontiguousarray(m.coords), cn.shape[0], cn.shape[1], cn,
+                             cd.shape[1], V.nfree, V.ndiri, cd, rows, cols, max_cells, chunk, C.byref(nnz), colptr,
+                             rowval, nz, cap, C.byref(nt), err, 256)
+    assert rc == 0, err.value.decode()
+    nrows = V.nfree if rows == 1 else V.ndiri
+    ref = coo_reference(V.cell_dofs, rows, cols, nrows, len(colptr) - 1)
+    n = nnz.value
+    assert np.array_equal(colptr - 1, ref.indptr)
+    assert np.array_equal(rowval[:n] - 1, ref.indices)
+    assert np.array_equal(nz[:n], ref.data)
+    return nt.value
+
+
+@pytest.mark.parametrize("cells,order,simplex,diri", [
+    ((6, 5), 1, False, 1), ((5, 4), 2, False, 0), ((4, 5), 3, False, 1), ((4, 3, 5), 1, False, 0),
+    ((3, 4, 3), 2, False, 1), ((3, 3, 2), 2, True, 0), ((4, 4), 2, True, 1)])
+@pytest.mark.parametrize("max_cells,chunk", [(64, 8), (27, 1), (200, 64)])
+def test_plan_replay(harness, GT, cells, order, simplex, diri, max_cells, chunk):
+    run(harness, GT, cells, order, simplex, diri, max_cells=max_cells, chunk=chunk)
+
+
+def test_plan_dirichlet_columns(harness, GT):
+    run(harness, GT, (5, 6), 2, diri=1, rows=1, cols=2)
