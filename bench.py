@@ -47,6 +47,24 @@ def survey_bytes_per_cell_q2(n):
     return 8 * 3 * nodes + 4 * 8 * cells + 4 * 27 * cells + 4 * 729 * cells + 8 * nnz
 
 
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of nbytes (the HIP runtime's copy
+    kernel) so hardware byte counters can be scaled against known traffic."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"copy calibration: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr)
+
+
 def cpu_baseline(n_sample, seed):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker, timed here only as the CPU baseline
@@ -73,6 +91,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=40, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one plain device-to-device copy of this many MiB (torch) so hardware "
+                         "byte counters can be scaled against a copy of known size")
+    ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both",
+                    help="time only one arithmetic form (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; a multi-GPU run is not part of it (use --gpus 1)",
@@ -108,8 +131,13 @@ def main():
         kern_ms = ctx.time_matrix(L.GTK_LAPLACE, prec, L.GTK_FREE, L.GTK_FREE, args.steps)
         return wall_ms, kern_ms
 
-    wall64, kern64 = run(L.GTK_FP64)
-    wallmx, kernmx = run(L.GTK_MIXED)
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
+    wall64 = kern64 = wallmx = kernmx = float("nan")
+    if args.only in ("both", "fp64"):
+        wall64, kern64 = run(L.GTK_FP64)
+    if args.only in ("both", "mixed"):
+        wallmx, kernmx = run(L.GTK_MIXED)
     cells = mesh.cells.shape[0]
     bytes_alg = survey_bytes_per_cell_q2(n)
     flops_alg = cells * 27 * (2 * 3 * 27 * 27 + 2 * 9 * 27 + 2 * 8 * 9)

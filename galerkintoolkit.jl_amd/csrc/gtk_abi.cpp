@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -44,6 +45,7 @@ struct MatrixPlan {
   gtk::TilePlan tiles;
   gtk::OffsetPlan offs;
   DevBuf colptr, tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, ctr_off, nzval;
+  DevBuf tile_batch_ptr, batch_col;
   bool valid = false;
 };
 
@@ -154,24 +156,47 @@ static gtk::ElemKey elem_key(const gtk_ctx *ctx) {
   return gtk::ElemKey{ctx->dim, ctx->nv, ctx->nshape, ctx->nq, ctx->ncomp};
 }
 
-// Largest geometry footprint per cell among the forms of this element, so
-// one plan serves every form and precision.
-static void tile_limits(const gtk_ctx *ctx, int &max_cells, int &chunk) {
+// Tile limits: the largest cell count whose staged geometry fits the LDS
+// left by the reference tables, the FP64 column accumulators and (matrix-core
+// kernels) the staged plan metadata, for the most demanding form/precision
+// of this element (one plan serves all).
+static constexpr size_t kLdsBudget = 150 * 1024;
+static constexpr size_t kMetaBudget = 16 * 1024;
+
+static void tile_limits(const gtk_ctx *ctx, int max_seg, int &max_cells, int &chunk, int &max_ctr) {
   gtk::ElemKey key = elem_key(ctx);
-  size_t geo = 0;
+  size_t geo = 0, tables = 0, acc = 0, meta = 0;
+  max_ctr = 1 << 20;
   for (int form = 0; form < 3; form++) {
     gtk::KernelInfo k;
-    if (gtk::find_matrix_kernel(key, form, GTK_FP64, k)) geo = std::max(geo, k.geo_bytes_per_cell);
+    if (!gtk::find_matrix_kernel(key, form, GTK_FP64, 2, true, k)) continue;
+    geo = std::max(geo, k.geo_bytes_per_cell);
+    tables = std::max(tables, k.table_bytes);
+    const size_t waves = (size_t)k.block / 64;
+    size_t a = k.mfma ? waves * gtk::kBatchSlots * ctx->dofs_per_cell * sizeof(double)
+                      : waves * k.cols_per_wave * std::max(1, max_seg) * sizeof(double);
+    acc = std::max(acc, a);
+    if (k.mfma) {
+      meta = kMetaBudget + waves * 256;
+      const int stride = (ctx->dofs_per_cell + 15) / 16 * 16;
+      const int obytes = max_seg < 0xFF ? 1 : 2;
+      max_ctr = (int)((kMetaBudget - 2048) / (4 + (size_t)stride * obytes));
+    }
   }
-  if (!geo) geo = (size_t)ctx->nq * 8 * (ctx->dim * ctx->dim + 3);
-  const size_t budget = 48 * 1024;
-  max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, budget / geo));
-  // chunk: cells whose owned DOFs form a tile; (c^(1/D)+1)^D cells are needed
+  if (!geo) {  // no matrix kernel: vector-only use of the plan
+    geo = (size_t)ctx->nq * 8 * ctx->ncomp;
+    tables = (size_t)ctx->nq * ctx->nshape * 8;
+  }
+  const size_t fixed = tables + acc + meta + 1024;
+  const size_t avail = kLdsBudget > fixed ? kLdsBudget - fixed : 8 * geo;
+  max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, avail / geo));
+  // chunk: cells whose owned DOFs form a tile; about (c^(1/D)+1)^D cells are
+  // then needed (more around simplices)
   chunk = 1;
   for (int c = 2; c <= 4096; c *= 2) {
     double side = std::pow((double)c, 1.0 / ctx->dim);
     double need = std::pow(side + 1.0, ctx->dim);
-    if (ctx->nv == ctx->dim + 1) need *= 2.5;  // simplices: larger neighbourhoods
+    if (ctx->nv == ctx->dim + 1) need *= 2.5;
     if (need <= max_cells) chunk = c;
   }
 }
@@ -379,9 +404,12 @@ int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *n
   int nt = nthreads_host();
   std::string e = gtk::build_pattern(m, rows, cols, plan->pat, nt);
   if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
-  int max_cells, chunk;
-  tile_limits(ctx, max_cells, chunk);
-  e = gtk::build_tiles(m, cols, max_cells, 1 << 20, chunk, plan->tiles, nt);
+  int max_seg = 0;
+  for (int64_t j = 0; j < plan->pat.ncols; j++)
+    max_seg = std::max(max_seg, plan->pat.colptr[j + 1] - plan->pat.colptr[j]);
+  int max_cells, chunk, max_ctr;
+  tile_limits(ctx, max_seg, max_cells, chunk, max_ctr);
+  e = gtk::build_tiles(m, cols, max_cells, 1 << 20, max_ctr, chunk, plan->tiles, nt);
   if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
   e = gtk::build_offsets(m, plan->pat, plan->tiles, plan->offs, nt);
   if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
@@ -390,7 +418,10 @@ int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *n
   if ((rc = upload_tiles(ctx, plan->tiles, plan->tile_cell_ptr, plan->tile_cells, plan->tile_col_ptr, plan->col_id,
                          plan->col_ctr_ptr, plan->ctr_slot, plan->ctr_b)))
     return rc;
-  if ((rc = upload(ctx, plan->ctr_off, plan->offs.off.data(), plan->offs.off.size()))) return rc;
+  if ((rc = upload(ctx, plan->ctr_off, plan->offs.data.data(), plan->offs.data.size()))) return rc;
+  if ((rc = upload(ctx, plan->tile_batch_ptr, plan->tiles.tile_batch_ptr.data(), plan->tiles.tile_batch_ptr.size())))
+    return rc;
+  if ((rc = upload(ctx, plan->batch_col, plan->tiles.batch_col.data(), plan->tiles.batch_col.size()))) return rc;
   if ((rc = alloc(ctx, plan->nzval, (size_t)std::max<int64_t>(1, plan->pat.nnz) * sizeof(double)))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   plan->valid = true;
@@ -414,15 +445,33 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   gtk::KernelInfo k;
   if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (!gtk::find_matrix_kernel(elem_key(ctx), form, precision, k))
+  const bool batches_fit = plan.tiles.max_ctr <= gtk::kBatchSlots;
+  if (!gtk::find_matrix_kernel(elem_key(ctx), form, precision, plan.offs.bytes, batches_fit, k))
     FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
   if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
   if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
   if (plan.tiles.ntiles == 0) return GTK_OK;
-  size_t lds = k.table_bytes + (size_t)plan.tiles.max_tile_cells * k.geo_bytes_per_cell;
-  lds = (lds + 15) & ~(size_t)15;
+  auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+  const size_t waves = (size_t)k.block / 64;
+  size_t lds = align16(k.table_bytes + (size_t)plan.tiles.max_tile_cells * k.geo_bytes_per_cell);
+  int lds_col = 0, lds_ctr = 0, lds_off = 0, lds_bat = 0, lds_wave = 0;
+  if (k.mfma) {
+    lds_col = (int)lds;
+    lds += align16((size_t)plan.tiles.max_tile_cols * 16);
+    lds_ctr = (int)lds;
+    lds += align16((size_t)plan.tiles.max_tile_ctr * 4);
+    lds_off = (int)lds;
+    lds += align16((size_t)plan.tiles.max_tile_ctr * plan.offs.stride * plan.offs.bytes);
+    lds_bat = (int)lds;
+    lds += align16((size_t)(plan.tiles.max_tile_batches + 1) * 4);
+    lds_wave = (int)lds;
+    lds += waves * 256;
+  }
   size_t acc_off = lds;
-  lds += (size_t)(gtk::kBlock / 64) * k.cols_per_wave * std::max(1, plan.offs.max_seg) * sizeof(double);
+  // matrix-core kernels: batch accumulators plus 64 private sink slots
+  const int acc_per_wave =
+      k.mfma ? plan.offs.max_batch_acc + 64 : k.cols_per_wave * std::max(1, plan.offs.max_seg);
+  lds += waves * acc_per_wave * sizeof(double);
   if (lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
   if (lds > 64 * 1024) HIPCHK(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   gtk::MatrixArgs a;
@@ -438,15 +487,25 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   a.tile_col_ptr = plan.tile_col_ptr.as<int32_t>();
   a.col_id = plan.col_id.as<int32_t>();
   a.col_ctr_ptr = plan.col_ctr_ptr.as<int32_t>();
+  a.tile_batch_ptr = plan.tile_batch_ptr.as<int32_t>();
+  a.batch_col = plan.batch_col.as<int32_t>();
   a.ctr_slot = plan.ctr_slot.as<uint16_t>();
   a.ctr_b = plan.ctr_b.as<uint16_t>();
-  a.ctr_off = plan.ctr_off.as<uint16_t>();
+  a.ctr_off = plan.ctr_off.as<uint8_t>();
+  a.off_stride = plan.offs.stride;
+  a.off_bytes = plan.offs.bytes;
   a.colptr = plan.colptr.as<int32_t>();
   a.nzval = plan.nzval.as<double>();
   a.max_seg = std::max(1, plan.offs.max_seg);
   a.acc_offset = (int)acc_off;
+  a.acc_per_wave = acc_per_wave;
+  a.lds_col = lds_col;
+  a.lds_ctr = lds_ctr;
+  a.lds_off = lds_off;
+  a.lds_bat = lds_bat;
+  a.lds_wave = lds_wave;
   void *args[] = {&a};
-  HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(plan.tiles.ntiles), dim3(gtk::kBlock), args, lds, ctx->stream));
+  HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(plan.tiles.ntiles), dim3(k.block), args, lds, ctx->stream));
   return GTK_OK;
 }
 
@@ -461,7 +520,7 @@ static void fill_stats(const gtk_ctx *ctx, const MatrixPlan &p, gtk_stats *st) {
   // algorithmic bytes: coordinates, cell->node map, coefficients, the plan's
   // index streams read once, and the values written once
   int64_t b = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 +
-              (int64_t)ctx->n_cells * ctx->ncoef * 8 + (int64_t)p.offs.off.size() * 2 +
+              (int64_t)ctx->n_cells * ctx->ncoef * 8 + (int64_t)p.offs.data.size() +
               (int64_t)p.tiles.ctr_slot.size() * 4 + (int64_t)p.tiles.col_id.size() * 8 + p.pat.nnz * 8;
   st->bytes_alg = b;
   const int64_t D = ctx->dim, nd = ctx->dofs_per_cell;
@@ -559,9 +618,9 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
   auto &slot = ctx->vplans[rows];
   if (!slot) {
     auto vp = std::make_unique<VectorPlan>();
-    int max_cells, chunk;
-    tile_limits(ctx, max_cells, chunk);
-    std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, chunk, vp->tiles, nthreads_host());
+    int max_cells, chunk, max_ctr;
+    tile_limits(ctx, 0, max_cells, chunk, max_ctr);
+    std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, 1 << 20, chunk, vp->tiles, nthreads_host());
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     int rc = upload_tiles(ctx, vp->tiles, vp->tile_cell_ptr, vp->tile_cells, vp->tile_col_ptr, vp->col_id,
                           vp->col_ctr_ptr, vp->ctr_slot, vp->ctr_b);

@@ -217,10 +217,261 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
           v = fma(t, G[D * D], v);
         }
       }
-      const uint16_t off = args.ctr_off[(int64_t)k * NDOF + a];
-      if (off != kNoRowDev) acc[off] += (double)v;
+      const int off = args.off_bytes == 1 ? (int)args.ctr_off[(int64_t)k * args.off_stride + a]
+                                          : (int)reinterpret_cast<const uint16_t *>(args.ctr_off)[(int64_t)k * args.off_stride + a];
+      if (off != (args.off_bytes == 1 ? 0xFF : 0xFFFF)) acc[off] += (double)v;
     }
     for (int r = a; r < len; r += NDOF) args.nzval[seg0 + r] = acc[r];
+  }
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename CT>
+struct Acc4;
+template <>
+struct Acc4<double> {
+  using type = v4d;
+};
+template <>
+struct Acc4<float> {
+  using type = v4f;
+};
+
+// D(16x16) += A(16x4) B(4x16); lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result rows held by lane l, r = 0..3 (measured on gfx950): FP32 holds
+// D[4*(l/16) + r][l%16], FP64 holds D[l/16 + 4*r][l%16].
+__device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline v4f mfma16x16x4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Row of the A operand that lane i supplies so that, for both precisions,
+// lane l ends up holding rows 4*(l/16) + r of the product: FP64 rows are
+// permuted by i -> 4*(i%4) + i/4, FP32 rows are taken in order.
+template <typename CT>
+__device__ constexpr int a_row(int i) {
+  return sizeof(CT) == 8 ? 4 * (i & 3) + (i >> 2) : i;
+}
+
+template <typename OffT>
+struct Off4;
+template <>
+struct Off4<uint8_t> {
+  using type = uint32_t;  // four 8-bit offsets
+  static constexpr uint32_t none = 0xFF;
+  __device__ static int get(type v, int r) { return (v >> (8 * r)) & 0xFF; }
+};
+template <>
+struct Off4<uint16_t> {
+  using type = uint2;  // four 16-bit offsets
+  static constexpr uint32_t none = 0xFFFF;
+  __device__ static int get(type v, int r) { return r < 2 ? (v.x >> (16 * r)) & 0xFFFF : (v.y >> (16 * (r - 2))) & 0xFFFF; }
+};
+
+// Scalar forms on the matrix cores.  A batch packs whole columns of the tile
+// whose contributors total at most 16; for its 16 contributor slots the local
+// columns are one product
+//   A[a, n] = sum_k Bt[a, k] W[k, n],  k = (q, d),
+//   Bt[a, (q,d)] = d_d phi_a(q)                     (reference table, registers)
+//   W[(q,d), n]  = (G_{s_n,q} grad phi_{b_n}(q))_d   (built per batch)
+// (mass form: k = q, Bt = phi_a(q), W = G phi_b(q)).  Batches own disjoint
+// columns, so any wave can take any batch.  Each lane holds 4 rows per
+// 16-row block of one slot and adds them into the FP64 column accumulators in
+// LDS; the adds step over the contributor rank t of the slot within its
+// column, so no two lanes of one step touch one column and every nonzero is
+// summed in contributor order (cells ascending).  Finished segments are
+// written once to HBM.  The tile's plan metadata (column segments,
+// contributors, row offsets, batches) is staged in LDS by the prologue,
+// overlapping the geometry phase, so the batch loop reads no global memory.
+struct ColInfo {
+  int seg0, len, kb, kc;  // segment start (0-based), length, first contributor (tile-relative), count
+};
+
+template <int D, int NV, int NS, int NQ, int FORM, typename CT, typename OffT>
+__global__ __launch_bounds__(kMfmaBlock) void matrix_tile_mfma_kernel(MatrixArgs args) {
+  constexpr int kBlock = kMfmaBlock;
+  constexpr int NG = NGeo<D, FORM>::value;
+  constexpr int KD = FORM == kLaplace ? D : 1;
+  constexpr int K = NQ * KD;
+  constexpr int KS = (K + 3) / 4;
+  constexpr int MB = (NS + 15) / 16;
+  constexpr int NWAVES = kBlock / 64;
+  constexpr int NT = NQ * KD * NS;  // reference table entries
+  using V4 = typename Acc4<CT>::type;
+  using O4 = Off4<OffT>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  CT *tab_s = reinterpret_cast<CT *>(lds_raw);  // Laplace: [NQ][D][NS] gradients, mass: [NQ][NS] values
+  CT *G_s = tab_s + NT;                          // [cells][NQ][NG]
+  ColInfo *col_s = reinterpret_cast<ColInfo *>(lds_raw + args.lds_col);
+  uint32_t *ctr_s = reinterpret_cast<uint32_t *>(lds_raw + args.lds_ctr);  // slot | b << 16
+  unsigned char *off_s = lds_raw + args.lds_off;                            // row offsets
+  int *bat_s = reinterpret_cast<int *>(lds_raw + args.lds_bat);            // batch -> first column
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nl = lane & 15, g = lane >> 4;
+  double *acc = reinterpret_cast<double *>(lds_raw + args.acc_offset) + (size_t)wave * args.acc_per_wave;
+
+  const int tile = blockIdx.x;
+  const int cell0 = args.tile_cell_ptr[tile];
+  const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
+  const int col0 = args.tile_col_ptr[tile];
+  const int ncol = args.tile_col_ptr[tile + 1] - col0;
+  const int ctr0 = args.col_ctr_ptr[col0];
+  const int nctr = args.col_ctr_ptr[col0 + ncol] - ctr0;
+  const int b0 = args.tile_batch_ptr[tile];
+  const int nbat = args.tile_batch_ptr[tile + 1] - b0;
+
+  // prologue: tables, plan metadata, accumulators
+  for (int i = threadIdx.x; i < NT; i += kBlock) tab_s[i] = (CT)(FORM == kLaplace ? args.gphi[i] : args.phi[i]);
+  for (int i = threadIdx.x; i < ncol; i += kBlock) {
+    const int pc = col0 + i;
+    const int j = args.col_id[pc];
+    const int c0 = args.colptr[j];
+    const int k0 = args.col_ctr_ptr[pc];
+    col_s[i] = ColInfo{c0 - 1, args.colptr[j + 1] - c0, k0 - ctr0, args.col_ctr_ptr[pc + 1] - k0};
+  }
+  for (int i = threadIdx.x; i < nctr; i += kBlock)
+    ctr_s[i] = (uint32_t)args.ctr_slot[ctr0 + i] | ((uint32_t)args.ctr_b[ctr0 + i] << 16);
+  {
+    const int row_bytes = args.off_stride * (int)sizeof(OffT);  // multiple of 16
+    const uint4 *src = reinterpret_cast<const uint4 *>(args.ctr_off + (int64_t)ctr0 * row_bytes);
+    uint4 *dst = reinterpret_cast<uint4 *>(off_s);
+    const int n16 = nctr * row_bytes / 16;
+    for (int i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
+  }
+  for (int i = threadIdx.x; i <= nbat; i += kBlock) bat_s[i] = args.batch_col[b0 + i] - col0;
+  for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
+
+  // phase A: FP64 geometry per (cell, point)
+  for (int idx = threadIdx.x; idx < ncell * NQ; idx += kBlock) {
+    const int s = idx / NQ, q = idx - s * NQ;
+    const int64_t cell = args.tile_cells[cell0 + s];
+    double Ji[D][D];
+    const double dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji) * args.w[q];
+    const double kdV = (args.coef ? args.coef[cell] : 1.0) * dV;
+    CT *G = G_s + (s * NQ + q) * NG;
+    if constexpr (FORM == kLaplace) {
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = i; j < D; j++) {
+          double gg = 0.0;
+#pragma unroll
+          for (int m = 0; m < D; m++) gg = fma(Ji[i][m], Ji[j][m], gg);
+          G[sym_index<D>(i, j)] = (CT)(kdV * gg);
+        }
+    } else {
+      G[0] = (CT)kdV;
+    }
+  }
+  __syncthreads();
+
+  CT afrag[MB][KS];
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const int a = mb * 16 + a_row<CT>(nl), k = 4 * ks + g;
+      afrag[mb][ks] = (a < NS && k < K) ? tab_s[k * NS + a] : (CT)0;  // [q][d][a] or [q][a]
+    }
+  const int ostride4 = args.off_stride >> 2;  // 4-offset words per contributor
+  const typename O4::type *offw = reinterpret_cast<const typename O4::type *>(off_s);
+  for (int bi = wave; bi < nbat; bi += NWAVES) {
+    const int cb = bat_s[bi];
+    const int m = bat_s[bi + 1] - cb;  // columns in the batch (<= 16)
+    const bool colv = nl < m;
+    const ColInfo ci_info = col_s[cb + (colv ? nl : 0)];
+    const int len = colv ? ci_info.len : 0;
+    const int kc = colv ? ci_info.kc : 0;
+    // exclusive prefixes of contributor counts and segment lengths
+    int slot0 = kc, abase = len;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const int t1 = __shfl_up(slot0, o, 16), t2 = __shfl_up(abase, o, 16);
+      if (nl >= o) {
+        slot0 += t1;
+        abase += t2;
+      }
+    }
+    const int nslots = __shfl(slot0, 15, 16);
+    slot0 -= kc;
+    abase -= len;
+    int tmax = kc;
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) tmax = max(tmax, __shfl_xor(tmax, o, 16));
+    // slot nl: its column ci and rank t_n within the column
+    int ci = 0;
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+      const int si = __shfl(slot0, i, 16);
+      if (i < m && si <= nl) ci = i;
+    }
+    const bool valid = nl < nslots;
+    const int t_n = nl - __shfl(slot0, ci, 16);
+    const int k = __shfl(ci_info.kb, ci, 16) + t_n;  // tile-relative contributor
+    const int my_acc = __shfl(abase, ci, 16);
+    int s = 0, b = 0;
+    typename O4::type o[MB];
+    if (valid) {
+      const uint32_t sb = ctr_s[k];
+      s = sb & 0xFFFF;
+      b = sb >> 16;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) o[mb] = offw[k * ostride4 + mb * 4 + g];
+    }
+    const CT *Gs = G_s + s * NQ * NG;
+    V4 c[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
+    // W without branches: padded k and empty slots read in-bounds entries
+    // and are zeroed by a select, so every LDS read can be issued early
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const int kk = 4 * ks + g;
+      const int kq = kk < K ? kk : K - 1;
+      CT w;
+      if constexpr (FORM == kLaplace) {
+        const int q = kq / D, d = kq - q * D;
+        const CT *G = Gs + q * NG;
+        w = G[sym_index<D>(d, 0)] * tab_s[(q * D) * NS + b];
+#pragma unroll
+        for (int e = 1; e < D; e++) w = fma(G[sym_index<D>(d, e)], tab_s[(q * D + e) * NS + b], w);
+      } else {
+        w = Gs[kq] * tab_s[kq * NS + b];
+      }
+      w = (valid && kk < K) ? w : (CT)0;
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(afrag[mb][ks], w, c[mb]);
+    }
+    // destinations: rows outside the row set or padding go to a private
+    // per-lane slot past the accumulators
+    int dst[MB][4];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int off = valid ? O4::get(o[mb], r) : (int)O4::none;
+        dst[mb][r] = off != (int)O4::none ? my_acc + off : args.acc_per_wave - 64 + lane;
+      }
+    for (int t = 0; t < tmax; t++) {
+      if (valid && t_n == t) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+      }
+    }
+    // write the finished segments and clear the accumulators
+    for (int i = 0; i < m; i++) {
+      const int li = __shfl(len, i, 16), ai = __shfl(abase, i, 16), si = __shfl(ci_info.seg0, i, 16);
+      for (int r = lane; r < li; r += 64) {
+        args.nzval[(int64_t)si + r] = acc[ai + r];
+        acc[ai + r] = 0.0;
+      }
+    }
   }
 }
 
@@ -288,6 +539,23 @@ struct Elem {
   static constexpr int d = D, nv = NV, ns = NS, nq = NQ, nc = NC;
 };
 
+template <typename E, int FORM, typename CT, typename OffT>
+static KernelInfo matrix_mfma_info() {
+  KernelInfo k;
+  k.fn = reinterpret_cast<const void *>(&matrix_tile_mfma_kernel<E::d, E::nv, E::ns, E::nq, FORM, CT, OffT>);
+  k.table_bytes = (size_t)(E::nq * (FORM == kLaplace ? E::d : 1) * E::ns) * sizeof(CT);
+  k.geo_bytes_per_cell = (size_t)E::nq * NGeo<E::d, FORM>::value * sizeof(CT);
+  k.cols_per_wave = 16;
+  k.mfma = true;
+  k.block = kMfmaBlock;
+  return k;
+}
+
+template <typename E, int FORM, typename CT>
+static KernelInfo matrix_mfma_info(int off_bytes) {
+  return off_bytes == 1 ? matrix_mfma_info<E, FORM, CT, uint8_t>() : matrix_mfma_info<E, FORM, CT, uint16_t>();
+}
+
 template <typename E, int FORM, typename CT>
 static KernelInfo matrix_info() {
   KernelInfo k;
@@ -314,12 +582,22 @@ static bool match(const ElemKey &key) {
 }
 
 template <typename E>
-static bool try_matrix(const ElemKey &key, int form, int precision, KernelInfo &out) {
+static bool try_matrix(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, KernelInfo &out) {
   if (!match<E>(key)) return false;
-  if (E::nc == 1) {
-    if (form == kLaplace) out = precision == 0 ? matrix_info<E, kLaplace, double>() : matrix_info<E, kLaplace, float>();
-    else if (form == kMass) out = precision == 0 ? matrix_info<E, kMass, double>() : matrix_info<E, kMass, float>();
-    else return false;
+  if constexpr (E::nc == 1) {
+    if (form != kLaplace && form != kMass) return false;
+    if (allow_mfma) {
+      if (form == kLaplace)
+        out = precision == 0 ? matrix_mfma_info<E, kLaplace, double>(off_bytes)
+                             : matrix_mfma_info<E, kLaplace, float>(off_bytes);
+      else
+        out = precision == 0 ? matrix_mfma_info<E, kMass, double>(off_bytes) : matrix_mfma_info<E, kMass, float>(off_bytes);
+    } else {
+      if (form == kLaplace)
+        out = precision == 0 ? matrix_info<E, kLaplace, double>() : matrix_info<E, kLaplace, float>();
+      else
+        out = precision == 0 ? matrix_info<E, kMass, double>() : matrix_info<E, kMass, float>();
+    }
   } else {
     if (form != kElasticity || E::nc != E::d) return false;
     out = precision == 0 ? matrix_info<E, kElasticity, double>() : matrix_info<E, kElasticity, float>();
@@ -345,10 +623,14 @@ using HexQ1 = Elem<3, 8, 8, 8, 1>;
 using HexQ2 = Elem<3, 8, 27, 27, 1>;
 using TetP2v = Elem<3, 4, 10, 14, 3>;
 
-bool find_matrix_kernel(const ElemKey &key, int form, int precision, KernelInfo &out) {
-  return try_matrix<QuadQ1>(key, form, precision, out) || try_matrix<QuadQ2>(key, form, precision, out) ||
-         try_matrix<QuadQ3>(key, form, precision, out) || try_matrix<HexQ1>(key, form, precision, out) ||
-         try_matrix<HexQ2>(key, form, precision, out) || try_matrix<TetP2v>(key, form, precision, out);
+bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma,
+                        KernelInfo &out) {
+  return try_matrix<QuadQ1>(key, form, precision, off_bytes, allow_mfma, out) ||
+         try_matrix<QuadQ2>(key, form, precision, off_bytes, allow_mfma, out) ||
+         try_matrix<QuadQ3>(key, form, precision, off_bytes, allow_mfma, out) ||
+         try_matrix<HexQ1>(key, form, precision, off_bytes, allow_mfma, out) ||
+         try_matrix<HexQ2>(key, form, precision, off_bytes, allow_mfma, out) ||
+         try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, out);
 }
 
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {

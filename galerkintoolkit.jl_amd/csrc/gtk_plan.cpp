@@ -151,8 +151,8 @@ std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, i
   return "";
 }
 
-std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int chunk, TilePlan &out,
-                        int nthreads) {
+std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int max_ctr, int chunk,
+                        TilePlan &out, int nthreads) {
   (void)nthreads;
   if (max_cells < 1 || max_cells > 65535) return "max_cells out of range";
   out = TilePlan();
@@ -208,12 +208,33 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
   // greedy tiles inside each chunk of `chunk` consecutive owner ranks
   std::vector<int32_t> mark(m.n_cells, -1);
   std::vector<int32_t> cur_cells, cur_cols;
+  int cur_ctr = 0;
   auto flush = [&]() {
     if (cur_cols.empty()) return;
     int tile = out.ntiles++;
     std::sort(cur_cells.begin(), cur_cells.end());
+    // columns with more contributors first: the kernels batch the t-th
+    // contributor of consecutive columns, so similar counts waste no slots
+    std::stable_sort(cur_cols.begin(), cur_cols.end(), [&](int32_t x, int32_t y) {
+      return inc.ptr[x + 1] - inc.ptr[x] > inc.ptr[y + 1] - inc.ptr[y];
+    });
     out.tile_cell_ptr.push_back((int32_t)out.tile_cells.size());
     out.tile_col_ptr.push_back((int32_t)out.col_id.size());
+    out.tile_batch_ptr.push_back((int32_t)out.batch_col.size());
+    {
+      int slots = kBatchSlots + 1, ncolb = 0;
+      for (size_t i = 0; i < cur_cols.size(); i++) {
+        const int cnt = (int)(inc.ptr[cur_cols[i] + 1] - inc.ptr[cur_cols[i]]);
+        if (slots + cnt > kBatchSlots) {
+          out.batch_col.push_back((int32_t)(out.col_id.size() + i));
+          slots = 0;
+          ncolb = 0;
+        }
+        slots += cnt;
+        ncolb++;
+        out.max_batch_cols = std::max(out.max_batch_cols, ncolb);
+      }
+    }
     for (size_t s = 0; s < cur_cells.size(); s++) mark[cur_cells[s]] = (int32_t)s;  // slot
     out.max_tile_cells = std::max<int>(out.max_tile_cells, (int)cur_cells.size());
     out.max_tile_cols = std::max<int>(out.max_tile_cols, (int)cur_cols.size());
@@ -231,8 +252,10 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
     }
     for (int32_t c : cur_cells) mark[c] = -1;
     (void)tile;
+    out.max_tile_ctr = std::max<int>(out.max_tile_ctr, cur_ctr);
     cur_cells.clear();
     cur_cols.clear();
+    cur_ctr = 0;
   };
   std::vector<int32_t> in_tile(m.n_cells, 0);
   int32_t stamp = 0;
@@ -247,12 +270,15 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
     }
     // cells this column would add
     int add = 0;
+    const int nctr_j = (int)(inc.ptr[j + 1] - inc.ptr[j]);
     for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++)
       if (in_tile[inc.cell[k]] != stamp) add++;
-    if (!cur_cols.empty() && ((int)cur_cells.size() + add > max_cells || (int)cur_cols.size() + 1 > max_cols)) {
+    if (!cur_cols.empty() && ((int)cur_cells.size() + add > max_cells || (int)cur_cols.size() + 1 > max_cols ||
+                              cur_ctr + nctr_j > max_ctr)) {
       flush();
       stamp++;
     }
+    cur_ctr += nctr_j;
     if (add > max_cells) return "a single DOF touches more cells than a tile may hold";
     for (int64_t k = inc.ptr[j]; k < inc.ptr[j + 1]; k++) {
       int32_t c = inc.cell[k];
@@ -267,7 +293,11 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
   out.tile_cell_ptr.push_back((int32_t)out.tile_cells.size());
   out.tile_col_ptr.push_back((int32_t)out.col_id.size());
   out.col_ctr_ptr.push_back((int32_t)out.ctr_slot.size());
+  out.tile_batch_ptr.push_back((int32_t)out.batch_col.size());
+  out.batch_col.push_back((int32_t)out.col_id.size());
   if ((int64_t)out.col_id.size() != nset) return "tile plan does not cover every DOF of the set";
+  for (int tl = 0; tl < out.ntiles; tl++)
+    out.max_tile_batches = std::max(out.max_tile_batches, out.tile_batch_ptr[tl + 1] - out.tile_batch_ptr[tl]);
   return "";
 }
 
@@ -275,11 +305,31 @@ std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t
   if (p.cols != t.set) return "tile plan set differs from the pattern columns";
   const int nd = m.dofs_per_cell;
   const int64_t ncol_plan = (int64_t)t.col_id.size();
-  out.off.assign(t.ctr_slot.size() * (size_t)nd, kNoRow);
   int max_seg = 0;
   for (int64_t j = 0; j < p.ncols; j++) max_seg = std::max(max_seg, p.colptr[j + 1] - p.colptr[j]);
   if (max_seg >= kNoRow) return "column segment too long for 16-bit offsets";
   out.max_seg = max_seg;
+  out.stride = (nd + 15) / 16 * 16;
+  out.bytes = max_seg < 0xFF ? 1 : 2;
+  out.data.assign(t.ctr_slot.size() * (size_t)out.stride * out.bytes, 0xFF);
+  // accumulator footprint of a group of kColGroup consecutive planned columns
+  int max_group = 0;
+  for (int tl = 0; tl < t.ntiles; tl++)
+    for (int32_t g0 = t.tile_col_ptr[tl]; g0 < t.tile_col_ptr[tl + 1]; g0 += kColGroup) {
+      int sum = 0;
+      for (int32_t i = g0; i < std::min(g0 + kColGroup, t.tile_col_ptr[tl + 1]); i++)
+        sum += p.colptr[t.col_id[i] + 1] - p.colptr[t.col_id[i]];
+      max_group = std::max(max_group, sum);
+    }
+  out.max_group_acc = max_group;
+  int max_batch = 0;
+  for (size_t b = 0; b + 1 < t.batch_col.size(); b++) {
+    int sum = 0;
+    for (int32_t i = t.batch_col[b]; i < t.batch_col[b + 1]; i++)
+      sum += p.colptr[t.col_id[i] + 1] - p.colptr[t.col_id[i]];
+    max_batch = std::max(max_batch, sum);
+  }
+  out.max_batch_acc = max_batch;
   // tile of each planned column (to read tile cells)
   std::vector<int32_t> col_tile(ncol_plan);
   for (int tl = 0; tl < t.ntiles; tl++)
@@ -298,7 +348,9 @@ std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t
           int32_t r = map_dof(dofs[a], p.rows);
           if (!r) continue;
           const int32_t *it = std::lower_bound(seg, seg + len, r);
-          out.off[(size_t)k * nd + a] = (uint16_t)(it - seg);
+          const size_t pos = (size_t)k * out.stride + a;
+          if (out.bytes == 1) out.data[pos] = (uint8_t)(it - seg);
+          else reinterpret_cast<uint16_t *>(out.data.data())[pos] = (uint16_t)(it - seg);
         }
       }
     }

@@ -8,6 +8,8 @@
 namespace gtk {
 
 constexpr uint16_t kNoRow = 0xFFFF;  // contributor row not in the row set
+constexpr int kColGroup = 16;        // columns batched together by the MFMA kernels
+constexpr int kBatchSlots = 16;      // contributors per matrix-core batch (16 = MFMA N)
 
 // CSC pattern of one (rows, cols) block, 1-based like SparseMatrixCSC.
 struct Pattern {
@@ -28,6 +30,7 @@ struct TilePlan {
   int64_t nset = 0;             // number of dofs in S
   int ntiles = 0;
   int max_tile_cells = 0, max_tile_cols = 0, max_ctr = 0;
+  int max_tile_ctr = 0, max_tile_batches = 0;
   std::vector<int32_t> tile_cell_ptr;  // ntiles+1
   std::vector<int32_t> tile_cells;     // 0-based cell ids, ascending within a tile
   std::vector<int32_t> tile_col_ptr;   // ntiles+1, into col_*
@@ -35,12 +38,25 @@ struct TilePlan {
   std::vector<int32_t> col_ctr_ptr;    // size ncolumns+1, into ctr_*
   std::vector<uint16_t> ctr_slot;      // tile-local cell slot of each contributor
   std::vector<uint16_t> ctr_b;         // local dof b of the contributor in its cell
+  // batches: runs of consecutive planned columns of one tile whose
+  // contributors total at most kBatchSlots (one column per batch when it has
+  // more); batch i covers planned columns [batch_col[i], batch_col[i+1])
+  std::vector<int32_t> tile_batch_ptr;  // ntiles+1
+  std::vector<int32_t> batch_col;       // nbatches+1
+  int max_batch_cols = 0;
 };
 
-// Row offsets of each contributor in its column segment (matrix blocks).
+// Row offsets of each contributor in its column segment (matrix blocks):
+// `stride` entries per contributor (ndofs_cell rounded up to 16), one byte
+// each when every segment is shorter than 255 rows, else two; the all-ones
+// value marks a row outside the row set (or padding).
 struct OffsetPlan {
-  std::vector<uint16_t> off;  // per contributor, ndofs_cell entries, kNoRow if excluded
+  std::vector<uint8_t> data;
+  int stride = 0;
+  int bytes = 1;
   int max_seg = 0;
+  int max_group_acc = 0;  // largest sum of segment lengths over kColGroup consecutive planned columns
+  int max_batch_acc = 0;  // largest sum of segment lengths over the columns of one batch
 };
 
 struct MeshView {
@@ -56,10 +72,10 @@ struct MeshView {
 
 // Builds the CSC pattern for (rows, cols); returns an error string on failure.
 std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, int nthreads);
-// Builds the tile plan over set S with at most max_cells cells per tile and
-// at most max_cols owned DOFs per tile.
-std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int chunk, TilePlan &out,
-                        int nthreads);
+// Builds the tile plan over set S with at most max_cells cells, max_cols
+// owned DOFs and max_ctr contributors per tile.
+std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int max_ctr, int chunk,
+                        TilePlan &out, int nthreads);
 // Row offsets of every contributor of a tile plan over the columns of `p`.
 std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads);
 

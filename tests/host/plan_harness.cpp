@@ -22,7 +22,7 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
   gtk::TilePlan t;
   gtk::OffsetPlan o;
   std::string e = gtk::build_pattern(m, rows, cols, p, 4);
-  if (e.empty()) e = gtk::build_tiles(m, cols, max_cells, 1 << 20, chunk, t, 4);
+  if (e.empty()) e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, chunk, t, 4);
   if (e.empty()) e = gtk::build_offsets(m, p, t, o, 4);
   if (!e.empty()) { std::strncpy(err, e.c_str(), errlen - 1); return -1; }
   *nnz_out = p.nnz;
@@ -30,6 +30,24 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
   if (p.nnz > cap) return 1;
   std::memcpy(colptr, p.colptr.data(), sizeof(int32_t) * p.colptr.size());
   std::memcpy(rowval, p.rowval.data(), sizeof(int32_t) * p.nnz);
+  // batches: cover each tile's planned columns in order, at most
+  // kBatchSlots contributors unless a single column has more
+  if ((int)t.tile_batch_ptr.size() != t.ntiles + 1) { std::strncpy(err, "tile_batch_ptr size", errlen - 1); return -1; }
+  for (int tl = 0; tl < t.ntiles; tl++) {
+    int b0 = t.tile_batch_ptr[tl], b1 = t.tile_batch_ptr[tl + 1];
+    if (b1 <= b0 || t.batch_col[b0] != t.tile_col_ptr[tl] || t.batch_col[b1] != t.tile_col_ptr[tl + 1]) {
+      std::strncpy(err, "batches do not tile the columns", errlen - 1);
+      return -1;
+    }
+    for (int b = b0; b < b1; b++) {
+      int c0 = t.batch_col[b], c1 = t.batch_col[b + 1];
+      int slots = t.col_ctr_ptr[c1] - t.col_ctr_ptr[c0];
+      if (c1 <= c0 || c1 - c0 > 16 || (slots > gtk::kBatchSlots && c1 - c0 != 1)) {
+        std::strncpy(err, "batch exceeds its slots", errlen - 1);
+        return -1;
+      }
+    }
+  }
   for (int64_t i = 0; i < p.nnz; i++) nzval[i] = -1.0;  // every entry must be written
   for (int tl = 0; tl < t.ntiles; tl++) {
     const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[tl];
@@ -44,8 +62,9 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
         if (s >= ncell) { std::strncpy(err, "slot out of range", errlen - 1); return -1; }
         int64_t c = cells[s];
         for (int a = 0; a < nd; a++) {
-          uint16_t off = o.off[(size_t)k * nd + a];
-          if (off == gtk::kNoRow) continue;
+          const size_t pos = (size_t)k * o.stride + a;
+          int off = o.bytes == 1 ? o.data[pos] : reinterpret_cast<const uint16_t *>(o.data.data())[pos];
+          if (off == (o.bytes == 1 ? 0xFF : 0xFFFF)) continue;
           if (off >= len) { std::strncpy(err, "offset out of range", errlen - 1); return -1; }
           acc[off] += 1.0 + c * 7.0 + a * 131.0 + b * 977.0;
         }
