@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -40,12 +41,24 @@ struct DevBuf {
   }
 };
 
+// Work plan of one (form, precision) over a block: the tiles, and either
+// the batch records of the matrix-core kernel or the row offsets of the
+// VALU kernel, with the kernel and its LDS layout.
+struct ExecPlan {
+  gtk::TilePlan tiles;
+  gtk::BatchPlan bat;
+  gtk::OffsetPlan offs;
+  gtk::KernelInfo k;
+  DevBuf tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, ctr_off, tile_batch_ptr, rec;
+  size_t lds = 0;
+  int lds_geo = 0, acc_offset = 0, acc_per_wave = 0, stage_cells = 0;
+};
+
 struct MatrixPlan {
   gtk::Pattern pat;
-  gtk::TilePlan tiles;
-  gtk::OffsetPlan offs;
-  DevBuf colptr, tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, ctr_off, nzval;
-  DevBuf tile_batch_ptr, batch_col;
+  int max_seg = 0;
+  DevBuf colptr, nzval;
+  std::map<int, std::unique_ptr<ExecPlan>> exec;  // key: form * 2 + precision
   bool valid = false;
 };
 
@@ -82,6 +95,7 @@ struct gtk_ctx {
   int ncoef = 0;
   // device copies
   DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef;
+  int waves = 8;  // wavefronts per workgroup of the matrix-core kernels
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
   std::map<int, std::unique_ptr<VectorPlan>> vplans;
   bool has_mesh = false, has_space = false, has_ref = false;
@@ -156,49 +170,40 @@ static gtk::ElemKey elem_key(const gtk_ctx *ctx) {
   return gtk::ElemKey{ctx->dim, ctx->nv, ctx->nshape, ctx->nq, ctx->ncomp};
 }
 
-// Tile limits: the largest cell count whose staged geometry fits the LDS
-// left by the reference tables, the FP64 column accumulators and (matrix-core
-// kernels) the staged plan metadata, for the most demanding form/precision
-// of this element (one plan serves all).
-static constexpr size_t kLdsBudget = 150 * 1024;
-static constexpr size_t kMetaBudget = 16 * 1024;
+// LDS available to one workgroup (gfx950: 160 KiB per CU), less headroom.
+static constexpr size_t kLdsBudget = 156 * 1024;
 
-static void tile_limits(const gtk_ctx *ctx, int max_seg, int &max_cells, int &chunk, int &max_ctr) {
-  gtk::ElemKey key = elem_key(ctx);
-  size_t geo = 0, tables = 0, acc = 0, meta = 0;
-  max_ctr = 1 << 20;
-  for (int form = 0; form < 3; form++) {
-    gtk::KernelInfo k;
-    if (!gtk::find_matrix_kernel(key, form, GTK_FP64, 2, true, k)) continue;
-    geo = std::max(geo, k.geo_bytes_per_cell);
-    tables = std::max(tables, k.table_bytes);
-    const size_t waves = (size_t)k.block / 64;
-    size_t a = k.mfma ? waves * gtk::kBatchSlots * ctx->dofs_per_cell * sizeof(double)
-                      : waves * k.cols_per_wave * std::max(1, max_seg) * sizeof(double);
-    acc = std::max(acc, a);
-    if (k.mfma) {
-      meta = kMetaBudget + waves * 256;
-      const int stride = (ctx->dofs_per_cell + 15) / 16 * 16;
-      const int obytes = max_seg < 0xFF ? 1 : 2;
-      max_ctr = (int)((kMetaBudget - 2048) / (4 + (size_t)stride * obytes));
-    }
-  }
-  if (!geo) {  // no matrix kernel: vector-only use of the plan
-    geo = (size_t)ctx->nq * 8 * ctx->ncomp;
-    tables = (size_t)ctx->nq * ctx->nshape * 8;
-  }
-  const size_t fixed = tables + acc + meta + 1024;
-  const size_t avail = kLdsBudget > fixed ? kLdsBudget - fixed : 8 * geo;
-  max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, avail / geo));
-  // chunk: cells whose owned DOFs form a tile; about (c^(1/D)+1)^D cells are
-  // then needed (more around simplices)
-  chunk = 1;
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// chunk: cells whose owned DOFs form a tile; about (c^(1/D)+1)^D cells are
+// then needed (more around simplices)
+static int tile_chunk(const gtk_ctx *ctx, int max_cells) {
+  int chunk = 1;
   for (int c = 2; c <= 4096; c *= 2) {
     double side = std::pow((double)c, 1.0 / ctx->dim);
     double need = std::pow(side + 1.0, ctx->dim);
     if (ctx->nv == ctx->dim + 1) need *= 2.5;
     if (need <= max_cells) chunk = c;
   }
+  return chunk;
+}
+
+// Largest cell count per tile whose staged geometry fits the LDS left by
+// the tables and the accumulators of kernel k.
+static int tile_cells_for(const gtk::KernelInfo &k, size_t acc_bytes) {
+  const size_t fixed = align16(k.table_bytes) + acc_bytes + 1024;
+  const size_t geo = std::max<size_t>(1, k.geo_bytes_per_cell);
+  const size_t avail = kLdsBudget > fixed ? kLdsBudget - fixed : 8 * geo;
+  return (int)std::max<size_t>(8, std::min<size_t>(4095, avail / geo));
+}
+
+// Tile limits of the vector kernel (no accumulators).
+static void vector_tile_limits(const gtk_ctx *ctx, int &max_cells, int &chunk) {
+  const size_t geo = (size_t)ctx->nq * 8 * ctx->ncomp;
+  const size_t tables = (size_t)ctx->nq * ctx->nshape * 8;
+  const size_t avail = kLdsBudget > tables + 1024 ? kLdsBudget - tables - 1024 : 8 * geo;
+  max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, avail / geo));
+  chunk = tile_chunk(ctx, max_cells);
 }
 
 extern "C" {
@@ -239,6 +244,9 @@ int gtk_create(int device, gtk_ctx **out) {
     return GTK_ENOMEM;
   }
   ctx->device = device;
+  if (const char *w = std::getenv("GTK_WAVES")) {  // matrix-core workgroup shape (tuning)
+    if (std::strcmp(w, "4") == 0) ctx->waves = 4;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     g_create_err = "stream/event creation failed";
@@ -352,6 +360,7 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
     for (int s = 0; s < nshape; s++)
       for (int d = 0; d < D; d++) ctx->gphi_t[((size_t)q * D + d) * nshape + s] = grad_phi[((size_t)q * nshape + s) * D + d];
   ctx->gN.assign(grad_geo, grad_geo + (size_t)nq * ctx->nv * D);
+  for (auto &mp : ctx->mplans) mp.second->exec.clear();  // kernels depend on the tables
   int rc;
   if ((rc = upload(ctx, ctx->d_w, ctx->w.data(), ctx->w.size()))) return rc;
   if ((rc = upload(ctx, ctx->d_phi, ctx->phi.data(), ctx->phi.size()))) return rc;
@@ -404,24 +413,10 @@ int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *n
   int nt = nthreads_host();
   std::string e = gtk::build_pattern(m, rows, cols, plan->pat, nt);
   if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
-  int max_seg = 0;
   for (int64_t j = 0; j < plan->pat.ncols; j++)
-    max_seg = std::max(max_seg, plan->pat.colptr[j + 1] - plan->pat.colptr[j]);
-  int max_cells, chunk, max_ctr;
-  tile_limits(ctx, max_seg, max_cells, chunk, max_ctr);
-  e = gtk::build_tiles(m, cols, max_cells, 1 << 20, max_ctr, chunk, plan->tiles, nt);
-  if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
-  e = gtk::build_offsets(m, plan->pat, plan->tiles, plan->offs, nt);
-  if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    plan->max_seg = std::max(plan->max_seg, plan->pat.colptr[j + 1] - plan->pat.colptr[j]);
   int rc;
   if ((rc = upload(ctx, plan->colptr, plan->pat.colptr.data(), plan->pat.colptr.size()))) return rc;
-  if ((rc = upload_tiles(ctx, plan->tiles, plan->tile_cell_ptr, plan->tile_cells, plan->tile_col_ptr, plan->col_id,
-                         plan->col_ctr_ptr, plan->ctr_slot, plan->ctr_b)))
-    return rc;
-  if ((rc = upload(ctx, plan->ctr_off, plan->offs.data.data(), plan->offs.data.size()))) return rc;
-  if ((rc = upload(ctx, plan->tile_batch_ptr, plan->tiles.tile_batch_ptr.data(), plan->tiles.tile_batch_ptr.size())))
-    return rc;
-  if ((rc = upload(ctx, plan->batch_col, plan->tiles.batch_col.data(), plan->tiles.batch_col.size()))) return rc;
   if ((rc = alloc(ctx, plan->nzval, (size_t)std::max<int64_t>(1, plan->pat.nnz) * sizeof(double)))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   plan->valid = true;
@@ -441,40 +436,92 @@ int gtk_get_pattern(gtk_ctx *ctx, int rows, int cols, int32_t *colptr, int32_t *
   return GTK_OK;
 }
 
+// Work plan of (form, precision) over a block, built on first use: the
+// matrix-core kernel when the space is scalar and no column has more than
+// kBatchSlots contributors, the VALU kernel otherwise.
+static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, ExecPlan *&out) {
+  const int key = form * 2 + precision;
+  auto it = plan.exec.find(key);
+  if (it != plan.exec.end()) {
+    out = it->second.get();
+    return GTK_OK;
+  }
+  if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
+  auto ep = std::make_unique<ExecPlan>();
+  const gtk::MeshView m = view(ctx);
+  const int nt = nthreads_host();
+  const int cols = plan.pat.cols;
+  const int off_bytes = plan.max_seg < 0xFF ? 1 : 2;
+  const gtk::ElemKey key_e = elem_key(ctx);
+  const int nd = ctx->dofs_per_cell;
+  std::string e;
+  bool mfma = gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, ctx->waves, ep->k) && ep->k.mfma;
+  if (mfma) {
+    // accumulator bound: a batch adds at most nd rows per slot
+    const size_t acc_bound = (size_t)ctx->waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
+    const int max_cells = tile_cells_for(ep->k, acc_bound);
+    e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    if (ep->tiles.max_ctr > gtk::kBatchSlots) mfma = false;
+  }
+  if (mfma) {
+    e = gtk::build_batches(m, plan.pat, ep->tiles, ep->bat, nt);
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    ep->lds_geo = (int)align16(ep->k.table_bytes);
+    ep->acc_offset = (int)align16(ep->lds_geo + (size_t)ep->tiles.max_tile_cells * ep->k.geo_bytes_per_cell);
+    ep->acc_per_wave = (ep->bat.max_batch_acc + 64 + 1) & ~1;
+    const size_t acc_bytes = (size_t)ctx->waves * ep->acc_per_wave * sizeof(double);
+    ep->stage_cells = (int)std::max<size_t>(1, acc_bytes / ((size_t)ctx->nv * ctx->dim * sizeof(double)));
+    ep->lds = ep->acc_offset + acc_bytes;
+  } else {
+    if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, ctx->waves, ep->k))
+      FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
+    const size_t waves = (size_t)ep->k.block / 64;
+    const size_t acc_bytes = waves * ep->k.cols_per_wave * std::max(1, plan.max_seg) * sizeof(double);
+    const int max_cells = tile_cells_for(ep->k, acc_bytes);
+    e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    e = gtk::build_offsets(m, plan.pat, ep->tiles, ep->offs, nt);
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    ep->acc_offset = (int)align16(ep->k.table_bytes + (size_t)ep->tiles.max_tile_cells * ep->k.geo_bytes_per_cell);
+    ep->acc_per_wave = ep->k.cols_per_wave * std::max(1, plan.max_seg);
+    ep->lds = ep->acc_offset + acc_bytes;
+  }
+  if (ep->lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
+  const gtk::TilePlan &t = ep->tiles;
+  int rc;
+  if ((rc = upload(ctx, ep->tile_cell_ptr, t.tile_cell_ptr.data(), t.tile_cell_ptr.size()))) return rc;
+  if ((rc = upload(ctx, ep->tile_cells, t.tile_cells.data(), t.tile_cells.size()))) return rc;
+  if (mfma) {
+    if ((rc = upload(ctx, ep->tile_batch_ptr, t.tile_batch_ptr.data(), t.tile_batch_ptr.size()))) return rc;
+    if ((rc = upload(ctx, ep->rec, ep->bat.rec.data(), ep->bat.rec.size()))) return rc;
+  } else {
+    if ((rc = upload(ctx, ep->tile_col_ptr, t.tile_col_ptr.data(), t.tile_col_ptr.size()))) return rc;
+    if ((rc = upload(ctx, ep->col_id, t.col_id.data(), t.col_id.size()))) return rc;
+    if ((rc = upload(ctx, ep->col_ctr_ptr, t.col_ctr_ptr.data(), t.col_ctr_ptr.size()))) return rc;
+    if ((rc = upload(ctx, ep->ctr_slot, t.ctr_slot.data(), t.ctr_slot.size()))) return rc;
+    if ((rc = upload(ctx, ep->ctr_b, t.ctr_b.data(), t.ctr_b.size()))) return rc;
+    if ((rc = upload(ctx, ep->ctr_off, ep->offs.data.data(), ep->offs.data.size()))) return rc;
+  }
+  if (ep->lds > 64 * 1024)
+    HIPCHK(ctx, hipFuncSetAttribute(ep->k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ep->lds));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  out = ep.get();
+  plan.exec[key] = std::move(ep);
+  return GTK_OK;
+}
+
 static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision) {
-  gtk::KernelInfo k;
   if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  const bool batches_fit = plan.tiles.max_ctr <= gtk::kBatchSlots;
-  if (!gtk::find_matrix_kernel(elem_key(ctx), form, precision, plan.offs.bytes, batches_fit, k))
-    FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
+  if (form < GTK_LAPLACE || form > GTK_ELASTICITY) FAIL(ctx, GTK_EINVAL, "unknown form");
   if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
   if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
-  if (plan.tiles.ntiles == 0) return GTK_OK;
-  auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-  const size_t waves = (size_t)k.block / 64;
-  size_t lds = align16(k.table_bytes + (size_t)plan.tiles.max_tile_cells * k.geo_bytes_per_cell);
-  int lds_col = 0, lds_ctr = 0, lds_off = 0, lds_bat = 0, lds_wave = 0;
-  if (k.mfma) {
-    lds_col = (int)lds;
-    lds += align16((size_t)plan.tiles.max_tile_cols * 16);
-    lds_ctr = (int)lds;
-    lds += align16((size_t)plan.tiles.max_tile_ctr * 4);
-    lds_off = (int)lds;
-    lds += align16((size_t)plan.tiles.max_tile_ctr * plan.offs.stride * plan.offs.bytes);
-    lds_bat = (int)lds;
-    lds += align16((size_t)(plan.tiles.max_tile_batches + 1) * 4);
-    lds_wave = (int)lds;
-    lds += waves * 256;
-  }
-  size_t acc_off = lds;
-  // matrix-core kernels: batch accumulators plus 64 private sink slots
-  const int acc_per_wave =
-      k.mfma ? plan.offs.max_batch_acc + 64 : k.cols_per_wave * std::max(1, plan.offs.max_seg);
-  lds += waves * acc_per_wave * sizeof(double);
-  if (lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
-  if (lds > 64 * 1024) HIPCHK(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  gtk::MatrixArgs a;
+  ExecPlan *ep = nullptr;
+  int rc = get_exec(ctx, plan, form, precision, ep);
+  if (rc) return rc;
+  if (ep->tiles.ntiles == 0) return GTK_OK;
+  gtk::MatrixArgs a = {};
   a.xyz = ctx->d_xyz.as<double>();
   a.cell_nodes = ctx->d_cell_nodes.as<int32_t>();
   a.coef = ctx->ncoef ? ctx->d_coef.as<double>() : nullptr;
@@ -482,30 +529,33 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   a.phi = ctx->d_phi.as<double>();
   a.gphi = ctx->d_gphi.as<double>();
   a.gN = ctx->d_gN.as<double>();
-  a.tile_cell_ptr = plan.tile_cell_ptr.as<int32_t>();
-  a.tile_cells = plan.tile_cells.as<int32_t>();
-  a.tile_col_ptr = plan.tile_col_ptr.as<int32_t>();
-  a.col_id = plan.col_id.as<int32_t>();
-  a.col_ctr_ptr = plan.col_ctr_ptr.as<int32_t>();
-  a.tile_batch_ptr = plan.tile_batch_ptr.as<int32_t>();
-  a.batch_col = plan.batch_col.as<int32_t>();
-  a.ctr_slot = plan.ctr_slot.as<uint16_t>();
-  a.ctr_b = plan.ctr_b.as<uint16_t>();
-  a.ctr_off = plan.ctr_off.as<uint8_t>();
-  a.off_stride = plan.offs.stride;
-  a.off_bytes = plan.offs.bytes;
+  a.tile_cell_ptr = ep->tile_cell_ptr.as<int32_t>();
+  a.tile_cells = ep->tile_cells.as<int32_t>();
+  a.tile_col_ptr = ep->tile_col_ptr.as<int32_t>();
+  a.col_id = ep->col_id.as<int32_t>();
+  a.col_ctr_ptr = ep->col_ctr_ptr.as<int32_t>();
+  a.tile_batch_ptr = ep->tile_batch_ptr.as<int32_t>();
+  a.ctr_slot = ep->ctr_slot.as<uint16_t>();
+  a.ctr_b = ep->ctr_b.as<uint16_t>();
+  a.ctr_off = ep->ctr_off.as<uint8_t>();
+  a.rec = ep->rec.as<unsigned char>();
+  a.rec_bytes = ep->bat.rec_bytes;
   a.colptr = plan.colptr.as<int32_t>();
   a.nzval = plan.nzval.as<double>();
-  a.max_seg = std::max(1, plan.offs.max_seg);
-  a.acc_offset = (int)acc_off;
-  a.acc_per_wave = acc_per_wave;
-  a.lds_col = lds_col;
-  a.lds_ctr = lds_ctr;
-  a.lds_off = lds_off;
-  a.lds_bat = lds_bat;
-  a.lds_wave = lds_wave;
+  if (ep->k.mfma) {
+    a.off_stride = ep->bat.off_stride;
+    a.off_bytes = ep->bat.off_bytes;
+  } else {
+    a.off_stride = ep->offs.stride;
+    a.off_bytes = ep->offs.bytes;
+  }
+  a.max_seg = std::max(1, plan.max_seg);
+  a.lds_geo = ep->lds_geo;
+  a.acc_offset = ep->acc_offset;
+  a.acc_per_wave = ep->acc_per_wave;
+  a.stage_cells = ep->stage_cells;
   void *args[] = {&a};
-  HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(plan.tiles.ntiles), dim3(k.block), args, lds, ctx->stream));
+  HIPCHK(ctx, hipLaunchKernel(ep->k.fn, dim3(ep->tiles.ntiles), dim3(ep->k.block), args, ep->lds, ctx->stream));
   return GTK_OK;
 }
 
@@ -514,14 +564,22 @@ static MatrixPlan *find_plan(gtk_ctx *ctx, int rows, int cols) {
   return it == ctx->mplans.end() ? nullptr : it->second.get();
 }
 
-static void fill_stats(const gtk_ctx *ctx, const MatrixPlan &p, gtk_stats *st) {
+static void fill_stats(const gtk_ctx *ctx, const MatrixPlan &p, int form, int precision, gtk_stats *st) {
   st->cells = ctx->n_cells;
   st->nnz = p.pat.nnz;
   // algorithmic bytes: coordinates, cell->node map, coefficients, the plan's
-  // index streams read once, and the values written once
+  // streams read once (batch records, or offsets and contributor lists),
+  // and the values written once
+  int64_t plan_bytes = 0;
+  auto it = p.exec.find(form * 2 + precision);
+  if (it != p.exec.end()) {
+    const ExecPlan &e = *it->second;
+    plan_bytes = e.k.mfma ? (int64_t)e.bat.rec.size()
+                          : (int64_t)e.offs.data.size() + (int64_t)e.tiles.ctr_slot.size() * 4 +
+                                (int64_t)e.tiles.col_id.size() * 8;
+  }
   int64_t b = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 +
-              (int64_t)ctx->n_cells * ctx->ncoef * 8 + (int64_t)p.offs.data.size() +
-              (int64_t)p.tiles.ctr_slot.size() * 4 + (int64_t)p.tiles.col_id.size() * 8 + p.pat.nnz * 8;
+              (int64_t)ctx->n_cells * ctx->ncoef * 8 + plan_bytes + p.pat.nnz * 8;
   st->bytes_alg = b;
   const int64_t D = ctx->dim, nd = ctx->dofs_per_cell;
   st->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * D * nd * nd + 2 * D * D * nd + 2 * ctx->nv * D * D);
@@ -559,7 +617,7 @@ int gtk_assemble_matrix(gtk_ctx *ctx, int form, int precision, int rows, int col
     float ms = 0;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     stats->kernel_ms = ms;
-    fill_stats(ctx, *p, stats);
+    fill_stats(ctx, *p, form, precision, stats);
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return GTK_OK;
@@ -618,8 +676,8 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
   auto &slot = ctx->vplans[rows];
   if (!slot) {
     auto vp = std::make_unique<VectorPlan>();
-    int max_cells, chunk, max_ctr;
-    tile_limits(ctx, 0, max_cells, chunk, max_ctr);
+    int max_cells, chunk;
+    vector_tile_limits(ctx, max_cells, chunk);
     std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, 1 << 20, chunk, vp->tiles, nthreads_host());
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     int rc = upload_tiles(ctx, vp->tiles, vp->tile_cell_ptr, vp->tile_cells, vp->tile_col_ptr, vp->col_id,

@@ -7,7 +7,6 @@
 namespace gtk {
 
 constexpr int kBlock = 256;            // threads per workgroup (4 waves of 64)
-constexpr int kMfmaBlock = 512;        // matrix-core kernels: 8 waves, two per SIMD
 
 enum { kLaplace = 0, kMass = 1, kElasticity = 2 };
 enum { kFConst = 0, kFCell = 1, kFNodal = 2 };
@@ -18,19 +17,19 @@ struct MatrixArgs {
   const double *coef;
   const double *w, *phi, *gphi, *gN;
   const int32_t *tile_cell_ptr, *tile_cells, *tile_col_ptr, *col_id, *col_ctr_ptr;
-  const int32_t *tile_batch_ptr, *batch_col;
+  const int32_t *tile_batch_ptr;
   const uint16_t *ctr_slot, *ctr_b;
-  const uint8_t *ctr_off;  // off_stride entries per contributor, off_bytes each
+  const uint8_t *ctr_off;  // VALU kernel: off_stride entries per contributor, off_bytes each
+  const unsigned char *rec;  // matrix-core kernels: batch records (BatchPlan)
+  int rec_bytes;
   const int32_t *colptr;
   double *nzval;
   int off_stride, off_bytes;
   int max_seg;
+  int lds_geo;       // byte offset of the staged geometry factors in LDS
   int acc_offset;    // byte offset of the accumulators in LDS
-  int acc_per_wave;  // FP64 accumulator slots per wave (MFMA kernels)
-  // byte offsets in LDS of the staged tile metadata (MFMA kernels): column
-  // info (16 B each), contributors (4 B), row offsets, batch table, per-wave
-  // batch columns (16 x 16 B per wave)
-  int lds_col, lds_ctr, lds_off, lds_bat, lds_wave;
+  int acc_per_wave;  // FP64 accumulator slots per wave (matrix-core kernels)
+  int stage_cells;   // cells whose coordinates fit the accumulator space (matrix-core kernels)
 };
 
 struct VectorArgs {
@@ -54,13 +53,14 @@ struct KernelInfo {
   size_t table_bytes = 0;         // LDS bytes of the reference tables
   size_t geo_bytes_per_cell = 0;  // LDS bytes of staged geometry per cell
   int cols_per_wave = 1;
-  bool mfma = false;  // batch accumulators and LDS-staged plan metadata
+  bool mfma = false;  // matrix-core kernel driven by batch records
   int block = kBlock;
 };
 
 // allow_mfma: the plan's columns fit the matrix-core batches (at most
 // kBatchSlots contributors per column); otherwise the VALU kernel is used.
-bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma,
+// waves: wavefronts per workgroup of the matrix-core kernels (4 or 8).
+bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out);
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);
 

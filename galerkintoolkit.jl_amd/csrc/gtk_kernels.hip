@@ -263,188 +263,256 @@ template <>
 struct Off4<uint8_t> {
   using type = uint32_t;  // four 8-bit offsets
   static constexpr uint32_t none = 0xFF;
+  __device__ static type all_none() { return 0xFFFFFFFFu; }
   __device__ static int get(type v, int r) { return (v >> (8 * r)) & 0xFF; }
 };
 template <>
 struct Off4<uint16_t> {
   using type = uint2;  // four 16-bit offsets
   static constexpr uint32_t none = 0xFFFF;
+  __device__ static type all_none() { return make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu); }
   __device__ static int get(type v, int r) { return r < 2 ? (v.x >> (16 * r)) & 0xFFFF : (v.y >> (16 * (r - 2))) & 0xFFFF; }
 };
 
-// Scalar forms on the matrix cores.  A batch packs whole columns of the tile
-// whose contributors total at most 16; for its 16 contributor slots the local
-// columns are one product
-//   A[a, n] = sum_k Bt[a, k] W[k, n],  k = (q, d),
-//   Bt[a, (q,d)] = d_d phi_a(q)                     (reference table, registers)
-//   W[(q,d), n]  = (G_{s_n,q} grad phi_{b_n}(q))_d   (built per batch)
-// (mass form: k = q, Bt = phi_a(q), W = G phi_b(q)).  Batches own disjoint
-// columns, so any wave can take any batch.  Each lane holds 4 rows per
-// 16-row block of one slot and adds them into the FP64 column accumulators in
-// LDS; the adds step over the contributor rank t of the slot within its
-// column, so no two lanes of one step touch one column and every nonzero is
-// summed in contributor order (cells ascending).  Finished segments are
-// written once to HBM.  The tile's plan metadata (column segments,
-// contributors, row offsets, batches) is staged in LDS by the prologue,
-// overlapping the geometry phase, so the batch loop reads no global memory.
-struct ColInfo {
-  int seg0, len, kb, kc;  // segment start (0-based), length, first contributor (tile-relative), count
+// N consecutive CT values from LDS with the widest loads the alignment
+// `ALIGN` (bytes) allows.
+template <typename CT, int N, int ALIGN>
+__device__ inline void lds_load(const CT *__restrict__ p, CT (&out)[N]) {
+  constexpr int BYTES = N * (int)sizeof(CT);
+  if constexpr (ALIGN % 16 == 0 && BYTES % 16 == 0) {
+    using V = __attribute__((ext_vector_type(16 / sizeof(CT)))) CT;
+    constexpr int L = 16 / sizeof(CT);
+#pragma unroll
+    for (int i = 0; i < N / L; i++) {
+      V v = reinterpret_cast<const V *>(p)[i];
+#pragma unroll
+      for (int j = 0; j < L; j++) out[i * L + j] = v[j];
+    }
+  } else if constexpr (ALIGN % 8 == 0 && BYTES % 8 == 0 && sizeof(CT) <= 8) {
+    using V = __attribute__((ext_vector_type(8 / sizeof(CT)))) CT;
+    constexpr int L = 8 / sizeof(CT);
+    if constexpr (L == 1) {
+#pragma unroll
+      for (int i = 0; i < N; i++) out[i] = p[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < N / L; i++) {
+        V v = reinterpret_cast<const V *>(p)[i];
+#pragma unroll
+        for (int j = 0; j < L; j++) out[i * L + j] = v[j];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; i++) out[i] = p[i];
+  }
+}
+
+// Per-batch record (layout documented with BatchPlan in gtk_plan.h).
+template <typename OffT, int MB>
+struct BatchRegs {
+  uint32_t info;   // nslots | tmax << 8 | ncols << 16 (wave-uniform)
+  uint32_t slot;   // this lane's slot (lane & 15)
+  int32_t seg0;    // column (lane & 15) of the batch
+  uint32_t la;
+  typename Off4<OffT>::type off[MB];  // rows 16 mb + 4 g + r of the slot
 };
 
-template <int D, int NV, int NS, int NQ, int FORM, typename CT, typename OffT>
-__global__ __launch_bounds__(kMfmaBlock) void matrix_tile_mfma_kernel(MatrixArgs args) {
-  constexpr int kBlock = kMfmaBlock;
+template <typename OffT, int MB>
+__device__ inline void load_batch(const unsigned char *__restrict__ rec, int rec_bytes, int off_stride, int64_t bi,
+                                  int nl, int g, BatchRegs<OffT, MB> &r) {
+  const unsigned char *R = rec + bi * rec_bytes;
+  r.info = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R));
+  r.slot = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R + 16) + nl);
+  r.seg0 = __builtin_nontemporal_load(reinterpret_cast<const int32_t *>(R + 80) + nl);
+  r.la = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R + 144) + nl);
+  const unsigned char *O = R + 208 + nl * off_stride * (int)sizeof(OffT);
+#pragma unroll
+  for (int mb = 0; mb < MB; mb++) {
+    const int a0 = mb * 16 + 4 * g;
+    r.off[mb] = a0 < off_stride ? *reinterpret_cast<const typename Off4<OffT>::type *>(O + a0 * (int)sizeof(OffT))
+                                : Off4<OffT>::all_none();
+  }
+}
+
+// Scalar forms on the matrix cores, one workgroup per tile, NW wavefronts.
+//
+// Phase A (all waves): node coordinates of the tile's cells are staged in
+// LDS (in chunks, in the space the accumulators use later); each thread
+// keeps one quadrature point q with its geometry gradients in registers and
+// computes, per cell, J = sum_n x_n (x) grad N_n(q), det J, J^-1 in FP64
+// and the factor G = kappa |det J| w_q J^-1 J^-T (mass: kappa |det J| w_q),
+// stored in LDS in the contraction precision CT.
+//
+// Phase B (each wave independently): batches of the tile, strided over the
+// waves.  A batch packs whole columns whose contributors (cell slot s,
+// local column b) total at most 16; for its 16 slots one product gives the
+// local columns
+//   A[a, n] = sum_k Bt[a, k] W[k, n],  k = (q, d),
+//   Bt[a, (q,d)] = d_d phi_a(q)                    (reference table, registers)
+//   W[(q,d), n]  = (G_{s_n,q} grad phi_{b_n}(q))_d  (built per batch)
+// with the k dimension ordered so that lane group g = lane/16 covers the
+// quadrature points g*QG .. g*QG+QG-1 with all their directions: each lane
+// reads G and grad phi_b once per point with vector LDS loads.  The 4 rows
+// per 16-row block a lane holds are added into FP64 column accumulators in
+// LDS through the record's row offsets, stepping over the contributor rank
+// t so that every nonzero is summed in contributor order (cells ascending)
+// and no two lanes of one step touch one address.  Finished segments are
+// written once to HBM and the accumulators cleared.  The record of the next
+// batch is loaded while the current one is computed.
+template <int D, int NV, int NS, int NQ, int FORM, typename CT, typename OffT, int NW>
+__global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) {
+  constexpr int kB = NW * 64;
   constexpr int NG = NGeo<D, FORM>::value;
+  constexpr int NGP = NG == 3 ? 4 : NG;  // staged factors per (cell, point)
   constexpr int KD = FORM == kLaplace ? D : 1;
-  constexpr int K = NQ * KD;
-  constexpr int KS = (K + 3) / 4;
+  constexpr int TP = FORM == kLaplace ? (D == 3 ? 4 : 2) : 1;  // table entries per (q, b)
+  constexpr int QG = (NQ + 3) / 4;
+  constexpr int KS = QG * KD;
   constexpr int MB = (NS + 15) / 16;
-  constexpr int NWAVES = kBlock / 64;
-  constexpr int NT = NQ * KD * NS;  // reference table entries
+  constexpr int CPR = kB / NQ;  // cells per geometry round
   using V4 = typename Acc4<CT>::type;
   using O4 = Off4<OffT>;
+  static_assert(CPR >= 1, "block too small for the quadrature");
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  CT *tab_s = reinterpret_cast<CT *>(lds_raw);  // Laplace: [NQ][D][NS] gradients, mass: [NQ][NS] values
-  CT *G_s = tab_s + NT;                          // [cells][NQ][NG]
-  ColInfo *col_s = reinterpret_cast<ColInfo *>(lds_raw + args.lds_col);
-  uint32_t *ctr_s = reinterpret_cast<uint32_t *>(lds_raw + args.lds_ctr);  // slot | b << 16
-  unsigned char *off_s = lds_raw + args.lds_off;                            // row offsets
-  int *bat_s = reinterpret_cast<int *>(lds_raw + args.lds_bat);            // batch -> first column
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nl = lane & 15, g = lane >> 4;
-  double *acc = reinterpret_cast<double *>(lds_raw + args.acc_offset) + (size_t)wave * args.acc_per_wave;
+  CT *tab_s = reinterpret_cast<CT *>(lds_raw);                     // [NQ][NS][TP]
+  CT *G_s = reinterpret_cast<CT *>(lds_raw + args.lds_geo);        // [cells][NQ][NGP]
+  double *acc_all = reinterpret_cast<double *>(lds_raw + args.acc_offset);
+  double *X_s = acc_all;  // phase A: [stage cells][NV][D]
 
+  const int tid = threadIdx.x;
   const int tile = blockIdx.x;
   const int cell0 = args.tile_cell_ptr[tile];
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
-  const int col0 = args.tile_col_ptr[tile];
-  const int ncol = args.tile_col_ptr[tile + 1] - col0;
-  const int ctr0 = args.col_ctr_ptr[col0];
-  const int nctr = args.col_ctr_ptr[col0 + ncol] - ctr0;
-  const int b0 = args.tile_batch_ptr[tile];
-  const int nbat = args.tile_batch_ptr[tile + 1] - b0;
 
-  // prologue: tables, plan metadata, accumulators
-  for (int i = threadIdx.x; i < NT; i += kBlock) tab_s[i] = (CT)(FORM == kLaplace ? args.gphi[i] : args.phi[i]);
-  for (int i = threadIdx.x; i < ncol; i += kBlock) {
-    const int pc = col0 + i;
-    const int j = args.col_id[pc];
-    const int c0 = args.colptr[j];
-    const int k0 = args.col_ctr_ptr[pc];
-    col_s[i] = ColInfo{c0 - 1, args.colptr[j + 1] - c0, k0 - ctr0, args.col_ctr_ptr[pc + 1] - k0};
+  for (int i = tid; i < NQ * NS * TP; i += kB) {
+    const int q = i / (NS * TP), rem = i - q * NS * TP, b = rem / TP, d = rem - b * TP;
+    double v;
+    if constexpr (FORM == kLaplace) v = d < D ? args.gphi[(q * D + d) * NS + b] : 0.0;
+    else v = args.phi[q * NS + b];
+    tab_s[i] = (CT)v;
   }
-  for (int i = threadIdx.x; i < nctr; i += kBlock)
-    ctr_s[i] = (uint32_t)args.ctr_slot[ctr0 + i] | ((uint32_t)args.ctr_b[ctr0 + i] << 16);
+
+  // phase A: geometry
   {
-    const int row_bytes = args.off_stride * (int)sizeof(OffT);  // multiple of 16
-    const uint4 *src = reinterpret_cast<const uint4 *>(args.ctr_off + (int64_t)ctr0 * row_bytes);
-    uint4 *dst = reinterpret_cast<uint4 *>(off_s);
-    const int n16 = nctr * row_bytes / 16;
-    for (int i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
-  }
-  for (int i = threadIdx.x; i <= nbat; i += kBlock) bat_s[i] = args.batch_col[b0 + i] - col0;
-  for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
-
-  // phase A: FP64 geometry per (cell, point)
-  for (int idx = threadIdx.x; idx < ncell * NQ; idx += kBlock) {
-    const int s = idx / NQ, q = idx - s * NQ;
-    const int64_t cell = args.tile_cells[cell0 + s];
-    double Ji[D][D];
-    const double dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji) * args.w[q];
-    const double kdV = (args.coef ? args.coef[cell] : 1.0) * dV;
-    CT *G = G_s + (s * NQ + q) * NG;
-    if constexpr (FORM == kLaplace) {
+    const int tq = tid % NQ, tcs = tid / NQ;
+    const bool gl = tcs < CPR;
+    double gq[NV][D];
 #pragma unroll
-      for (int i = 0; i < D; i++)
+    for (int n = 0; n < NV; n++)
 #pragma unroll
-        for (int j = i; j < D; j++) {
-          double gg = 0.0;
+      for (int d = 0; d < D; d++) gq[n][d] = gl ? args.gN[(tq * NV + n) * D + d] : 0.0;
+    const double wq = gl ? args.w[tq] : 0.0;
+    for (int c0 = 0; c0 < ncell; c0 += args.stage_cells) {
+      const int nch = min(args.stage_cells, ncell - c0);
+      for (int i = tid; i < nch * NV; i += kB) {
+        const int sl = i / NV, n = i - sl * NV;
+        const int64_t cell = args.tile_cells[cell0 + c0 + sl];
+        const int64_t node = args.cell_nodes[cell * NV + n];
 #pragma unroll
-          for (int m = 0; m < D; m++) gg = fma(Ji[i][m], Ji[j][m], gg);
-          G[sym_index<D>(i, j)] = (CT)(kdV * gg);
+        for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[node * D + d];
+      }
+      __syncthreads();
+      if (gl) {
+        for (int sl = tcs; sl < nch; sl += CPR) {
+          const double *X = X_s + sl * NV * D;
+          double J[D][D];
+#pragma unroll
+          for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) J[i][j] = 0.0;
+#pragma unroll
+          for (int n = 0; n < NV; n++)
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+              for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
+          double Ji[D][D];
+          const double det = inverse<D>(J, Ji);
+          const int64_t cell = args.tile_cells[cell0 + c0 + sl];
+          const double kdV = (args.coef ? args.coef[cell] : 1.0) * fabs(det) * wq;
+          CT *G = G_s + ((c0 + sl) * NQ + tq) * NGP;
+          if constexpr (FORM == kLaplace) {
+            CT gv[NGP];
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+              for (int j = i; j < D; j++) {
+                double gg = 0.0;
+#pragma unroll
+                for (int m = 0; m < D; m++) gg = fma(Ji[i][m], Ji[j][m], gg);
+                gv[sym_index<D>(i, j)] = (CT)(kdV * gg);
+              }
+            if constexpr (NGP > NG) gv[NGP - 1] = (CT)0;
+#pragma unroll
+            for (int i = 0; i < NGP; i++) G[i] = gv[i];
+          } else {
+            G[0] = (CT)kdV;
+          }
         }
-    } else {
-      G[0] = (CT)kdV;
+      }
+      __syncthreads();
     }
   }
-  __syncthreads();
 
+  // phase B: batches
+  const int wave = tid >> 6, lane = tid & 63;
+  const int nl = lane & 15, g = lane >> 4;
+  double *acc = acc_all + (size_t)wave * args.acc_per_wave;
+  for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
   CT afrag[MB][KS];
 #pragma unroll
   for (int mb = 0; mb < MB; mb++)
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
-      const int a = mb * 16 + a_row<CT>(nl), k = 4 * ks + g;
-      afrag[mb][ks] = (a < NS && k < K) ? tab_s[k * NS + a] : (CT)0;  // [q][d][a] or [q][a]
+      const int a = mb * 16 + a_row<CT>(nl), q = g * QG + ks / KD, d = ks % KD;
+      afrag[mb][ks] = (a < NS && q < NQ) ? tab_s[(q * NS + a) * TP + d] : (CT)0;
     }
-  const int ostride4 = args.off_stride >> 2;  // 4-offset words per contributor
-  const typename O4::type *offw = reinterpret_cast<const typename O4::type *>(off_s);
-  for (int bi = wave; bi < nbat; bi += NWAVES) {
-    const int cb = bat_s[bi];
-    const int m = bat_s[bi + 1] - cb;  // columns in the batch (<= 16)
-    const bool colv = nl < m;
-    const ColInfo ci_info = col_s[cb + (colv ? nl : 0)];
-    const int len = colv ? ci_info.len : 0;
-    const int kc = colv ? ci_info.kc : 0;
-    // exclusive prefixes of contributor counts and segment lengths
-    int slot0 = kc, abase = len;
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-      const int t1 = __shfl_up(slot0, o, 16), t2 = __shfl_up(abase, o, 16);
-      if (nl >= o) {
-        slot0 += t1;
-        abase += t2;
-      }
-    }
-    const int nslots = __shfl(slot0, 15, 16);
-    slot0 -= kc;
-    abase -= len;
-    int tmax = kc;
-#pragma unroll
-    for (int o = 8; o >= 1; o >>= 1) tmax = max(tmax, __shfl_xor(tmax, o, 16));
-    // slot nl: its column ci and rank t_n within the column
-    int ci = 0;
-#pragma unroll
-    for (int i = 1; i < 16; i++) {
-      const int si = __shfl(slot0, i, 16);
-      if (i < m && si <= nl) ci = i;
-    }
+  const int bt0 = args.tile_batch_ptr[tile], bt1 = args.tile_batch_ptr[tile + 1];
+  int bi = bt0 + wave;
+  BatchRegs<OffT, MB> nxt;
+  if (bi < bt1) load_batch<OffT, MB>(args.rec, args.rec_bytes, args.off_stride, bi, nl, g, nxt);
+  for (; bi < bt1; bi += NW) {
+    BatchRegs<OffT, MB> cur = nxt;
+    cur.info = __builtin_amdgcn_readfirstlane(cur.info);
+    if (bi + NW < bt1) load_batch<OffT, MB>(args.rec, args.rec_bytes, args.off_stride, bi + NW, nl, g, nxt);
+    const int nslots = cur.info & 0xFF, tmax = (cur.info >> 8) & 0xFF, ncols = cur.info >> 16;
     const bool valid = nl < nslots;
-    const int t_n = nl - __shfl(slot0, ci, 16);
-    const int k = __shfl(ci_info.kb, ci, 16) + t_n;  // tile-relative contributor
-    const int my_acc = __shfl(abase, ci, 16);
-    int s = 0, b = 0;
-    typename O4::type o[MB];
-    if (valid) {
-      const uint32_t sb = ctr_s[k];
-      s = sb & 0xFFFF;
-      b = sb >> 16;
-#pragma unroll
-      for (int mb = 0; mb < MB; mb++) o[mb] = offw[k * ostride4 + mb * 4 + g];
-    }
-    const CT *Gs = G_s + s * NQ * NG;
+    const int s = cur.slot & 0xFFF, b = (cur.slot >> 12) & 0xFF, t_n = (cur.slot >> 20) & 0xF;
+    const int ci = cur.slot >> 24;
+    const int my_acc = (int)(__shfl((int)cur.la, ci, 16) >> 16);
+    const CT *Gc = G_s + s * NQ * NGP;
+    const CT *Tb = tab_s + b * TP;
     V4 c[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
-    // W without branches: padded k and empty slots read in-bounds entries
-    // and are zeroed by a select, so every LDS read can be issued early
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-      const int kk = 4 * ks + g;
-      const int kq = kk < K ? kk : K - 1;
-      CT w;
+    for (int j = 0; j < QG; j++) {
+      const int q = g * QG + j;
+      const int qq = q < NQ ? q : NQ - 1;
+      CT G[NGP], T[TP];
+      lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + qq * NGP, G);
+      lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + qq * NS * TP, T);
+      const bool on = valid && q < NQ;
+      CT W[KD];
       if constexpr (FORM == kLaplace) {
-        const int q = kq / D, d = kq - q * D;
-        const CT *G = Gs + q * NG;
-        w = G[sym_index<D>(d, 0)] * tab_s[(q * D) * NS + b];
-#pragma unroll
-        for (int e = 1; e < D; e++) w = fma(G[sym_index<D>(d, e)], tab_s[(q * D + e) * NS + b], w);
+        if constexpr (D == 3) {
+          W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
+          W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
+          W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+        } else {
+          W[0] = G[0] * T[0] + G[1] * T[1];
+          W[1] = G[1] * T[0] + G[2] * T[1];
+        }
       } else {
-        w = Gs[kq] * tab_s[kq * NS + b];
+        W[0] = G[0] * T[0];
       }
-      w = (valid && kk < K) ? w : (CT)0;
 #pragma unroll
-      for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(afrag[mb][ks], w, c[mb]);
+      for (int d = 0; d < KD; d++) {
+        const CT wv = on ? W[d] : (CT)0;
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(afrag[mb][j * KD + d], wv, c[mb]);
+      }
     }
     // destinations: rows outside the row set or padding go to a private
     // per-lane slot past the accumulators
@@ -453,7 +521,7 @@ __global__ __launch_bounds__(kMfmaBlock) void matrix_tile_mfma_kernel(MatrixArgs
     for (int mb = 0; mb < MB; mb++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int off = valid ? O4::get(o[mb], r) : (int)O4::none;
+        const int off = valid ? O4::get(cur.off[mb], r) : (int)O4::none;
         dst[mb][r] = off != (int)O4::none ? my_acc + off : args.acc_per_wave - 64 + lane;
       }
     for (int t = 0; t < tmax; t++) {
@@ -464,12 +532,15 @@ __global__ __launch_bounds__(kMfmaBlock) void matrix_tile_mfma_kernel(MatrixArgs
           for (int r = 0; r < 4; r++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
       }
     }
-    // write the finished segments and clear the accumulators
-    for (int i = 0; i < m; i++) {
-      const int li = __shfl(len, i, 16), ai = __shfl(abase, i, 16), si = __shfl(ci_info.seg0, i, 16);
-      for (int r = lane; r < li; r += 64) {
-        args.nzval[(int64_t)si + r] = acc[ai + r];
-        acc[ai + r] = 0.0;
+    // write the finished segments and clear the accumulators: lane group g
+    // takes columns g, g+4, ...
+    for (int i = g; i < ncols; i += 4) {
+      const uint32_t lai = (uint32_t)__shfl((int)cur.la, i, 16);
+      const int64_t s0 = __shfl(cur.seg0, i, 16);
+      const int len = lai & 0xFFFF, ab = lai >> 16;
+      for (int r = nl; r < len; r += 16) {
+        args.nzval[s0 + r] = acc[ab + r];
+        acc[ab + r] = 0.0;
       }
     }
   }
@@ -539,21 +610,26 @@ struct Elem {
   static constexpr int d = D, nv = NV, ns = NS, nq = NQ, nc = NC;
 };
 
-template <typename E, int FORM, typename CT, typename OffT>
-static KernelInfo matrix_mfma_info() {
+template <typename E, int FORM, typename CT, typename OffT, int NW>
+static KernelInfo matrix_batch_info() {
+  constexpr int NG = NGeo<E::d, FORM>::value;
+  constexpr int NGP = NG == 3 ? 4 : NG;
+  constexpr int TP = FORM == kLaplace ? (E::d == 3 ? 4 : 2) : 1;
   KernelInfo k;
-  k.fn = reinterpret_cast<const void *>(&matrix_tile_mfma_kernel<E::d, E::nv, E::ns, E::nq, FORM, CT, OffT>);
-  k.table_bytes = (size_t)(E::nq * (FORM == kLaplace ? E::d : 1) * E::ns) * sizeof(CT);
-  k.geo_bytes_per_cell = (size_t)E::nq * NGeo<E::d, FORM>::value * sizeof(CT);
+  k.fn = reinterpret_cast<const void *>(&matrix_batch_kernel<E::d, E::nv, E::ns, E::nq, FORM, CT, OffT, NW>);
+  k.table_bytes = (size_t)E::nq * E::ns * TP * sizeof(CT);
+  k.geo_bytes_per_cell = (size_t)E::nq * NGP * sizeof(CT);
   k.cols_per_wave = 16;
   k.mfma = true;
-  k.block = kMfmaBlock;
+  k.block = NW * 64;
   return k;
 }
 
 template <typename E, int FORM, typename CT>
-static KernelInfo matrix_mfma_info(int off_bytes) {
-  return off_bytes == 1 ? matrix_mfma_info<E, FORM, CT, uint8_t>() : matrix_mfma_info<E, FORM, CT, uint16_t>();
+static KernelInfo matrix_batch_info(int off_bytes, int waves) {
+  if (waves == 4)
+    return off_bytes == 1 ? matrix_batch_info<E, FORM, CT, uint8_t, 4>() : matrix_batch_info<E, FORM, CT, uint16_t, 4>();
+  return off_bytes == 1 ? matrix_batch_info<E, FORM, CT, uint8_t, 8>() : matrix_batch_info<E, FORM, CT, uint16_t, 8>();
 }
 
 template <typename E, int FORM, typename CT>
@@ -582,16 +658,18 @@ static bool match(const ElemKey &key) {
 }
 
 template <typename E>
-static bool try_matrix(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, KernelInfo &out) {
+static bool try_matrix(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
+                       KernelInfo &out) {
   if (!match<E>(key)) return false;
   if constexpr (E::nc == 1) {
     if (form != kLaplace && form != kMass) return false;
     if (allow_mfma) {
       if (form == kLaplace)
-        out = precision == 0 ? matrix_mfma_info<E, kLaplace, double>(off_bytes)
-                             : matrix_mfma_info<E, kLaplace, float>(off_bytes);
+        out = precision == 0 ? matrix_batch_info<E, kLaplace, double>(off_bytes, waves)
+                             : matrix_batch_info<E, kLaplace, float>(off_bytes, waves);
       else
-        out = precision == 0 ? matrix_mfma_info<E, kMass, double>(off_bytes) : matrix_mfma_info<E, kMass, float>(off_bytes);
+        out = precision == 0 ? matrix_batch_info<E, kMass, double>(off_bytes, waves)
+                             : matrix_batch_info<E, kMass, float>(off_bytes, waves);
     } else {
       if (form == kLaplace)
         out = precision == 0 ? matrix_info<E, kLaplace, double>() : matrix_info<E, kLaplace, float>();
@@ -623,14 +701,14 @@ using HexQ1 = Elem<3, 8, 8, 8, 1>;
 using HexQ2 = Elem<3, 8, 27, 27, 1>;
 using TetP2v = Elem<3, 4, 10, 14, 3>;
 
-bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma,
+bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out) {
-  return try_matrix<QuadQ1>(key, form, precision, off_bytes, allow_mfma, out) ||
-         try_matrix<QuadQ2>(key, form, precision, off_bytes, allow_mfma, out) ||
-         try_matrix<QuadQ3>(key, form, precision, off_bytes, allow_mfma, out) ||
-         try_matrix<HexQ1>(key, form, precision, off_bytes, allow_mfma, out) ||
-         try_matrix<HexQ2>(key, form, precision, off_bytes, allow_mfma, out) ||
-         try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, out);
+  return try_matrix<QuadQ1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadQ2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadQ3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexQ1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexQ2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, waves, out);
 }
 
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {

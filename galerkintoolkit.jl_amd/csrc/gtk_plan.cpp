@@ -301,6 +301,72 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
   return "";
 }
 
+std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t, BatchPlan &out, int nthreads) {
+  if (p.cols != t.set) return "tile plan set differs from the pattern columns";
+  const int nd = m.dofs_per_cell;
+  out = BatchPlan();
+  if (t.max_ctr > kBatchSlots) return "a column has more contributors than a batch holds";
+  if (t.max_tile_cells > 4095) return "tile too large for batch records";
+  if (nd > 255) return "too many local dofs for batch records";
+  int max_seg = 0;
+  for (int64_t j = 0; j < p.ncols; j++) max_seg = std::max(max_seg, p.colptr[j + 1] - p.colptr[j]);
+  out.off_bytes = max_seg < 0xFF ? 1 : 2;
+  if (max_seg >= 0xFFFF) return "column segment too long for 16-bit offsets";
+  out.off_stride = (nd + 3) / 4 * 4;
+  const int slot_bytes = out.off_stride * out.off_bytes;
+  out.rec_bytes = (BatchPlan::kHeader + kBatchSlots * slot_bytes + 15) / 16 * 16;
+  out.nbatches = (int64_t)t.batch_col.size() - 1;
+  out.rec.assign((size_t)out.nbatches * out.rec_bytes, 0);
+  std::vector<int32_t> batch_tile(std::max<int64_t>(0, out.nbatches));
+  for (int tl = 0; tl < t.ntiles; tl++)
+    for (int32_t b = t.tile_batch_ptr[tl]; b < t.tile_batch_ptr[tl + 1]; b++) batch_tile[b] = tl;
+  std::vector<int> bacc(std::max<int64_t>(1, out.nbatches), 0);
+  std::string err;
+  parallel_for(out.nbatches, nthreads, [&](int, int64_t b0, int64_t b1) {
+    for (int64_t b = b0; b < b1; b++) {
+      uint8_t *r = out.rec.data() + (size_t)b * out.rec_bytes;
+      uint32_t *slot = reinterpret_cast<uint32_t *>(r + 16);
+      int32_t *seg0 = reinterpret_cast<int32_t *>(r + 80);
+      uint32_t *la = reinterpret_cast<uint32_t *>(r + 144);
+      uint8_t *off = r + BatchPlan::kHeader;
+      std::memset(off, 0xFF, (size_t)kBatchSlots * slot_bytes);
+      const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[batch_tile[b]];
+      int ns = 0, tmax = 0, abase = 0;
+      const int ncols = t.batch_col[b + 1] - t.batch_col[b];
+      for (int ci = 0; ci < ncols; ci++) {
+        const int32_t pc = t.batch_col[b] + ci;
+        const int32_t j = t.col_id[pc];
+        const int32_t *seg = p.rowval.data() + (p.colptr[j] - 1);
+        const int32_t len = p.colptr[j + 1] - p.colptr[j];
+        seg0[ci] = p.colptr[j] - 1;
+        la[ci] = (uint32_t)len | (uint32_t)abase << 16;
+        const int cnt = t.col_ctr_ptr[pc + 1] - t.col_ctr_ptr[pc];
+        tmax = std::max(tmax, cnt);
+        for (int tt = 0; tt < cnt; tt++, ns++) {
+          const int32_t k = t.col_ctr_ptr[pc] + tt;
+          slot[ns] = (uint32_t)t.ctr_slot[k] | (uint32_t)t.ctr_b[k] << 12 | (uint32_t)tt << 20 | (uint32_t)ci << 24;
+          const int32_t *dofs = m.cell_dofs + (int64_t)cells[t.ctr_slot[k]] * nd;
+          for (int a = 0; a < nd; a++) {
+            int32_t rr = map_dof(dofs[a], p.rows);
+            if (!rr) continue;
+            const int o = (int)(std::lower_bound(seg, seg + len, rr) - seg);
+            const size_t pos = (size_t)ns * out.off_stride + a;
+            if (out.off_bytes == 1) off[pos] = (uint8_t)o;
+            else reinterpret_cast<uint16_t *>(off)[pos] = (uint16_t)o;
+          }
+        }
+        abase += len;
+      }
+      reinterpret_cast<uint32_t *>(r)[0] = (uint32_t)ns | (uint32_t)tmax << 8 | (uint32_t)ncols << 16;
+      reinterpret_cast<uint32_t *>(r)[1] = (uint32_t)abase;
+      bacc[b] = abase;
+    }
+  });
+  if (out.nbatches > 0) out.max_batch_acc = *std::max_element(bacc.begin(), bacc.end());
+  if (out.max_batch_acc >= 0xFFFF) return "batch accumulator exceeds 16-bit offsets";
+  return err;
+}
+
 std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads) {
   if (p.cols != t.set) return "tile plan set differs from the pattern columns";
   const int nd = m.dofs_per_cell;

@@ -59,6 +59,30 @@ struct OffsetPlan {
   int max_batch_acc = 0;  // largest sum of segment lengths over the columns of one batch
 };
 
+// Matrix-core batch records (scalar spaces whose columns have at most
+// kBatchSlots contributors).  One fixed-size record per batch, in tile
+// order, holds everything a wavefront needs for the batch, so the kernel
+// streams it with coalesced loads:
+//   u32  info     nslots | tmax << 8 | ncols << 16   (tmax: most contributors of one column)
+//   u32  acc      sum of the segment lengths of the batch's columns
+//   u32  pad[2]
+//   u32  slot[16] s | b << 12 | t << 20 | ci << 24  (tile cell slot, local column b,
+//                 rank t of the contributor in its column, batch column ci)
+//   i32  seg0[16] 0-based start of column ci in the value array
+//   u32  la[16]   len | abase << 16  (segment length, offset in the batch accumulator)
+//   off  [16][off_stride]  row offset of local row a in the slot's column segment
+//                 (bytes each: off_bytes), all ones for rows outside the row set
+//                 and for padding
+struct BatchPlan {
+  std::vector<uint8_t> rec;
+  int64_t nbatches = 0;
+  int rec_bytes = 0;
+  int off_stride = 0;  // dofs per cell rounded up to 4
+  int off_bytes = 1;
+  int max_batch_acc = 0;
+  static constexpr int kHeader = 208;  // bytes before the offsets
+};
+
 struct MeshView {
   int dim = 0;
   int64_t n_nodes = 0, n_cells = 0;
@@ -76,6 +100,9 @@ std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, i
 // owned DOFs and max_ctr contributors per tile.
 std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int max_ctr, int chunk,
                         TilePlan &out, int nthreads);
+// Batch records of a tile plan over the columns of `p` (every column must
+// have at most kBatchSlots contributors).
+std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t, BatchPlan &out, int nthreads);
 // Row offsets of every contributor of a tile plan over the columns of `p`.
 std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads);
 

@@ -9,11 +9,81 @@
 
 #include "gtk_plan.h"
 
+// Replays the batch records exactly as the matrix-core kernel consumes
+// them: per batch, slots decoded from their words, rows added through the
+// record's offsets into a batch accumulator in rank steps t < tmax, and the
+// columns' segments copied out.  Also checks that each column's
+// contributors are ranked in ascending cell order and that every nonzero
+// is written exactly once.
+static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const gtk::TilePlan &t, double *nzval,
+                          char *err, int errlen) {
+  auto fail = [&](const char *msg) {
+    std::strncpy(err, msg, errlen - 1);
+    return -1;
+  };
+  if (t.max_ctr > gtk::kBatchSlots) return 2;  // matrix-core batches not applicable
+  gtk::BatchPlan bp;
+  std::string e = gtk::build_batches(m, p, t, bp, 4);
+  if (!e.empty()) return fail(e.c_str());
+  const int nd = m.dofs_per_cell;
+  const uint32_t none = bp.off_bytes == 1 ? 0xFF : 0xFFFF;
+  std::vector<int> written(p.nnz, 0);
+  for (int tl = 0; tl < t.ntiles; tl++) {
+    const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[tl];
+    const int ncell = t.tile_cell_ptr[tl + 1] - t.tile_cell_ptr[tl];
+    for (int32_t b = t.tile_batch_ptr[tl]; b < t.tile_batch_ptr[tl + 1]; b++) {
+      const uint8_t *R = bp.rec.data() + (size_t)b * bp.rec_bytes;
+      const uint32_t info = reinterpret_cast<const uint32_t *>(R)[0];
+      const int nslots = info & 0xFF, tmax = (info >> 8) & 0xFF, ncols = info >> 16;
+      const int acc_total = (int)reinterpret_cast<const uint32_t *>(R)[1];
+      const uint32_t *slot = reinterpret_cast<const uint32_t *>(R + 16);
+      const int32_t *seg0 = reinterpret_cast<const int32_t *>(R + 80);
+      const uint32_t *la = reinterpret_cast<const uint32_t *>(R + 144);
+      const uint8_t *off = R + gtk::BatchPlan::kHeader;
+      if (nslots > gtk::kBatchSlots || ncols > 16 || ncols < 1 || acc_total > bp.max_batch_acc)
+        return fail("batch header out of range");
+      std::vector<double> acc(acc_total, 0.0);
+      std::vector<int> last_cell(ncols, -1), count(ncols, 0);
+      for (int tt = 0; tt < tmax; tt++)
+        for (int n = 0; n < nslots; n++) {
+          const uint32_t w = slot[n];
+          const int s = w & 0xFFF, lb = (w >> 12) & 0xFF, tn = (w >> 20) & 0xF, ci = w >> 24;
+          if (tn != tt) continue;
+          if (s >= ncell || lb >= nd || ci >= ncols) return fail("slot word out of range");
+          if (cells[s] <= last_cell[ci]) return fail("contributors not in ascending cell order");
+          last_cell[ci] = cells[s];
+          count[ci]++;
+          const int len = la[ci] & 0xFFFF, abase = la[ci] >> 16;
+          for (int a = 0; a < bp.off_stride; a++) {
+            const size_t pos = (size_t)n * bp.off_stride + a;
+            const uint32_t o = bp.off_bytes == 1 ? off[pos] : reinterpret_cast<const uint16_t *>(off)[pos];
+            if (o == none) continue;
+            if (a >= nd || (int)o >= len || abase + (int)o >= acc_total) return fail("offset out of range");
+            acc[abase + o] += 1.0 + cells[s] * 7.0 + a * 131.0 + lb * 977.0;
+          }
+        }
+      for (int ci = 0; ci < ncols; ci++) {
+        const int len = la[ci] & 0xFFFF, abase = la[ci] >> 16;
+        if (count[ci] == 0) return fail("column without contributors");
+        for (int r = 0; r < len; r++) {
+          const int64_t z = seg0[ci] + r;
+          if (z < 0 || z >= p.nnz) return fail("segment out of range");
+          nzval[z] = acc[abase + r];
+          written[z]++;
+        }
+      }
+    }
+  }
+  for (int64_t z = 0; z < p.nnz; z++)
+    if (written[z] != 1) return fail("a nonzero is not written exactly once");
+  return 0;
+}
+
 extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t n_cells, int nv,
                            const int32_t *cell_nodes0, int nd, int64_t n_free, int64_t n_diri,
                            const int32_t *cell_dofs, int rows, int cols, int max_cells, int chunk,
                            int64_t *nnz_out, int32_t *colptr, int32_t *rowval, double *nzval, int64_t cap,
-                           int *ntiles, char *err, int errlen) {
+                           int *ntiles, int mode, char *err, int errlen) {
   gtk::MeshView m;
   m.dim = dim; m.n_nodes = n_nodes; m.n_cells = n_cells; m.nodes_per_cell = nv;
   m.xyz = xyz; m.cell_nodes = cell_nodes0; m.cell_dofs = cell_dofs; m.dofs_per_cell = nd;
@@ -48,6 +118,7 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
       }
     }
   }
+  if (mode == 1) return replay_batches(m, p, t, nzval, err, errlen);
   for (int64_t i = 0; i < p.nnz; i++) nzval[i] = -1.0;  // every entry must be written
   for (int tl = 0; tl < t.ntiles; tl++) {
     const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[tl];

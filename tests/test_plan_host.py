@@ -1,7 +1,8 @@
-"""Host logic of the symbolic phase: CSC pattern, column-owner tiles and
-row offsets, replayed on the CPU with exact integer-valued entries and
-compared bit for bit with a COO assembly in the reference's contribution
-order (cells ascending, trial outer, test inner; src/assembly.jl:189-208)."""
+"""Host logic of the symbolic phase: CSC pattern, column-owner tiles, row
+offsets (VALU kernel) and batch records (matrix-core kernel), replayed on
+the CPU with exact integer-valued entries and compared bit for bit with a
+COO assembly in the reference's contribution order (cells ascending, trial
+outer, test inner; src/assembly.jl:189-208)."""
 import ctypes as C
 import os
 import subprocess
@@ -25,7 +26,7 @@ def harness(tmp_path_factory):
     i32 = np.ctypeslib.ndpointer(np.int32, flags="C")
     L.harness_run.argtypes = [C.c_int, C.c_int64, f64, C.c_int64, C.c_int, i32, C.c_int, C.c_int64, C.c_int64, i32,
                               C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), i32, i32, f64, C.c_int64,
-                              C.POINTER(C.c_int), C.c_char_p, C.c_int]
+                              C.POINTER(C.c_int), C.c_int, C.c_char_p, C.c_int]
     return L
 
 
@@ -53,7 +54,7 @@ def coo_reference(cell_dofs, rows, cols, nrows, ncols):
     return A
 
 
-def run(harness, GT, cells, order, simplex=False, diri=0, rows=1, cols=1, max_cells=64, chunk=8):
+def run(harness, GT, cells, order, simplex=False, diri=0, rows=1, cols=1, max_cells=64, chunk=8, mode=0):
     D = len(cells)
     m = GT.cartesian_mesh(tuple([0.0, 1.0] * D), cells, simplexify=simplex)
     db = GT.boundary(m) if diri else None
@@ -69,7 +70,9 @@ def run(harness, GT, cells, order, simplex=False, diri=0, rows=1, cols=1, max_ce
     err = C.create_string_buffer(256)
     rc = harness.harness_run(D, m.coords.shape[0], np.ascontiguousarray(m.coords), cn.shape[0], cn.shape[1], cn,
                              cd.shape[1], V.nfree, V.ndiri, cd, rows, cols, max_cells, chunk, C.byref(nnz), colptr,
-                             rowval, nz, cap, C.byref(nt), err, 256)
+                             rowval, nz, cap, C.byref(nt), mode, err, 256)
+    if rc == 2:
+        return None  # batch records not applicable (a column with more than 16 contributors)
     assert rc == 0, err.value.decode()
     nrows = V.nfree if rows == 1 else V.ndiri
     ref = coo_reference(V.cell_dofs, rows, cols, nrows, len(colptr) - 1)
@@ -84,9 +87,17 @@ def run(harness, GT, cells, order, simplex=False, diri=0, rows=1, cols=1, max_ce
     ((6, 5), 1, False, 1), ((5, 4), 2, False, 0), ((4, 5), 3, False, 1), ((4, 3, 5), 1, False, 0),
     ((3, 4, 3), 2, False, 1), ((3, 3, 2), 2, True, 0), ((4, 4), 2, True, 1)])
 @pytest.mark.parametrize("max_cells,chunk", [(64, 8), (27, 1), (200, 64)])
-def test_plan_replay(harness, GT, cells, order, simplex, diri, max_cells, chunk):
-    run(harness, GT, cells, order, simplex, diri, max_cells=max_cells, chunk=chunk)
+@pytest.mark.parametrize("mode", [0, 1], ids=["offsets", "batches"])
+def test_plan_replay(harness, GT, cells, order, simplex, diri, max_cells, chunk, mode):
+    run(harness, GT, cells, order, simplex, diri, max_cells=max_cells, chunk=chunk, mode=mode)
 
 
-def test_plan_dirichlet_columns(harness, GT):
-    run(harness, GT, (5, 6), 2, diri=1, rows=1, cols=2)
+@pytest.mark.parametrize("mode", [0, 1], ids=["offsets", "batches"])
+def test_plan_dirichlet_columns(harness, GT, mode):
+    run(harness, GT, (5, 6), 2, diri=1, rows=1, cols=2, mode=mode)
+
+
+def test_batches_cover_hex_q2(harness, GT):
+    """Q2 hexahedra: every column has at most 8 contributors, so the batch
+    records apply (the replay must not report them as inapplicable)."""
+    assert run(harness, GT, (4, 3, 3), 2, diri=1, max_cells=180, chunk=32, mode=1) is not None
