@@ -7,7 +7,7 @@ in libgtk_hip.so (HIP kernels for gfx950 behind the C ABI of include/gtk_hip.h).
 """
 from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, Laplace, Load, Mass, Measure,
                        SparseMatrixCSC, assemble_matrix, assemble_matrix_with_free_and_dirichlet_columns,
-                       assemble_vector, face_constant_field, measure, update_matrix)
+                       assemble_vector, face_constant_field, linear_problem, measure, solve, update_matrix)
 from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
 from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space
 from . import inputs, refcell
@@ -15,7 +15,8 @@ from . import inputs, refcell
 __all__ = [
     "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "Laplace", "Load",
     "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
-    "assemble_vector", "face_constant_field", "measure", "update_matrix", "Mesh", "cartesian_mesh",
+    "assemble_vector", "face_constant_field", "linear_problem", "measure", "solve", "update_matrix", "Mesh",
+    "cartesian_mesh",
     "perturb_interior_nodes", "Domain", "LagrangeSpace", "boundary", "interior", "lagrange_space", "inputs",
     "refcell",
 ]

@@ -21,13 +21,18 @@ EXPORTS = [
     "gtk_create", "gtk_destroy", "gtk_last_error", "gtk_version", "gtk_set_mesh", "gtk_update_coordinates",
     "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
-    "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix",
+    "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve",
 ]
 
 
 class GtkStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("cells", C.c_int64), ("nnz", C.c_int64),
                 ("bytes_alg", C.c_int64), ("flops_alg", C.c_int64)]
+
+
+class GtkSolveStats(C.Structure):
+    _fields_ = [("outer_iterations", C.c_int), ("inner_iterations", C.c_int), ("converged", C.c_int),
+                ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
 
 
 class GtkError(RuntimeError):
@@ -69,6 +74,7 @@ def load():
     L.gtk_assemble_matrix_async.argtypes = [vp, i, i, i, i]
     L.gtk_sync.argtypes = [vp]
     L.gtk_time_matrix.argtypes = [vp, i, i, i, i, i, C.POINTER(C.c_double)]
+    L.gtk_solve.argtypes = [vp, i, f64, f64, C.c_double, i, i, C.c_double, C.POINTER(GtkSolveStats)]
     _lib = L
     return L
 
@@ -161,6 +167,14 @@ class Context:
         ms = C.c_double()
         self.check(self.L.gtk_time_matrix(self.h, form, precision, rows, cols, launches, C.byref(ms)), "gtk_time_matrix")
         return ms.value
+
+    def solve(self, precision, b, x0, rtol, max_outer, max_inner, inner_rtol):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.array(x0, np.float64, copy=True) if x0 is not None else np.zeros_like(b)
+        st = GtkSolveStats()
+        self.check(self.L.gtk_solve(self.h, precision, b, x, rtol, max_outer, max_inner, inner_rtol, C.byref(st)),
+                   "gtk_solve")
+        return x, st
 
     def values_device_ptr(self, rows, cols):
         p = C.c_void_p()

@@ -180,6 +180,7 @@ def assemble_matrix(form, T, trial_space, test_space, free_or_dirichlet=(FREE, F
     nzval, stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, nnz)
     A = SparseMatrixCSC(nrows, ncols, colptr, rowval, nzval)
     A.stats = stats
+    A.device = (a, rows, cols)
     if reuse:
         return A, (a, form, rows, cols, prec)
     return A
@@ -218,3 +219,38 @@ def assemble_matrix_with_free_and_dirichlet_columns(form, T, U, V, precision=FP6
     A = assemble_matrix(form, T, U, V, (FREE, FREE), precision=precision, device=device)
     Ad = assemble_matrix(form, T, U, V, (FREE, DIRICHLET), precision=precision, device=device)
     return A, Ad
+
+
+def solve(A, b, precision=FP64, rtol=1e-10, x0=None, max_outer=30, max_inner=20000, inner_rtol=1e-4):
+    """Solve A x = b on the GPU for a (FREE, FREE) matrix of assemble_matrix
+    (symmetric positive definite forms), with the values of its last device
+    assembly.  precision=MIXED: FP64 iterative refinement around an FP32
+    Jacobi-preconditioned CG (inner_rtol per step); FP64: one FP64
+    Jacobi-preconditioned CG.  Stands where the reference solves the
+    assembled linear problem (src/problems.jl:439-462, PS.solve).
+    Returns (x, stats); raises if rtol is not reached."""
+    dev = getattr(A, "device", None)
+    if dev is None:
+        raise ValueError("A must come from assemble_matrix (its values live on the device)")
+    a, rows, cols = dev
+    if (rows, cols) != (FREE, FREE):
+        raise ValueError("solve needs the (FREE, FREE) block")
+    prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+    x, st = a.ctx.solve(prec, np.asarray(b, np.float64), x0, rtol, max_outer, max_inner, inner_rtol)
+    if not st.converged:
+        raise L.GtkError(f"solve did not reach rtol={rtol}: relative residual {st.rel_residual:.3e} after "
+                         f"{st.outer_iterations} refinement steps, {st.inner_iterations} CG iterations")
+    return x, st
+
+
+def linear_problem(uhd, a_form, l_form, precision=FP64, device=0):
+    """GT.PartitionedSolvers_linear_problem(uhd, a, l): the (FREE, FREE)
+    matrix, and the load vector lifted by the Dirichlet values of uhd,
+    b = l - A_{FREE,DIRICHLET} x_D (src/problems.jl:439-452)."""
+    V = uhd.space
+    A, Ad = assemble_matrix_with_free_and_dirichlet_columns(a_form, float, V, V, precision=precision, device=device)
+    b = assemble_vector(l_form, float, V, device=device)
+    xd = np.asarray(uhd.values[V.nfree:], np.float64)
+    if xd.size and Ad.nnz:
+        b = b - Ad.to_scipy() @ xd
+    return A, b

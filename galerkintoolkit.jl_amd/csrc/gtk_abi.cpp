@@ -20,6 +20,7 @@
 #include "../../include/gtk_hip.h"
 #include "gtk_kernels.h"
 #include "gtk_plan.h"
+#include "gtk_solve.h"
 
 namespace {
 
@@ -51,13 +52,19 @@ struct ExecPlan {
   gtk::KernelInfo k;
   DevBuf tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, ctr_off, tile_batch_ptr, rec;
   size_t lds = 0;
-  int lds_geo = 0, acc_offset = 0, acc_per_wave = 0, stage_cells = 0;
+  int lds_geo = 0, acc_offset = 0, acc_per_wave = 0, stage_cells = 0, lds_claim = 0;
+};
+
+// Device work space of the solve on a (FREE, FREE) block.
+struct SolveWork {
+  DevBuf rowval, perm, val64, val32, dinv, x, b, r, d, rr, z, p, q, partial, partial2, sc, bad;
 };
 
 struct MatrixPlan {
   gtk::Pattern pat;
   int max_seg = 0;
   DevBuf colptr, nzval;
+  SolveWork sw;
   std::map<int, std::unique_ptr<ExecPlan>> exec;  // key: form * 2 + precision
   bool valid = false;
 };
@@ -96,6 +103,7 @@ struct gtk_ctx {
   // device copies
   DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef;
   int waves = 8;  // wavefronts per workgroup of the matrix-core kernels
+  size_t lds_budget = 156 * 1024;  // LDS a workgroup may use (less gives several workgroups per CU)
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
   std::map<int, std::unique_ptr<VectorPlan>> vplans;
   bool has_mesh = false, has_space = false, has_ref = false;
@@ -190,10 +198,10 @@ static int tile_chunk(const gtk_ctx *ctx, int max_cells) {
 
 // Largest cell count per tile whose staged geometry fits the LDS left by
 // the tables and the accumulators of kernel k.
-static int tile_cells_for(const gtk::KernelInfo &k, size_t acc_bytes) {
+static int tile_cells_for(const gtk::KernelInfo &k, size_t acc_bytes, size_t budget) {
   const size_t fixed = align16(k.table_bytes) + acc_bytes + 1024;
   const size_t geo = std::max<size_t>(1, k.geo_bytes_per_cell);
-  const size_t avail = kLdsBudget > fixed ? kLdsBudget - fixed : 8 * geo;
+  const size_t avail = budget > fixed ? budget - fixed : 8 * geo;
   return (int)std::max<size_t>(8, std::min<size_t>(4095, avail / geo));
 }
 
@@ -244,8 +252,13 @@ int gtk_create(int device, gtk_ctx **out) {
     return GTK_ENOMEM;
   }
   ctx->device = device;
-  if (const char *w = std::getenv("GTK_WAVES")) {  // matrix-core workgroup shape (tuning)
+  // matrix-core workgroup shape and LDS budget (tuning overrides)
+  if (const char *w = std::getenv("GTK_WAVES")) {
     if (std::strcmp(w, "4") == 0) ctx->waves = 4;
+  }
+  if (const char *l = std::getenv("GTK_LDS_KB")) {
+    const long kb = std::strtol(l, nullptr, 10);
+    if (kb >= 16 && kb <= 160) ctx->lds_budget = (size_t)kb * 1024;
   }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
@@ -459,7 +472,7 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
   if (mfma) {
     // accumulator bound: a batch adds at most nd rows per slot
     const size_t acc_bound = (size_t)ctx->waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
-    const int max_cells = tile_cells_for(ep->k, acc_bound);
+    const int max_cells = tile_cells_for(ep->k, acc_bound, ctx->lds_budget);
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     if (ep->tiles.max_ctr > gtk::kBatchSlots) mfma = false;
@@ -467,18 +480,26 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
   if (mfma) {
     e = gtk::build_batches(m, plan.pat, ep->tiles, ep->bat, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    if (ep->bat.endmask_words > 16) {  // the kernel reads at most 16 end-mask words
+      ep->bat = gtk::BatchPlan();
+      mfma = false;
+    }
+  }
+  if (mfma) {
     ep->lds_geo = (int)align16(ep->k.table_bytes);
     ep->acc_offset = (int)align16(ep->lds_geo + (size_t)ep->tiles.max_tile_cells * ep->k.geo_bytes_per_cell);
     ep->acc_per_wave = (ep->bat.max_batch_acc + 64 + 1) & ~1;
     const size_t acc_bytes = (size_t)ctx->waves * ep->acc_per_wave * sizeof(double);
-    ep->stage_cells = (int)std::max<size_t>(1, acc_bytes / ((size_t)ctx->nv * ctx->dim * sizeof(double)));
-    ep->lds = ep->acc_offset + acc_bytes;
+    // coordinates and coefficient of a staged cell
+    ep->stage_cells = (int)std::max<size_t>(1, acc_bytes / (((size_t)ctx->nv * ctx->dim + 1) * sizeof(double)));
+    ep->lds_claim = (int)(ep->acc_offset + acc_bytes);
+    ep->lds = ep->lds_claim + 16;
   } else {
     if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, ctx->waves, ep->k))
       FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
     const size_t waves = (size_t)ep->k.block / 64;
     const size_t acc_bytes = waves * ep->k.cols_per_wave * std::max(1, plan.max_seg) * sizeof(double);
-    const int max_cells = tile_cells_for(ep->k, acc_bytes);
+    const int max_cells = tile_cells_for(ep->k, acc_bytes, kLdsBudget);
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     e = gtk::build_offsets(m, plan.pat, ep->tiles, ep->offs, nt);
@@ -540,10 +561,11 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   a.ctr_off = ep->ctr_off.as<uint8_t>();
   a.rec = ep->rec.as<unsigned char>();
   a.rec_bytes = ep->bat.rec_bytes;
+  a.off_offset = ep->bat.off_offset;
+  a.endmask_words = ep->bat.endmask_words;
   a.colptr = plan.colptr.as<int32_t>();
   a.nzval = plan.nzval.as<double>();
   if (ep->k.mfma) {
-    a.off_stride = ep->bat.off_stride;
     a.off_bytes = ep->bat.off_bytes;
   } else {
     a.off_stride = ep->offs.stride;
@@ -554,6 +576,7 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   a.acc_offset = ep->acc_offset;
   a.acc_per_wave = ep->acc_per_wave;
   a.stage_cells = ep->stage_cells;
+  a.lds_claim = ep->lds_claim;
   void *args[] = {&a};
   HIPCHK(ctx, hipLaunchKernel(ep->k.fn, dim3(ep->tiles.ntiles), dim3(ep->k.block), args, ep->lds, ctx->stream));
   return GTK_OK;
@@ -730,6 +753,120 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
     stats->bytes_alg = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 + vp.nrows * 8;
     stats->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * ctx->dofs_per_cell + 2 * ctx->nv * ctx->dim * ctx->dim);
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return GTK_OK;
+}
+
+int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rtol, int max_outer, int max_inner,
+              double inner_rtol, gtk_solve_stats *stats) {
+  if (!ctx) return GTK_EINVAL;
+  auto t0 = std::chrono::steady_clock::now();
+  MatrixPlan *pp = find_plan(ctx, GTK_FREE, GTK_FREE);
+  if (!pp) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern(GTK_FREE, GTK_FREE) and gtk_assemble_matrix first");
+  if (!b || !x) FAIL(ctx, GTK_EINVAL, "b and x are required");
+  if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
+  if (!(rtol > 0) || max_outer < 1 || max_inner < 1 || !(inner_rtol > 0 && inner_rtol < 1))
+    FAIL(ctx, GTK_EINVAL, "rtol > 0, max_outer >= 1, max_inner >= 1, 0 < inner_rtol < 1");
+  MatrixPlan &P = *pp;
+  const int64_t n = P.pat.ncols, nnz = P.pat.nnz;
+  if (n != P.pat.nrows) FAIL(ctx, GTK_EINVAL, "the block is not square");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  SolveWork &w = P.sw;
+  const bool f32 = precision == GTK_MIXED;
+  const size_t vb = f32 ? sizeof(float) : sizeof(double);
+  int rc;
+  if (!w.perm.p) {
+    std::vector<int32_t> perm;
+    const std::string e = gtk::build_transpose_perm(P.pat, perm);
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    if ((rc = upload(ctx, w.perm, perm.data(), perm.size()))) return rc;
+    if ((rc = upload(ctx, w.rowval, P.pat.rowval.data(), P.pat.rowval.size()))) return rc;
+  }
+  if ((rc = alloc(ctx, w.val64, std::max<int64_t>(1, nnz) * sizeof(double)))) return rc;
+  if (f32 && (rc = alloc(ctx, w.val32, std::max<int64_t>(1, nnz) * sizeof(float)))) return rc;
+  const size_t nb = std::max<int64_t>(1, n);
+  if ((rc = alloc(ctx, w.dinv, nb * vb)) || (rc = alloc(ctx, w.x, nb * 8)) || (rc = alloc(ctx, w.b, nb * 8)) ||
+      (rc = alloc(ctx, w.r, nb * 8)) || (rc = alloc(ctx, w.d, nb * vb)) || (rc = alloc(ctx, w.rr, nb * vb)) ||
+      (rc = alloc(ctx, w.z, nb * vb)) || (rc = alloc(ctx, w.p, nb * vb)) || (rc = alloc(ctx, w.q, nb * vb)) ||
+      (rc = alloc(ctx, w.partial, gtk::kSolvePartials * 8)) || (rc = alloc(ctx, w.partial2, gtk::kSolvePartials * 8)) ||
+      (rc = alloc(ctx, w.sc, gtk::kSolveScalars * 8)) || (rc = alloc(ctx, w.bad, 16)))
+    return rc;
+  gtk::SolveMat M;
+  M.n = n;
+  M.ptr = P.colptr.as<int32_t>();
+  M.idx = w.rowval.as<int32_t>();
+  M.lanes = gtk::solve_rows_per_lane_group(n ? (double)nnz / n : 1.0);
+  hipStream_t st = ctx->stream;
+  double *sc = w.sc.as<double>();
+  HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
+  gtk::launch_gather_values(nnz, w.perm.as<int32_t>(), P.nzval.as<double>(), w.val64.as<double>(),
+                            f32 ? w.val32.as<float>() : nullptr, st);
+  gtk::launch_diag_inv(M, w.val64.as<double>(), w.dinv.p, !f32, w.bad.as<int>(), st);
+  HIPCHK(ctx, hipMemcpyAsync(w.b.p, b, n * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(w.x.p, x, n * 8, hipMemcpyHostToDevice, st));
+  int bad = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&bad, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  if (bad) FAIL(ctx, GTK_EINVAL, "the matrix has a non-positive diagonal entry (not SPD)");
+  auto read_sc = [&](int slot, double &out) -> int {
+    HIPCHK(ctx, hipMemcpyAsync(&out, sc + slot, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    return GTK_OK;
+  };
+  // FP64 residual r = b - A x and its norm
+  auto residual = [&](double &rn) -> int {
+    gtk::launch_spmv(M, w.val64.p, true, w.x.p, true, w.b.p, w.r.p, true, st);
+    gtk::launch_dot(n, w.r.p, w.r.p, true, w.partial.as<double>(), sc, 6, st);
+    double v = 0;
+    int e = read_sc(6, v);
+    rn = std::sqrt(std::max(0.0, v));
+    return e;
+  };
+  double bn2 = 0;
+  gtk::launch_dot(n, w.b.p, w.b.p, true, w.partial.as<double>(), sc, 6, st);
+  if ((rc = read_sc(6, bn2))) return rc;
+  const double bn = std::sqrt(bn2);
+  gtk::CgVectors v;
+  v.f64 = !f32;
+  v.d = w.d.p;
+  v.r = w.rr.p;
+  v.z = w.z.p;
+  v.p = w.p.p;
+  v.q = w.q.p;
+  v.dinv = w.dinv.p;
+  const void *val = f32 ? w.val32.p : w.val64.p;
+  constexpr int kCheck = 8;  // iterations between convergence reads
+  double rn = 0;
+  if ((rc = residual(rn))) return rc;
+  int outer = 0, inner_total = 0;
+  const double target = rtol * (bn > 0 ? bn : 1.0);
+  while (rn > target && outer < max_outer) {
+    // inner solve of A d = r / |r| (FP32 for the mixed form, FP64 otherwise)
+    const double s = rn > 0 ? 1.0 / rn : 1.0;
+    gtk::launch_cg_start(n, s, w.r.as<double>(), v, w.partial.as<double>(), sc, st);
+    // the FP64 form runs one CG to the outer tolerance; the mixed form to inner_rtol
+    const double itol = f32 ? inner_rtol : std::max(target / rn, 1e-300);
+    int it = 0;
+    while (it < max_inner) {
+      for (int k = 0; k < kCheck && it < max_inner; k++, it++)
+        gtk::launch_cg_iteration(M, val, v, w.partial.as<double>(), w.partial2.as<double>(), sc, st);
+      double rr = 0;
+      if ((rc = read_sc(5, rr))) return rc;
+      if (!(std::sqrt(std::max(0.0, rr)) > itol)) break;
+    }
+    inner_total += it;
+    gtk::launch_refine(n, rn, v.d, !f32, w.x.as<double>(), st);
+    outer++;
+    if ((rc = residual(rn))) return rc;
+  }
+  HIPCHK(ctx, hipMemcpyAsync(x, w.x.p, n * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  if (stats) {
+    stats->outer_iterations = outer;
+    stats->inner_iterations = inner_total;
+    stats->rel_residual = bn > 0 ? rn / bn : rn;
+    stats->converged = rn <= target ? 1 : 0;
+    stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return GTK_OK;
 }

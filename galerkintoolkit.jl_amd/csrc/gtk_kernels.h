@@ -21,7 +21,7 @@ struct MatrixArgs {
   const uint16_t *ctr_slot, *ctr_b;
   const uint8_t *ctr_off;  // VALU kernel: off_stride entries per contributor, off_bytes each
   const unsigned char *rec;  // matrix-core kernels: batch records (BatchPlan)
-  int rec_bytes;
+  int rec_bytes, off_offset, endmask_words;
   const int32_t *colptr;
   double *nzval;
   int off_stride, off_bytes;
@@ -30,6 +30,7 @@ struct MatrixArgs {
   int acc_offset;    // byte offset of the accumulators in LDS
   int acc_per_wave;  // FP64 accumulator slots per wave (matrix-core kernels)
   int stage_cells;   // cells whose coordinates fit the accumulator space (matrix-core kernels)
+  int lds_claim;     // byte offset in LDS of the batch counter (matrix-core kernels)
 };
 
 struct VectorArgs {

@@ -308,30 +308,42 @@ __device__ inline void lds_load(const CT *__restrict__ p, CT (&out)[N]) {
   }
 }
 
-// Per-batch record (layout documented with BatchPlan in gtk_plan.h).
+constexpr int kBatchSlotsK = 16;  // contributor slots per matrix-core batch (MFMA N)
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Per-batch record (layout documented with BatchPlan in gtk_plan.h), as
+// lane (nl = lane & 15, g = lane >> 4) needs it.
 template <typename OffT, int MB>
 struct BatchRegs {
-  uint32_t info;   // nslots | tmax << 8 | ncols << 16 (wave-uniform)
-  uint32_t slot;   // this lane's slot (lane & 15)
-  int32_t seg0;    // column (lane & 15) of the batch
-  uint32_t la;
-  typename Off4<OffT>::type off[MB];  // rows 16 mb + 4 g + r of the slot
+  uint32_t info, acc;  // wave-uniform header words
+  uint32_t sb;         // slot whose W column this lane builds: s | b << 16
+  u32x4 at;            // slots 4g .. 4g+3: abase | t << 16
+  int32_t delta;       // column nl: seg0 - abase
+  u32x2 em;            // end-mask word (lane & 15)
+  typename Off4<OffT>::type off[MB];  // row nl + 16 mb of slots 4g .. 4g+3
 };
 
-template <typename OffT, int MB>
-__device__ inline void load_batch(const unsigned char *__restrict__ rec, int rec_bytes, int off_stride, int64_t bi,
-                                  int nl, int g, BatchRegs<OffT, MB> &r) {
-  const unsigned char *R = rec + bi * rec_bytes;
-  r.info = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R));
-  r.slot = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R + 16) + nl);
-  r.seg0 = __builtin_nontemporal_load(reinterpret_cast<const int32_t *>(R + 80) + nl);
-  r.la = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R + 144) + nl);
-  const unsigned char *O = R + 208 + nl * off_stride * (int)sizeof(OffT);
+template <typename CT, typename OffT, int NS, int MB>
+__device__ inline void load_batch(const unsigned char *__restrict__ rec, const MatrixArgs &args, int64_t bi, int lane,
+                                  BatchRegs<OffT, MB> &r) {
+  const int nl = lane & 15, g = lane >> 4;
+  const unsigned char *R = rec + bi * args.rec_bytes;
+  const u32x2 h = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(R));
+  r.info = h.x;
+  r.acc = h.y;
+  r.sb = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(R + 16) + a_row<CT>(nl));
+  r.at = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(R + 80) + g);
+  r.delta = __builtin_nontemporal_load(reinterpret_cast<const int32_t *>(R + 144) + nl);
+  const int w = lane & 15;
+  r.em = w < args.endmask_words ? __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(R + 208) + w)
+                                : u32x2{0u, 0u};
 #pragma unroll
   for (int mb = 0; mb < MB; mb++) {
-    const int a0 = mb * 16 + 4 * g;
-    r.off[mb] = a0 < off_stride ? *reinterpret_cast<const typename Off4<OffT>::type *>(O + a0 * (int)sizeof(OffT))
-                                : Off4<OffT>::all_none();
+    const int a = mb * 16 + nl;
+    r.off[mb] = a < NS ? *reinterpret_cast<const typename Off4<OffT>::type *>(
+                             R + args.off_offset + (a * 16 + 4 * g) * (int)sizeof(OffT))
+                       : Off4<OffT>::all_none();
   }
 }
 
@@ -348,18 +360,22 @@ __device__ inline void load_batch(const unsigned char *__restrict__ rec, int rec
 // waves.  A batch packs whole columns whose contributors (cell slot s,
 // local column b) total at most 16; for its 16 slots one product gives the
 // local columns
-//   A[a, n] = sum_k Bt[a, k] W[k, n],  k = (q, d),
-//   Bt[a, (q,d)] = d_d phi_a(q)                    (reference table, registers)
-//   W[(q,d), n]  = (G_{s_n,q} grad phi_{b_n}(q))_d  (built per batch)
+//   A[n, a] = sum_k W[n, k] Bt[k, a],  k = (q, d),
+//   W[n, (q,d)]  = (G_{s_n,q} grad phi_{b_n}(q))_d  (built per batch)
+//   Bt[(q,d), a] = d_d phi_a(q)                    (reference table, registers)
 // with the k dimension ordered so that lane group g = lane/16 covers the
 // quadrature points g*QG .. g*QG+QG-1 with all their directions: each lane
-// reads G and grad phi_b once per point with vector LDS loads.  The 4 rows
-// per 16-row block a lane holds are added into FP64 column accumulators in
-// LDS through the record's row offsets, stepping over the contributor rank
-// t so that every nonzero is summed in contributor order (cells ascending)
-// and no two lanes of one step touch one address.  Finished segments are
-// written once to HBM and the accumulators cleared.  The record of the next
-// batch is loaded while the current one is computed.
+// reads G and grad phi_b once per point with vector LDS loads.  Lane
+// (nl, g) ends up holding local row a = nl (+16) of slots 4g .. 4g+3.  The
+// rows are added into FP64 column accumulators in LDS through the record's
+// row offsets in rank steps t: step t adds, for each r, the rows of the
+// slots 4g+r whose rank within their column is t.  Slots of one step belong
+// to distinct columns, so no two lanes of one addition touch one address,
+// and every nonzero is summed in contributor order (cells ascending); an
+// addition no lane needs is skipped.  The accumulators of a batch are then
+// written flat and coalesced: entry e goes to e + delta[c], c = number of
+// column ends before e (an end mask per batch), and cleared.  The record of
+// the next batch is loaded while the current one is computed.
 template <int D, int NV, int NS, int NQ, int FORM, typename CT, typename OffT, int NW>
 __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) {
   constexpr int kB = NW * 64;
@@ -371,6 +387,8 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   constexpr int KS = QG * KD;
   constexpr int MB = (NS + 15) / 16;
   constexpr int CPR = kB / NQ;  // cells per geometry round
+  constexpr int KR = (kBatchSlotsK * NS + 63) / 64;  // write-out rounds of the largest batch
+  static_assert(KR <= 16, "end mask words are read from lanes 0..15");
   using V4 = typename Acc4<CT>::type;
   using O4 = Off4<OffT>;
   static_assert(CPR >= 1, "block too small for the quadrature");
@@ -379,8 +397,10 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   CT *G_s = reinterpret_cast<CT *>(lds_raw + args.lds_geo);        // [cells][NQ][NGP]
   double *acc_all = reinterpret_cast<double *>(lds_raw + args.acc_offset);
   double *X_s = acc_all;  // phase A: [stage cells][NV][D]
+  int *claim_ctr = reinterpret_cast<int *>(lds_raw + args.lds_claim);
 
   const int tid = threadIdx.x;
+  if (tid == 0) *claim_ctr = NW;  // read after the geometry phase's barriers
   const int tile = blockIdx.x;
   const int cell0 = args.tile_cell_ptr[tile];
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
@@ -405,12 +425,31 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
     const double wq = gl ? args.w[tq] : 0.0;
     for (int c0 = 0; c0 < ncell; c0 += args.stage_cells) {
       const int nch = min(args.stage_cells, ncell - c0);
-      for (int i = tid; i < nch * NV; i += kB) {
-        const int sl = i / NV, n = i - sl * NV;
-        const int64_t cell = args.tile_cells[cell0 + c0 + sl];
-        const int64_t node = args.cell_nodes[cell * NV + n];
+      double *K_s = X_s + (size_t)args.stage_cells * NV * D;  // coefficient per staged cell
+      // staging: SU entries per thread and pass, each level of the
+      // (cell -> node -> coordinates) gather issued for all of them at once
+      constexpr int SU = 4;
+      for (int base = 0; base < nch * NV; base += kB * SU) {
+        int64_t cellv[SU], nodev[SU];
 #pragma unroll
-        for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[node * D + d];
+        for (int u = 0; u < SU; u++) {
+          const int i = base + u * kB + tid;
+          cellv[u] = i < nch * NV ? args.tile_cells[cell0 + c0 + i / NV] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const int i = base + u * kB + tid;
+          nodev[u] = i < nch * NV ? args.cell_nodes[cellv[u] * NV + (i % NV)] : 0;
+          if (i < nch * NV && i % NV == 0) K_s[i / NV] = args.coef ? args.coef[cellv[u]] : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const int i = base + u * kB + tid;
+          if (i < nch * NV) {
+#pragma unroll
+            for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[nodev[u] * D + d];
+          }
+        }
       }
       __syncthreads();
       if (gl) {
@@ -429,8 +468,7 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
               for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
           double Ji[D][D];
           const double det = inverse<D>(J, Ji);
-          const int64_t cell = args.tile_cells[cell0 + c0 + sl];
-          const double kdV = (args.coef ? args.coef[cell] : 1.0) * fabs(det) * wq;
+          const double kdV = K_s[sl] * fabs(det) * wq;
           CT *G = G_s + ((c0 + sl) * NQ + tq) * NGP;
           if constexpr (FORM == kLaplace) {
             CT gv[NGP];
@@ -460,27 +498,42 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   const int nl = lane & 15, g = lane >> 4;
   double *acc = acc_all + (size_t)wave * args.acc_per_wave;
   for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
-  CT afrag[MB][KS];
+  CT bfrag[MB][KS];
 #pragma unroll
   for (int mb = 0; mb < MB; mb++)
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
-      const int a = mb * 16 + a_row<CT>(nl), q = g * QG + ks / KD, d = ks % KD;
-      afrag[mb][ks] = (a < NS && q < NQ) ? tab_s[(q * NS + a) * TP + d] : (CT)0;
+      const int a = mb * 16 + nl, q = g * QG + ks / KD, d = ks % KD;
+      bfrag[mb][ks] = (a < NS && q < NQ) ? tab_s[(q * NS + a) * TP + d] : (CT)0;
     }
+  const int sink = args.acc_per_wave - 64 + lane;
   const int bt0 = args.tile_batch_ptr[tile], bt1 = args.tile_batch_ptr[tile + 1];
-  int bi = bt0 + wave;
+  // batches are claimed from a per-tile counter (wave w starts with batch w);
+  // the result does not depend on which wave runs a batch
+  auto claim = [&]() -> int {
+    int v = 0;
+    if (lane == 0) v = atomicAdd(claim_ctr, 1);
+    return bt0 + (int)__builtin_amdgcn_readfirstlane(v);
+  };
+  int bi = bt0 + wave, bn = bt1;
   BatchRegs<OffT, MB> nxt;
-  if (bi < bt1) load_batch<OffT, MB>(args.rec, args.rec_bytes, args.off_stride, bi, nl, g, nxt);
-  for (; bi < bt1; bi += NW) {
-    BatchRegs<OffT, MB> cur = nxt;
-    cur.info = __builtin_amdgcn_readfirstlane(cur.info);
-    if (bi + NW < bt1) load_batch<OffT, MB>(args.rec, args.rec_bytes, args.off_stride, bi + NW, nl, g, nxt);
-    const int nslots = cur.info & 0xFF, tmax = (cur.info >> 8) & 0xFF, ncols = cur.info >> 16;
-    const bool valid = nl < nslots;
-    const int s = cur.slot & 0xFFF, b = (cur.slot >> 12) & 0xFF, t_n = (cur.slot >> 20) & 0xF;
-    const int ci = cur.slot >> 24;
-    const int my_acc = (int)(__shfl((int)cur.la, ci, 16) >> 16);
+  if (bi < bt1) {
+    load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
+    bn = claim();
+  }
+  while (bi < bt1) {
+    const BatchRegs<OffT, MB> cur = nxt;
+    bi = bn;
+    if (bi < bt1) {
+      load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
+      bn = claim();
+    }
+    const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
+    const int acc_total = (int)__builtin_amdgcn_readfirstlane(cur.acc);
+    const int nslots = info & 0xFF, tmax = (info >> 8) & 0xFF;
+    // W column of slot a_row(nl)
+    const bool wvalid = a_row<CT>(nl) < nslots;
+    const int s = cur.sb & 0xFFFF, b = cur.sb >> 16;
     const CT *Gc = G_s + s * NQ * NGP;
     const CT *Tb = tab_s + b * TP;
     V4 c[MB];
@@ -493,7 +546,7 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
       CT G[NGP], T[TP];
       lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + qq * NGP, G);
       lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + qq * NS * TP, T);
-      const bool on = valid && q < NQ;
+      const bool on = wvalid && q < NQ;
       CT W[KD];
       if constexpr (FORM == kLaplace) {
         if constexpr (D == 3) {
@@ -511,37 +564,64 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
       for (int d = 0; d < KD; d++) {
         const CT wv = on ? W[d] : (CT)0;
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(afrag[mb][j * KD + d], wv, c[mb]);
+        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(wv, bfrag[mb][j * KD + d], c[mb]);
       }
     }
-    // destinations: rows outside the row set or padding go to a private
-    // per-lane slot past the accumulators
+    // destinations of the 4 slots x MB rows this lane holds; rows outside
+    // the row set and padding go to a private per-lane slot past the
+    // accumulators
+    const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
+    int rank[4];
     int dst[MB][4];
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++)
+    for (int r = 0; r < 4; r++) {
+      rank[r] = (int)(at4[r] >> 16);
+      const int ab = (int)(at4[r] & 0xFFFF);
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int off = valid ? O4::get(cur.off[mb], r) : (int)O4::none;
-        dst[mb][r] = off != (int)O4::none ? my_acc + off : args.acc_per_wave - 64 + lane;
-      }
-    for (int t = 0; t < tmax; t++) {
-      if (valid && t_n == t) {
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+      for (int mb = 0; mb < MB; mb++) {
+        const int off = O4::get(cur.off[mb], r);
+        dst[mb][r] = off != (int)O4::none ? ab + off : sink;
       }
     }
-    // write the finished segments and clear the accumulators: lane group g
-    // takes columns g, g+4, ...
-    for (int i = g; i < ncols; i += 4) {
-      const uint32_t lai = (uint32_t)__shfl((int)cur.la, i, 16);
-      const int64_t s0 = __shfl(cur.seg0, i, 16);
-      const int len = lai & 0xFFFF, ab = lai >> 16;
-      for (int r = nl; r < len; r += 16) {
-        args.nzval[s0 + r] = acc[ab + r];
-        acc[ab + r] = 0.0;
+    for (int t = 0; t < tmax; t++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (rank[r] == t) {
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+        }
       }
+    }
+    // flat write-out of the batch's segments in KR rounds of 64 entries
+    // (KR covers the largest batch, 16 slots x NS rows): all reads first,
+    // then the clears, then the stores.  Every round stores from every lane
+    // (lanes past the batch's entries store the last entry again, with its
+    // value), so each batch issues the same number of stores and the next
+    // batch's prefetched record is waited for without waiting for them.
+    {
+      int ends = 0;
+      const int ncols = (int)(info >> 16);
+      const int dlast = __shfl(cur.delta, ncols - 1);
+      double v[KR];
+      int32_t z[KR];
+#pragma unroll
+      for (int k = 0; k < KR; k++) {
+        const int e = k * 64 + lane;
+        const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);
+        const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
+        const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        ends += __builtin_popcount(lo) + __builtin_popcount(hi);
+        const int dlt = __shfl(cur.delta, cidx & 15);
+        const bool in = e < acc_total;
+        const int ee = in ? e : acc_total - 1;
+        v[k] = acc[ee];
+        z[k] = ee + (in ? dlt : dlast);
+      }
+#pragma unroll
+      for (int k = 0; k < KR; k++)
+        if (k * 64 + lane < acc_total) acc[k * 64 + lane] = 0.0;
+#pragma unroll
+      for (int k = 0; k < KR; k++) args.nzval[(int64_t)z[k]] = v[k];
     }
   }
 }

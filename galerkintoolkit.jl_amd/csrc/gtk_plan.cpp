@@ -306,30 +306,39 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
   const int nd = m.dofs_per_cell;
   out = BatchPlan();
   if (t.max_ctr > kBatchSlots) return "a column has more contributors than a batch holds";
-  if (t.max_tile_cells > 4095) return "tile too large for batch records";
-  if (nd > 255) return "too many local dofs for batch records";
+  if (t.max_tile_cells > 65535) return "tile too large for batch records";
   int max_seg = 0;
   for (int64_t j = 0; j < p.ncols; j++) max_seg = std::max(max_seg, p.colptr[j + 1] - p.colptr[j]);
-  out.off_bytes = max_seg < 0xFF ? 1 : 2;
   if (max_seg >= 0xFFFF) return "column segment too long for 16-bit offsets";
-  out.off_stride = (nd + 3) / 4 * 4;
-  const int slot_bytes = out.off_stride * out.off_bytes;
-  out.rec_bytes = (BatchPlan::kHeader + kBatchSlots * slot_bytes + 15) / 16 * 16;
+  out.off_bytes = max_seg < 0xFF ? 1 : 2;
   out.nbatches = (int64_t)t.batch_col.size() - 1;
+  // accumulator entries of each batch (sum of its columns' segment lengths)
+  std::vector<int> bacc(std::max<int64_t>(1, out.nbatches), 0);
+  for (int64_t b = 0; b < out.nbatches; b++)
+    for (int32_t pc = t.batch_col[b]; pc < t.batch_col[b + 1]; pc++) {
+      const int32_t j = t.col_id[pc];
+      bacc[b] += p.colptr[j + 1] - p.colptr[j];
+    }
+  if (out.nbatches > 0) out.max_batch_acc = *std::max_element(bacc.begin(), bacc.end());
+  if (out.max_batch_acc >= 0xFFFF) return "batch accumulator exceeds 16-bit offsets";
+  out.endmask_words = std::max(1, (out.max_batch_acc + 63) / 64);
+  out.off_offset = BatchPlan::kEndmask + 8 * out.endmask_words;
+  const int off_row = kBatchSlots * out.off_bytes;  // bytes per local row
+  out.rec_bytes = (out.off_offset + nd * off_row + 15) / 16 * 16;
   out.rec.assign((size_t)out.nbatches * out.rec_bytes, 0);
   std::vector<int32_t> batch_tile(std::max<int64_t>(0, out.nbatches));
   for (int tl = 0; tl < t.ntiles; tl++)
     for (int32_t b = t.tile_batch_ptr[tl]; b < t.tile_batch_ptr[tl + 1]; b++) batch_tile[b] = tl;
-  std::vector<int> bacc(std::max<int64_t>(1, out.nbatches), 0);
-  std::string err;
   parallel_for(out.nbatches, nthreads, [&](int, int64_t b0, int64_t b1) {
     for (int64_t b = b0; b < b1; b++) {
       uint8_t *r = out.rec.data() + (size_t)b * out.rec_bytes;
-      uint32_t *slot = reinterpret_cast<uint32_t *>(r + 16);
-      int32_t *seg0 = reinterpret_cast<int32_t *>(r + 80);
-      uint32_t *la = reinterpret_cast<uint32_t *>(r + 144);
-      uint8_t *off = r + BatchPlan::kHeader;
-      std::memset(off, 0xFF, (size_t)kBatchSlots * slot_bytes);
+      uint32_t *sb = reinterpret_cast<uint32_t *>(r + BatchPlan::kSlotSb);
+      uint32_t *at = reinterpret_cast<uint32_t *>(r + BatchPlan::kSlotAt);
+      int32_t *delta = reinterpret_cast<int32_t *>(r + BatchPlan::kDelta);
+      uint64_t *endmask = reinterpret_cast<uint64_t *>(r + BatchPlan::kEndmask);
+      uint8_t *off = r + out.off_offset;
+      std::memset(off, 0xFF, (size_t)nd * off_row);
+      for (int n = 0; n < kBatchSlots; n++) at[n] = 0xFFFFu << 16;
       const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[batch_tile[b]];
       int ns = 0, tmax = 0, abase = 0;
       const int ncols = t.batch_col[b + 1] - t.batch_col[b];
@@ -338,19 +347,20 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
         const int32_t j = t.col_id[pc];
         const int32_t *seg = p.rowval.data() + (p.colptr[j] - 1);
         const int32_t len = p.colptr[j + 1] - p.colptr[j];
-        seg0[ci] = p.colptr[j] - 1;
-        la[ci] = (uint32_t)len | (uint32_t)abase << 16;
+        delta[ci] = p.colptr[j] - 1 - abase;
+        if (len > 0) endmask[(abase + len - 1) / 64] |= 1ull << ((abase + len - 1) % 64);
         const int cnt = t.col_ctr_ptr[pc + 1] - t.col_ctr_ptr[pc];
         tmax = std::max(tmax, cnt);
         for (int tt = 0; tt < cnt; tt++, ns++) {
           const int32_t k = t.col_ctr_ptr[pc] + tt;
-          slot[ns] = (uint32_t)t.ctr_slot[k] | (uint32_t)t.ctr_b[k] << 12 | (uint32_t)tt << 20 | (uint32_t)ci << 24;
+          sb[ns] = (uint32_t)t.ctr_slot[k] | (uint32_t)t.ctr_b[k] << 16;
+          at[ns] = (uint32_t)abase | (uint32_t)tt << 16;
           const int32_t *dofs = m.cell_dofs + (int64_t)cells[t.ctr_slot[k]] * nd;
           for (int a = 0; a < nd; a++) {
             int32_t rr = map_dof(dofs[a], p.rows);
             if (!rr) continue;
             const int o = (int)(std::lower_bound(seg, seg + len, rr) - seg);
-            const size_t pos = (size_t)ns * out.off_stride + a;
+            const size_t pos = (size_t)a * kBatchSlots + ns;
             if (out.off_bytes == 1) off[pos] = (uint8_t)o;
             else reinterpret_cast<uint16_t *>(off)[pos] = (uint16_t)o;
           }
@@ -359,12 +369,31 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
       }
       reinterpret_cast<uint32_t *>(r)[0] = (uint32_t)ns | (uint32_t)tmax << 8 | (uint32_t)ncols << 16;
       reinterpret_cast<uint32_t *>(r)[1] = (uint32_t)abase;
-      bacc[b] = abase;
     }
   });
-  if (out.nbatches > 0) out.max_batch_acc = *std::max_element(bacc.begin(), bacc.end());
-  if (out.max_batch_acc >= 0xFFFF) return "batch accumulator exceeds 16-bit offsets";
-  return err;
+  return "";
+}
+
+std::string build_transpose_perm(const Pattern &p, std::vector<int32_t> &perm) {
+  if (p.nrows != p.ncols) return "the block is not square";
+  const int64_t n = p.ncols;
+  // row counts of the transpose must equal the column counts
+  std::vector<int64_t> cnt(n, 0);
+  for (int64_t k = 0; k < p.nnz; k++) cnt[p.rowval[k] - 1]++;
+  for (int64_t i = 0; i < n; i++)
+    if (cnt[i] != p.colptr[i + 1] - p.colptr[i]) return "the pattern is not structurally symmetric";
+  perm.assign(p.nnz, 0);
+  std::vector<int64_t> next(n);
+  for (int64_t i = 0; i < n; i++) next[i] = p.colptr[i] - 1;
+  // columns ascending: row i receives its entries in ascending column order
+  for (int64_t j = 0; j < n; j++)
+    for (int64_t k = p.colptr[j] - 1; k < p.colptr[j + 1] - 1; k++) {
+      const int64_t i = p.rowval[k] - 1;
+      const int64_t slot = next[i]++;
+      if (p.rowval[slot] - 1 != j) return "the pattern is not structurally symmetric";
+      perm[slot] = (int32_t)k;
+    }
+  return "";
 }
 
 std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads) {

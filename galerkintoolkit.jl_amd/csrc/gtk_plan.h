@@ -62,25 +62,29 @@ struct OffsetPlan {
 // Matrix-core batch records (scalar spaces whose columns have at most
 // kBatchSlots contributors).  One fixed-size record per batch, in tile
 // order, holds everything a wavefront needs for the batch, so the kernel
-// streams it with coalesced loads:
-//   u32  info     nslots | tmax << 8 | ncols << 16   (tmax: most contributors of one column)
-//   u32  acc      sum of the segment lengths of the batch's columns
+// streams it with coalesced loads.  Slots are the batch's contributors,
+// column by column, ranked within each column in ascending cell order.
+//   u32  info        nslots | tmax << 8 | ncols << 16   (tmax: most contributors of one column)
+//   u32  acc         sum of the segment lengths of the batch's columns
 //   u32  pad[2]
-//   u32  slot[16] s | b << 12 | t << 20 | ci << 24  (tile cell slot, local column b,
-//                 rank t of the contributor in its column, batch column ci)
-//   i32  seg0[16] 0-based start of column ci in the value array
-//   u32  la[16]   len | abase << 16  (segment length, offset in the batch accumulator)
-//   off  [16][off_stride]  row offset of local row a in the slot's column segment
-//                 (bytes each: off_bytes), all ones for rows outside the row set
-//                 and for padding
+//   u32  slot_sb[16] s | b << 16          (tile cell slot, local column b)
+//   u32  slot_at[16] abase | t << 16      (offset of the slot's column in the batch
+//                    accumulator, rank t; t = 0xFFFF for empty slots)
+//   i32  delta[16]   seg0 - abase of column ci (seg0: 0-based start of the column's
+//                    segment in the value array)
+//   u64  endmask[endmask_words]  bit e set iff accumulator entry e ends a column
+//   off  [dofs per cell][16]  row offset of local row a of each slot in its column
+//                    segment (off_bytes each), all ones for rows outside the row set
+//                    and for empty slots
 struct BatchPlan {
   std::vector<uint8_t> rec;
   int64_t nbatches = 0;
   int rec_bytes = 0;
-  int off_stride = 0;  // dofs per cell rounded up to 4
   int off_bytes = 1;
+  int endmask_words = 0;
+  int off_offset = 0;  // byte offset of the row offsets in a record
   int max_batch_acc = 0;
-  static constexpr int kHeader = 208;  // bytes before the offsets
+  static constexpr int kSlotSb = 16, kSlotAt = 80, kDelta = 144, kEndmask = 208;
 };
 
 struct MeshView {
@@ -103,6 +107,10 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
 // Batch records of a tile plan over the columns of `p` (every column must
 // have at most kBatchSlots contributors).
 std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t, BatchPlan &out, int nthreads);
+// CSR view of a square, structurally symmetric pattern: perm[k] is the CSC
+// position of the k-th entry in row-major order (row i's entries by
+// ascending column), so the rows reuse colptr/rowval as pointers/indices.
+std::string build_transpose_perm(const Pattern &p, std::vector<int32_t> &perm);
 // Row offsets of every contributor of a tile plan over the columns of `p`.
 std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads);
 

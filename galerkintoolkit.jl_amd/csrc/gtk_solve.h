@@ -1,0 +1,44 @@
+// gtk_solve.h -- launch interface of the solve kernels (gtk_solve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gtk {
+
+constexpr int kSolvePartials = 1024;  // blocks (and partial sums) of the vector kernels
+
+// A matrix with a structurally symmetric pattern, in CSR form over the
+// library's CSC pattern arrays (values gathered into row order).
+struct SolveMat {
+  int64_t n = 0;
+  const int32_t *ptr = nullptr;  // colptr, 1-based
+  const int32_t *idx = nullptr;  // rowval, 1-based
+  int lanes = 16;                // lanes per row of the SpMV
+};
+
+// Vectors of one conjugate-gradient solve (FP64 or FP32).
+struct CgVectors {
+  bool f64 = false;
+  void *d = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  const void *dinv = nullptr;
+};
+
+// Scalars kept on the device in sc[]: [0] r.z, [1] p.q, [2] alpha,
+// [3] new r.z, [4] beta, [5] r.r, [6] scratch dot.
+constexpr int kSolveScalars = 8;
+
+int solve_rows_per_lane_group(double mean_row);
+void launch_spmv(const SolveMat &m, const void *val, bool val64, const void *x, bool x64, const void *b, void *y,
+                 bool y64, hipStream_t st);
+void launch_dot(int64_t n, const void *a, const void *b, bool f64, double *partial, double *sc, int slot,
+                hipStream_t st);
+void launch_cg_start(int64_t n, double s, const double *r64, const CgVectors &v, double *partial, double *sc,
+                     hipStream_t st);
+void launch_cg_iteration(const SolveMat &m, const void *val, const CgVectors &v, double *partial, double *partial2,
+                         double *sc, hipStream_t st);
+void launch_refine(int64_t n, double s, const void *d, bool f64, double *x, hipStream_t st);
+void launch_diag_inv(const SolveMat &m, const double *val, void *dinv, bool f64, int *bad, hipStream_t st);
+void launch_gather_values(int64_t n, const int32_t *perm, const double *a, double *v64, float *v32, hipStream_t st);
+
+}  // namespace gtk

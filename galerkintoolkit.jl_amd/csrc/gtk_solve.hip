@@ -1,0 +1,265 @@
+// gtk_solve.hip -- gfx950 kernels of the mixed-precision solve of an
+// assembled (FREE, FREE) block: FP64 iterative refinement around an FP32
+// Jacobi-preconditioned conjugate gradient.
+//
+// The matrices are the library's CSC blocks of symmetric forms, whose
+// patterns are structurally symmetric: (colptr, rowval) serve as CSR row
+// pointers and column indices (1-based), with the values gathered into row
+// order once per solve (the values themselves need not be exactly
+// symmetric: a mixed-precision assembly is not).
+//
+// Reductions are deterministic: every block sums a fixed strided subset in a
+// fixed tree order into its own partial, and one block sums the partials in
+// order; dot products accumulate in FP64 for both precisions.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gtk_solve.h"
+
+namespace gtk {
+
+namespace {
+
+constexpr int kRB = 256;  // threads per block of the vector kernels
+
+__device__ inline double block_sum(double v, double *sh) {
+  // wave reduction (fixed xor tree) then the waves in order
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sh[i];
+  }
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+// y = A x (or y = b - A x when b != nullptr); S lanes per row.
+template <typename AT, typename XT, typename YT, int S>
+__global__ __launch_bounds__(kRB) void spmv_kernel(int64_t n, const int32_t *__restrict__ ptr,
+                                                   const int32_t *__restrict__ idx, const AT *__restrict__ val,
+                                                   const XT *__restrict__ x, const YT *__restrict__ b,
+                                                   YT *__restrict__ y) {
+  using Acc = typename std::conditional<sizeof(AT) == 8 || sizeof(XT) == 8, double, float>::type;
+  const int64_t row = ((int64_t)blockIdx.x * kRB + threadIdx.x) / S;
+  const int sl = threadIdx.x & (S - 1);
+  Acc acc = 0;
+  if (row < n) {
+    const int32_t k0 = ptr[row] - 1, k1 = ptr[row + 1] - 1;
+    for (int32_t k = k0 + sl; k < k1; k += S) acc += (Acc)val[k] * (Acc)x[idx[k] - 1];
+  }
+#pragma unroll
+  for (int o = S / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, S);
+  if (row < n && sl == 0) y[row] = b ? (YT)((Acc)b[row] - acc) : (YT)acc;
+}
+
+template <int S>
+void launch_spmv_s(const SolveMat &m, const void *val, bool val64, const void *x, bool x64, const void *b, void *y,
+                   bool y64, hipStream_t st) {
+  const int64_t threads = m.n * S;
+  const dim3 grid((unsigned)((threads + kRB - 1) / kRB));
+  if (val64 && x64 && y64)
+    spmv_kernel<double, double, double, S><<<grid, kRB, 0, st>>>(m.n, m.ptr, m.idx, (const double *)val,
+                                                                  (const double *)x, (const double *)b, (double *)y);
+  else
+    spmv_kernel<float, float, float, S><<<grid, kRB, 0, st>>>(m.n, m.ptr, m.idx, (const float *)val,
+                                                               (const float *)x, (const float *)b, (float *)y);
+}
+
+// partial[blockIdx] = sum over this block's strided share of a_i * b_i
+template <typename T>
+__global__ __launch_bounds__(kRB) void dot_partial_kernel(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
+                                                          double *__restrict__ partial) {
+  __shared__ double sh[kRB / 64];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB)
+    v += (double)a[i] * (double)b[i];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// sums np partials in order into sc[slot]
+__global__ __launch_bounds__(kRB) void sum_partials_kernel(int np, const double *__restrict__ partial,
+                                                           double *__restrict__ sc, int slot) {
+  __shared__ double sh[kRB / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRB) v += partial[i];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) sc[slot] = v;
+}
+
+// CG scalars in sc: [0] rz, [1] pq, [2] alpha, [3] rz_new, [4] beta, [5] rr
+__global__ void alpha_kernel(double *sc) {
+  sc[2] = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0;
+}
+__global__ void beta_kernel(double *sc) {
+  sc[4] = sc[0] != 0.0 ? sc[3] / sc[0] : 0.0;
+  sc[0] = sc[3];
+}
+
+// d += alpha p; r -= alpha q; z = dinv r; partials of r.z and r.r
+template <typename T>
+__global__ __launch_bounds__(kRB) void cg_update_kernel(int64_t n, const double *__restrict__ sc, T *__restrict__ d,
+                                                        T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
+                                                        const T *__restrict__ q, const T *__restrict__ dinv,
+                                                        double *__restrict__ part_rz, double *__restrict__ part_rr) {
+  __shared__ double sh[kRB / 64];
+  const T alpha = (T)sc[2];
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
+    d[i] += alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = dinv[i] * ri;
+    z[i] = zi;
+    rz += (double)ri * (double)zi;
+    rr += (double)ri * (double)ri;
+  }
+  rz = block_sum(rz, sh);
+  if (threadIdx.x == 0) part_rz[blockIdx.x] = rz;
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+}
+
+// p = z + beta p
+template <typename T>
+__global__ __launch_bounds__(kRB) void cg_direction_kernel(int64_t n, const double *__restrict__ sc,
+                                                           const T *__restrict__ z, T *__restrict__ p) {
+  const T beta = (T)sc[4];
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) p[i] = z[i] + beta * p[i];
+}
+
+// start of an inner solve: d = 0, r = s * r64, z = dinv r, p = z; partials of r.z
+template <typename T>
+__global__ __launch_bounds__(kRB) void cg_start_kernel(int64_t n, double s, const double *__restrict__ r64,
+                                                       T *__restrict__ d, T *__restrict__ r, T *__restrict__ z,
+                                                       T *__restrict__ p, const T *__restrict__ dinv,
+                                                       double *__restrict__ part_rz) {
+  __shared__ double sh[kRB / 64];
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
+    const T ri = (T)(s * r64[i]);
+    d[i] = (T)0;
+    r[i] = ri;
+    const T zi = dinv[i] * ri;
+    z[i] = zi;
+    p[i] = zi;
+    rz += (double)ri * (double)zi;
+  }
+  rz = block_sum(rz, sh);
+  if (threadIdx.x == 0) part_rz[blockIdx.x] = rz;
+}
+
+// x += s * d (FP64 refinement step)
+template <typename T>
+__global__ __launch_bounds__(kRB) void refine_kernel(int64_t n, double s, const T *__restrict__ d,
+                                                     double *__restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) x[i] += s * (double)d[i];
+}
+
+// inverse diagonal of the CSR rows (rows sorted by column index)
+template <typename T>
+__global__ __launch_bounds__(kRB) void diag_inv_kernel(int64_t n, const int32_t *__restrict__ ptr,
+                                                       const int32_t *__restrict__ idx, const double *__restrict__ val,
+                                                       T *__restrict__ dinv, int *__restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x;
+  if (i >= n) return;
+  int32_t lo = ptr[i] - 1, hi = ptr[i + 1] - 1;
+  const int32_t want = (int32_t)(i + 1);
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (idx[mid] < want) lo = mid + 1;
+    else hi = mid;
+  }
+  const double dv = (lo < ptr[i + 1] - 1 && idx[lo] == want) ? val[lo] : 0.0;
+  if (!(dv > 0.0)) atomicOr(bad, 1);
+  dinv[i] = dv > 0.0 ? (T)(1.0 / dv) : (T)0;
+}
+
+// CSR values from the CSC values: v64[k] = a[perm[k]], v32[k] = (float) v64[k]
+__global__ __launch_bounds__(kRB) void gather_values_kernel(int64_t n, const int32_t *__restrict__ perm,
+                                                            const double *__restrict__ a, double *__restrict__ v64,
+                                                            float *__restrict__ v32) {
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
+    const double v = a[perm[i]];
+    v64[i] = v;
+    if (v32) v32[i] = (float)v;
+  }
+}
+
+}  // namespace
+
+int solve_rows_per_lane_group(double mean_row) {
+  return mean_row > 48 ? 16 : (mean_row > 12 ? 8 : 4);
+}
+
+void launch_spmv(const SolveMat &m, const void *val, bool val64, const void *x, bool x64, const void *b, void *y,
+                 bool y64, hipStream_t st) {
+  switch (m.lanes) {
+    case 4: launch_spmv_s<4>(m, val, val64, x, x64, b, y, y64, st); break;
+    case 8: launch_spmv_s<8>(m, val, val64, x, x64, b, y, y64, st); break;
+    default: launch_spmv_s<16>(m, val, val64, x, x64, b, y, y64, st); break;
+  }
+}
+
+void launch_dot(int64_t n, const void *a, const void *b, bool f64, double *partial, double *sc, int slot,
+                hipStream_t st) {
+  if (f64) dot_partial_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, (const double *)a, (const double *)b, partial);
+  else dot_partial_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, (const float *)a, (const float *)b, partial);
+  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial, sc, slot);
+}
+
+void launch_cg_start(int64_t n, double s, const double *r64, const CgVectors &v, double *partial, double *sc,
+                     hipStream_t st) {
+  if (v.f64)
+    cg_start_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, s, r64, (double *)v.d, (double *)v.r, (double *)v.z,
+                                                            (double *)v.p, (const double *)v.dinv, partial);
+  else
+    cg_start_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, s, r64, (float *)v.d, (float *)v.r, (float *)v.z,
+                                                           (float *)v.p, (const float *)v.dinv, partial);
+  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial, sc, 0);
+}
+
+void launch_cg_iteration(const SolveMat &m, const void *val, const CgVectors &v, double *partial, double *partial2,
+                         double *sc, hipStream_t st) {
+  const int64_t n = m.n;
+  launch_spmv(m, val, v.f64, v.p, v.f64, nullptr, v.q, v.f64, st);
+  if (v.f64) dot_partial_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, (const double *)v.p, (const double *)v.q, partial);
+  else dot_partial_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, (const float *)v.p, (const float *)v.q, partial);
+  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial, sc, 1);
+  alpha_kernel<<<1, 1, 0, st>>>(sc);
+  if (v.f64)
+    cg_update_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, sc, (double *)v.d, (double *)v.r, (double *)v.z,
+                                                             (const double *)v.p, (const double *)v.q,
+                                                             (const double *)v.dinv, partial, partial2);
+  else
+    cg_update_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, sc, (float *)v.d, (float *)v.r, (float *)v.z,
+                                                            (const float *)v.p, (const float *)v.q,
+                                                            (const float *)v.dinv, partial, partial2);
+  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial, sc, 3);
+  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial2, sc, 5);
+  beta_kernel<<<1, 1, 0, st>>>(sc);
+  if (v.f64) cg_direction_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, sc, (const double *)v.z, (double *)v.p);
+  else cg_direction_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, sc, (const float *)v.z, (float *)v.p);
+}
+
+void launch_refine(int64_t n, double s, const void *d, bool f64, double *x, hipStream_t st) {
+  if (f64) refine_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, s, (const double *)d, x);
+  else refine_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, s, (const float *)d, x);
+}
+
+void launch_diag_inv(const SolveMat &m, const double *val, void *dinv, bool f64, int *bad, hipStream_t st) {
+  const dim3 grid((unsigned)((m.n + kRB - 1) / kRB));
+  if (f64) diag_inv_kernel<double><<<grid, kRB, 0, st>>>(m.n, m.ptr, m.idx, val, (double *)dinv, bad);
+  else diag_inv_kernel<float><<<grid, kRB, 0, st>>>(m.n, m.ptr, m.idx, val, (float *)dinv, bad);
+}
+
+void launch_gather_values(int64_t n, const int32_t *perm, const double *a, double *v64, float *v32, hipStream_t st) {
+  gather_values_kernel<<<kSolvePartials, kRB, 0, st>>>(n, perm, a, v64, v32);
+}
+
+}  // namespace gtk

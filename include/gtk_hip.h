@@ -142,6 +142,33 @@ int gtk_sync(gtk_ctx *ctx);
  * context stream around n launches). */
 int gtk_time_matrix(gtk_ctx *ctx, int form, int precision, int rows, int cols, int launches, double *ms_per_launch);
 
+typedef struct gtk_solve_stats {
+  int outer_iterations;  /* FP64 refinement steps */
+  int inner_iterations;  /* conjugate-gradient iterations, all steps */
+  int converged;         /* 1 if the FP64 relative residual reached rtol */
+  double rel_residual;   /* |b - A x| / |b| in FP64 at exit */
+  double solve_ms;       /* wall time of the call */
+} gtk_solve_stats;
+
+/* Solve A x = b for the (GTK_FREE, GTK_FREE) block with the values last
+ * assembled (they stay on the device), A positive definite from a
+ * symmetric form (Laplace, mass, elasticity): its pattern is structurally
+ * symmetric, its values symmetric up to the assembly's rounding (the solve
+ * uses the values as assembled, in row order).  x: initial guess on input,
+ * solution on output (host arrays of n_free).
+ *  - GTK_MIXED: FP64 iterative refinement; each step solves A d = r/|r| with
+ *    an FP32 Jacobi-preconditioned conjugate gradient to relative residual
+ *    inner_rtol (at most max_inner iterations), then x += |r| d in FP64 and
+ *    r = b - A x is recomputed in FP64; stops when |r| <= rtol |b| or after
+ *    max_outer steps.
+ *  - GTK_FP64: one FP64 Jacobi-preconditioned conjugate gradient to rtol
+ *    (max_inner iterations), refined as above.
+ * Returns GTK_OK also when rtol was not reached (stats->converged says).
+ * Replaces: the linear solve of the assembled system,
+ * PartitionedSolvers_linear_problem + solve (src/problems.jl:439-462). */
+int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rtol, int max_outer, int max_inner,
+              double inner_rtol, gtk_solve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,11 +10,13 @@
 #include "gtk_plan.h"
 
 // Replays the batch records exactly as the matrix-core kernel consumes
-// them: per batch, slots decoded from their words, rows added through the
-// record's offsets into a batch accumulator in rank steps t < tmax, and the
-// columns' segments copied out.  Also checks that each column's
-// contributors are ranked in ascending cell order and that every nonzero
-// is written exactly once.
+// them: per batch, in rank steps t < tmax, every slot of rank t adds its rows
+// into the batch accumulator through the record's row offsets (the replay
+// checks that no two additions of one step touch the same entry); then each
+// accumulator entry e is written to e + delta[c], c = number of column ends
+// before e in the record's end mask.  Also checks that each column's
+// contributors are ranked in ascending cell order and that every nonzero is
+// written exactly once.
 static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const gtk::TilePlan &t, double *nzval,
                           char *err, int errlen) {
   auto fail = [&](const char *msg) {
@@ -36,41 +38,60 @@ static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const g
       const uint32_t info = reinterpret_cast<const uint32_t *>(R)[0];
       const int nslots = info & 0xFF, tmax = (info >> 8) & 0xFF, ncols = info >> 16;
       const int acc_total = (int)reinterpret_cast<const uint32_t *>(R)[1];
-      const uint32_t *slot = reinterpret_cast<const uint32_t *>(R + 16);
-      const int32_t *seg0 = reinterpret_cast<const int32_t *>(R + 80);
-      const uint32_t *la = reinterpret_cast<const uint32_t *>(R + 144);
-      const uint8_t *off = R + gtk::BatchPlan::kHeader;
+      const uint32_t *sb = reinterpret_cast<const uint32_t *>(R + gtk::BatchPlan::kSlotSb);
+      const uint32_t *at = reinterpret_cast<const uint32_t *>(R + gtk::BatchPlan::kSlotAt);
+      const int32_t *delta = reinterpret_cast<const int32_t *>(R + gtk::BatchPlan::kDelta);
+      const uint64_t *endmask = reinterpret_cast<const uint64_t *>(R + gtk::BatchPlan::kEndmask);
+      const uint8_t *off = R + bp.off_offset;
       if (nslots > gtk::kBatchSlots || ncols > 16 || ncols < 1 || acc_total > bp.max_batch_acc)
         return fail("batch header out of range");
       std::vector<double> acc(acc_total, 0.0);
-      std::vector<int> last_cell(ncols, -1), count(ncols, 0);
+      std::vector<int> stamp(acc_total, -1);
+      std::vector<int> last_cell(16, -1);
+      int seen = 0;
       for (int tt = 0; tt < tmax; tt++)
-        for (int n = 0; n < nslots; n++) {
-          const uint32_t w = slot[n];
-          const int s = w & 0xFFF, lb = (w >> 12) & 0xFF, tn = (w >> 20) & 0xF, ci = w >> 24;
+        for (int n = 0; n < gtk::kBatchSlots; n++) {
+          const int tn = at[n] >> 16, abase = at[n] & 0xFFFF;
           if (tn != tt) continue;
-          if (s >= ncell || lb >= nd || ci >= ncols) return fail("slot word out of range");
+          if (n >= nslots) return fail("empty slot with a rank");
+          const int s = sb[n] & 0xFFFF, lb = sb[n] >> 16;
+          if (s >= ncell || lb >= nd) return fail("slot word out of range");
+          // the slot's column: the one whose accumulator range starts at abase
+          int ci = -1;
+          for (int c = 0, a0 = 0, k = 0; c < ncols; c++) {
+            if (a0 == abase) ci = c;
+            int len = 0;
+            while (k < acc_total && !((endmask[k / 64] >> (k % 64)) & 1)) k++, len++;
+            a0 += len + 1;
+            k++;
+          }
+          if (ci < 0) return fail("slot column not found");
           if (cells[s] <= last_cell[ci]) return fail("contributors not in ascending cell order");
           last_cell[ci] = cells[s];
-          count[ci]++;
-          const int len = la[ci] & 0xFFFF, abase = la[ci] >> 16;
-          for (int a = 0; a < bp.off_stride; a++) {
-            const size_t pos = (size_t)n * bp.off_stride + a;
+          seen++;
+          for (int a = 0; a < nd; a++) {
+            const size_t pos = (size_t)a * gtk::kBatchSlots + n;
             const uint32_t o = bp.off_bytes == 1 ? off[pos] : reinterpret_cast<const uint16_t *>(off)[pos];
             if (o == none) continue;
-            if (a >= nd || (int)o >= len || abase + (int)o >= acc_total) return fail("offset out of range");
-            acc[abase + o] += 1.0 + cells[s] * 7.0 + a * 131.0 + lb * 977.0;
+            const int ent = abase + (int)o;
+            if (ent >= acc_total) return fail("offset out of range");
+            if (stamp[ent] == tt) return fail("two additions of one rank step touch one entry");
+            stamp[ent] = tt;
+            acc[ent] += 1.0 + cells[s] * 7.0 + a * 131.0 + lb * 977.0;
           }
         }
-      for (int ci = 0; ci < ncols; ci++) {
-        const int len = la[ci] & 0xFFFF, abase = la[ci] >> 16;
-        if (count[ci] == 0) return fail("column without contributors");
-        for (int r = 0; r < len; r++) {
-          const int64_t z = seg0[ci] + r;
-          if (z < 0 || z >= p.nnz) return fail("segment out of range");
-          nzval[z] = acc[abase + r];
-          written[z]++;
-        }
+      if (seen != nslots) return fail("slots without a rank step");
+      int ends = 0;
+      for (int k = 0; k < acc_total; k++) ends += (endmask[k / 64] >> (k % 64)) & 1;
+      if (ends != ncols || !((endmask[(acc_total - 1) / 64] >> ((acc_total - 1) % 64)) & 1))
+        return fail("end mask does not close every column");
+      int c = 0;
+      for (int k = 0; k < acc_total; k++) {
+        const int64_t z = (int64_t)k + delta[c];
+        if (z < 0 || z >= p.nnz) return fail("segment out of range");
+        nzval[z] = acc[k];
+        written[z]++;
+        c += (endmask[k / 64] >> (k % 64)) & 1;
       }
     }
   }
@@ -143,5 +164,24 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
       for (int r = 0; r < len; r++) nzval[p.colptr[j] - 1 + r] = acc[r];
     }
   }
+  return 0;
+}
+
+// CSR row order of a square CSC pattern (build_transpose_perm); returns 0,
+// or -1 with the error text.
+extern "C" int harness_transpose(int64_t n, const int32_t *colptr, const int32_t *rowval, int32_t *perm, char *err,
+                                 int errlen) {
+  gtk::Pattern p;
+  p.nrows = p.ncols = n;
+  p.nnz = colptr[n] - 1;
+  p.colptr.assign(colptr, colptr + n + 1);
+  p.rowval.assign(rowval, rowval + p.nnz);
+  std::vector<int32_t> out;
+  std::string e = gtk::build_transpose_perm(p, out);
+  if (!e.empty()) {
+    std::strncpy(err, e.c_str(), errlen - 1);
+    return -1;
+  }
+  std::memcpy(perm, out.data(), out.size() * sizeof(int32_t));
   return 0;
 }

@@ -93,3 +93,25 @@ def test_dirichlet_columns(GT, O):
     cp, rv, nz = p.assemble_matrix(O.LAPLACE, rows=O.FREE, cols=O.DIRICHLET)
     assert np.array_equal(Ad.colptr, cp) and np.array_equal(Ad.rowval, rv)
     assert relfro(Ad.nzval, nz) <= TOL_FP64
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("mixed", TOL_MIXED)])
+def test_elasticity_p2_tets(GT, O, precision, tol):
+    """P2 vector tetrahedra, per-cell Lame parameters, clamped on x = 0
+    (group 2-face-5), body force (0, 0, -1)."""
+    print(f"case elasticity p2 tets {precision}", flush=True)
+    cells = (3, 2, 3)
+    m, Om, V, p = build(GT, O, cells, 2, simplex=True, diri=2, group=5, ncomp=3, perturb=0.1)
+    dO = GT.measure(Om, 5)
+    p.quadrature(5)
+    nc = m.cells.shape[0]
+    lam = 0.5 + 1.5 * GT.inputs.uniform(31, nc)
+    mu = 0.5 + 1.5 * GT.inputs.uniform(32, nc)
+    form = GT.Elasticity(dO, GT.face_constant_field(lam, Om), GT.face_constant_field(mu, Om))
+    A = GT.assemble_matrix(form, float, V, V, precision=GT.FP64 if precision == "fp64" else GT.MIXED)
+    cp, rv, nz = p.assemble_matrix(O.ELASTICITY, coef=np.stack([lam, mu], axis=1).ravel())
+    assert np.array_equal(A.colptr, cp) and np.array_equal(A.rowval, rv)
+    assert relfro(A.nzval, nz) <= tol
+    b = GT.assemble_vector(GT.Load(dO, (0.0, 0.0, -1.0)), float, V)
+    bo = p.assemble_vector(0, [0.0, 0.0, -1.0])
+    assert relfro(b, bo) <= TOL_FP64

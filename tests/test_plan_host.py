@@ -101,3 +101,37 @@ def test_batches_cover_hex_q2(harness, GT):
     """Q2 hexahedra: every column has at most 8 contributors, so the batch
     records apply (the replay must not report them as inapplicable)."""
     assert run(harness, GT, (4, 3, 3), 2, diri=1, max_cells=180, chunk=32, mode=1) is not None
+
+
+def test_transpose_perm(harness, GT):
+    """Row-order permutation of a square CSC block (the solver's CSR view):
+    values gathered through it equal scipy's CSR data, and a pattern that is
+    not structurally symmetric is rejected."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    m = GT.cartesian_mesh((0.0, 1.0) * 2, (5, 4))
+    V = GT.lagrange_space(GT.interior(m), 2, dirichlet_boundary=GT.boundary(m))
+    ref = coo_reference(V.cell_dofs, 1, 1, V.nfree, V.nfree)
+    ref.data = rng.standard_normal(ref.nnz)  # values need not be symmetric
+    colptr = (ref.indptr + 1).astype(np.int32)
+    rowval = (ref.indices + 1).astype(np.int32)
+    perm = np.zeros(ref.nnz, np.int32)
+    err = C.create_string_buffer(128)
+    H = harness
+    H.harness_transpose.argtypes = [C.c_int64, np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                    np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                    np.ctypeslib.ndpointer(np.int32, flags="C"), C.c_char_p, C.c_int]
+    assert H.harness_transpose(V.nfree, colptr, rowval, perm, err, 128) == 0, err.value
+    csr = ref.tocsr()
+    csr.sort_indices()
+    assert np.array_equal(ref.data[perm], csr.data)
+    assert np.array_equal(csr.indices + 1, rowval) and np.array_equal(csr.indptr + 1, colptr)
+    # drop one off-diagonal entry: no longer structurally symmetric
+    A = ref.tolil()
+    i, j = 0, int(ref.indices[ref.indptr[0] + 1])
+    A[j, i] = 0.0
+    B = sp.csc_matrix(A)
+    B.eliminate_zeros()
+    cp = (B.indptr + 1).astype(np.int32)
+    rv = (B.indices + 1).astype(np.int32)
+    assert H.harness_transpose(V.nfree, cp, rv, np.zeros(B.nnz + 1, np.int32), err, 128) != 0
