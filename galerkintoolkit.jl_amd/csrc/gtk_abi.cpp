@@ -102,7 +102,10 @@ struct gtk_ctx {
   int ncoef = 0;
   // device copies
   DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef;
-  int waves = 8;  // wavefronts per workgroup of the matrix-core kernels
+  // wavefronts per workgroup of the matrix-core kernels, per precision
+  // (FP64: 8, two per SIMD within 256 registers; mixed: 12, three per SIMD);
+  // 0 = the default, GTK_WAVES overrides both
+  int waves = 0;
   size_t lds_budget = 156 * 1024;  // LDS a workgroup may use (less gives several workgroups per CU)
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
   std::map<int, std::unique_ptr<VectorPlan>> vplans;
@@ -183,6 +186,10 @@ static constexpr size_t kLdsBudget = 156 * 1024;
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
+// doubles past the values that receive the padding entries of a batch
+// write-out (at most 16 rounds of 64)
+static constexpr size_t kValueTail = 16 * 64;
+
 // chunk: cells whose owned DOFs form a tile; about (c^(1/D)+1)^D cells are
 // then needed (more around simplices)
 static int tile_chunk(const gtk_ctx *ctx, int max_cells) {
@@ -254,7 +261,8 @@ int gtk_create(int device, gtk_ctx **out) {
   ctx->device = device;
   // matrix-core workgroup shape and LDS budget (tuning overrides)
   if (const char *w = std::getenv("GTK_WAVES")) {
-    if (std::strcmp(w, "4") == 0) ctx->waves = 4;
+    const long v = std::strtol(w, nullptr, 10);
+    if (v == 8 || v == 12 || v == 16) ctx->waves = (int)v;
   }
   if (const char *l = std::getenv("GTK_LDS_KB")) {
     const long kb = std::strtol(l, nullptr, 10);
@@ -430,7 +438,8 @@ int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *n
     plan->max_seg = std::max(plan->max_seg, plan->pat.colptr[j + 1] - plan->pat.colptr[j]);
   int rc;
   if ((rc = upload(ctx, plan->colptr, plan->pat.colptr.data(), plan->pat.colptr.size()))) return rc;
-  if ((rc = alloc(ctx, plan->nzval, (size_t)std::max<int64_t>(1, plan->pat.nnz) * sizeof(double)))) return rc;
+  // values plus a tail that absorbs the padding entries of the batch write-out
+  if ((rc = alloc(ctx, plan->nzval, ((size_t)plan->pat.nnz + kValueTail) * sizeof(double)))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   plan->valid = true;
   if (nnz) *nnz = plan->pat.nnz;
@@ -468,10 +477,11 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
   const gtk::ElemKey key_e = elem_key(ctx);
   const int nd = ctx->dofs_per_cell;
   std::string e;
-  bool mfma = gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, ctx->waves, ep->k) && ep->k.mfma;
+  const int waves = ctx->waves ? ctx->waves : (precision == GTK_FP64 ? 8 : 12);
+  bool mfma = gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, waves, ep->k) && ep->k.mfma;
   if (mfma) {
     // accumulator bound: a batch adds at most nd rows per slot
-    const size_t acc_bound = (size_t)ctx->waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
+    const size_t acc_bound = (size_t)waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
     const int max_cells = tile_cells_for(ep->k, acc_bound, ctx->lds_budget);
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
@@ -483,19 +493,22 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
     if (ep->bat.endmask_words > 16) {  // the kernel reads at most 16 end-mask words
       ep->bat = gtk::BatchPlan();
       mfma = false;
+    } else if (ep->bat.rounds != ep->k.rounds) {
+      FAIL(ctx, GTK_EINVAL, "batch records and kernel disagree on the write-out rounds");
     }
   }
   if (mfma) {
     ep->lds_geo = (int)align16(ep->k.table_bytes);
     ep->acc_offset = (int)align16(ep->lds_geo + (size_t)ep->tiles.max_tile_cells * ep->k.geo_bytes_per_cell);
-    ep->acc_per_wave = (ep->bat.max_batch_acc + 64 + 1) & ~1;
-    const size_t acc_bytes = (size_t)ctx->waves * ep->acc_per_wave * sizeof(double);
+    // batch entries + a private sink per lane; at least the write-out rounds
+    ep->acc_per_wave = (std::max(ep->bat.max_batch_acc + 64, 64 * ep->bat.rounds) + 1) & ~1;
+    const size_t acc_bytes = (size_t)waves * ep->acc_per_wave * sizeof(double);
     // coordinates and coefficient of a staged cell
     ep->stage_cells = (int)std::max<size_t>(1, acc_bytes / (((size_t)ctx->nv * ctx->dim + 1) * sizeof(double)));
     ep->lds_claim = (int)(ep->acc_offset + acc_bytes);
     ep->lds = ep->lds_claim + 16;
   } else {
-    if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, ctx->waves, ep->k))
+    if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, waves, ep->k))
       FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
     const size_t waves = (size_t)ep->k.block / 64;
     const size_t acc_bytes = waves * ep->k.cols_per_wave * std::max(1, plan.max_seg) * sizeof(double);

@@ -55,6 +55,7 @@ struct KernelInfo {
   size_t geo_bytes_per_cell = 0;  // LDS bytes of staged geometry per cell
   int cols_per_wave = 1;
   bool mfma = false;  // matrix-core kernel driven by batch records
+  int rounds = 0;     // write-out rounds of 64 accumulator entries per batch (matrix-core kernels)
   int block = kBlock;
 };
 

@@ -309,6 +309,17 @@ __device__ inline void lds_load(const CT *__restrict__ p, CT (&out)[N]) {
 }
 
 constexpr int kBatchSlotsK = 16;  // contributor slots per matrix-core batch (MFMA N)
+// Per-cell stride (in CT elements) of the staged geometry factors: the
+// NQP x NGP block rounded up to an odd number of the units the batch loop
+// reads (8 bytes for FP32 rows, 16 bytes for FP64 rows), so that the 16
+// slots of a batch, which read 16 different cells, spread over the LDS banks.
+template <int NQP, int NGP, typename CT>
+struct GeoStride {
+  static constexpr int unit = (NGP * (int)sizeof(CT)) % 16 == 0 ? 16 : ((NGP * (int)sizeof(CT)) % 8 == 0 ? 8 : 4);
+  static constexpr int units = (NQP * NGP * (int)sizeof(CT) + unit - 1) / unit;
+  static constexpr int value = ((units | 1) * unit) / (int)sizeof(CT);
+};
+
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -384,6 +395,8 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   constexpr int KD = FORM == kLaplace ? D : 1;
   constexpr int TP = FORM == kLaplace ? (D == 3 ? 4 : 2) : 1;  // table entries per (q, b)
   constexpr int QG = (NQ + 3) / 4;
+  constexpr int NQP = 4 * QG;  // quadrature rows padded with zero rows to the four lane groups
+  constexpr int GS = GeoStride<NQP, NGP, CT>::value;  // CT elements per cell of staged factors
   constexpr int KS = QG * KD;
   constexpr int MB = (NS + 15) / 16;
   constexpr int CPR = kB / NQ;  // cells per geometry round
@@ -393,8 +406,8 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   using O4 = Off4<OffT>;
   static_assert(CPR >= 1, "block too small for the quadrature");
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  CT *tab_s = reinterpret_cast<CT *>(lds_raw);                     // [NQ][NS][TP]
-  CT *G_s = reinterpret_cast<CT *>(lds_raw + args.lds_geo);        // [cells][NQ][NGP]
+  CT *tab_s = reinterpret_cast<CT *>(lds_raw);                     // [NQP][NS][TP]
+  CT *G_s = reinterpret_cast<CT *>(lds_raw + args.lds_geo);        // [cells: GS][NQP][NGP]
   double *acc_all = reinterpret_cast<double *>(lds_raw + args.acc_offset);
   double *X_s = acc_all;  // phase A: [stage cells][NV][D]
   int *claim_ctr = reinterpret_cast<int *>(lds_raw + args.lds_claim);
@@ -405,11 +418,13 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   const int cell0 = args.tile_cell_ptr[tile];
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
 
-  for (int i = tid; i < NQ * NS * TP; i += kB) {
+  for (int i = tid; i < NQP * NS * TP; i += kB) {
     const int q = i / (NS * TP), rem = i - q * NS * TP, b = rem / TP, d = rem - b * TP;
-    double v;
-    if constexpr (FORM == kLaplace) v = d < D ? args.gphi[(q * D + d) * NS + b] : 0.0;
-    else v = args.phi[q * NS + b];
+    double v = 0.0;
+    if (q < NQ) {
+      if constexpr (FORM == kLaplace) v = d < D ? args.gphi[(q * D + d) * NS + b] : 0.0;
+      else v = args.phi[q * NS + b];
+    }
     tab_s[i] = (CT)v;
   }
 
@@ -469,7 +484,13 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
           double Ji[D][D];
           const double det = inverse<D>(J, Ji);
           const double kdV = K_s[sl] * fabs(det) * wq;
-          CT *G = G_s + ((c0 + sl) * NQ + tq) * NGP;
+          CT *G = G_s + (c0 + sl) * GS + tq * NGP;
+          if constexpr (NQP > NQ) {  // padding rows of the cell: zero
+            if (tq < NQP - NQ) {
+#pragma unroll
+              for (int i = 0; i < NGP; i++) G_s[(c0 + sl) * GS + (NQ + tq) * NGP + i] = (CT)0;
+            }
+          }
           if constexpr (FORM == kLaplace) {
             CT gv[NGP];
 #pragma unroll
@@ -529,24 +550,20 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
       bn = claim();
     }
     const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
-    const int acc_total = (int)__builtin_amdgcn_readfirstlane(cur.acc);
-    const int nslots = info & 0xFF, tmax = (info >> 8) & 0xFF;
-    // W column of slot a_row(nl)
-    const bool wvalid = a_row<CT>(nl) < nslots;
+    const int tmax = (info >> 8) & 0xFF;
+    // W column of slot a_row(nl); empty slots read slot 0 (finite values,
+    // never scattered) and padding rows are zero in both operands
     const int s = cur.sb & 0xFFFF, b = cur.sb >> 16;
-    const CT *Gc = G_s + s * NQ * NGP;
-    const CT *Tb = tab_s + b * TP;
+    const CT *Gc = G_s + s * GS + g * QG * NGP;
+    const CT *Tb = tab_s + (g * QG * NS + b) * TP;
     V4 c[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < QG; j++) {
-      const int q = g * QG + j;
-      const int qq = q < NQ ? q : NQ - 1;
       CT G[NGP], T[TP];
-      lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + qq * NGP, G);
-      lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + qq * NS * TP, T);
-      const bool on = wvalid && q < NQ;
+      lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
+      lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
       CT W[KD];
       if constexpr (FORM == kLaplace) {
         if constexpr (D == 3) {
@@ -562,9 +579,8 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
       }
 #pragma unroll
       for (int d = 0; d < KD; d++) {
-        const CT wv = on ? W[d] : (CT)0;
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(wv, bfrag[mb][j * KD + d], c[mb]);
+        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
       }
     }
     // destinations of the 4 slots x MB rows this lane holds; rows outside
@@ -592,34 +608,27 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
         }
       }
     }
-    // flat write-out of the batch's segments in KR rounds of 64 entries
-    // (KR covers the largest batch, 16 slots x NS rows): all reads first,
-    // then the clears, then the stores.  Every round stores from every lane
-    // (lanes past the batch's entries store the last entry again, with its
-    // value), so each batch issues the same number of stores and the next
-    // batch's prefetched record is waited for without waiting for them.
+    // flat write-out of the batch's KR rounds of 64 accumulator entries:
+    // entry e goes to e + delta[c], c = column ends before e (the padding
+    // column sends the entries past the batch's columns to the value array's
+    // tail); all reads first, then the clears, then the stores, so every
+    // batch issues the same number of stores and the next batch's
+    // prefetched record is waited for without waiting for them
     {
       int ends = 0;
-      const int ncols = (int)(info >> 16);
-      const int dlast = __shfl(cur.delta, ncols - 1);
       double v[KR];
       int32_t z[KR];
 #pragma unroll
       for (int k = 0; k < KR; k++) {
-        const int e = k * 64 + lane;
-        const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);
+        const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);  // k < 16: end-mask words
         const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
         const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
         ends += __builtin_popcount(lo) + __builtin_popcount(hi);
-        const int dlt = __shfl(cur.delta, cidx & 15);
-        const bool in = e < acc_total;
-        const int ee = in ? e : acc_total - 1;
-        v[k] = acc[ee];
-        z[k] = ee + (in ? dlt : dlast);
+        v[k] = acc[k * 64 + lane];
+        z[k] = k * 64 + lane + __shfl(cur.delta, cidx);
       }
 #pragma unroll
-      for (int k = 0; k < KR; k++)
-        if (k * 64 + lane < acc_total) acc[k * 64 + lane] = 0.0;
+      for (int k = 0; k < KR; k++) acc[k * 64 + lane] = 0.0;
 #pragma unroll
       for (int k = 0; k < KR; k++) args.nzval[(int64_t)z[k]] = v[k];
     }
@@ -695,21 +704,30 @@ static KernelInfo matrix_batch_info() {
   constexpr int NG = NGeo<E::d, FORM>::value;
   constexpr int NGP = NG == 3 ? 4 : NG;
   constexpr int TP = FORM == kLaplace ? (E::d == 3 ? 4 : 2) : 1;
+  constexpr int NQP = 4 * ((E::nq + 3) / 4);
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&matrix_batch_kernel<E::d, E::nv, E::ns, E::nq, FORM, CT, OffT, NW>);
-  k.table_bytes = (size_t)E::nq * E::ns * TP * sizeof(CT);
-  k.geo_bytes_per_cell = (size_t)E::nq * NGP * sizeof(CT);
+  k.table_bytes = (size_t)NQP * E::ns * TP * sizeof(CT);
+  k.geo_bytes_per_cell = (size_t)GeoStride<NQP, NGP, CT>::value * sizeof(CT);
+  k.rounds = (kBatchSlotsK * E::ns + 63) / 64;
   k.cols_per_wave = 16;
   k.mfma = true;
   k.block = NW * 64;
   return k;
 }
 
+template <typename E, int FORM, typename CT, int NW>
+static KernelInfo matrix_batch_info(int off_bytes) {
+  return off_bytes == 1 ? matrix_batch_info<E, FORM, CT, uint8_t, NW>() : matrix_batch_info<E, FORM, CT, uint16_t, NW>();
+}
+
+// waves per workgroup: 8 (2 per SIMD), 12 or 16 (3 or 4 per SIMD, registers
+// limited to 168 or 128 by the launch bounds)
 template <typename E, int FORM, typename CT>
 static KernelInfo matrix_batch_info(int off_bytes, int waves) {
-  if (waves == 4)
-    return off_bytes == 1 ? matrix_batch_info<E, FORM, CT, uint8_t, 4>() : matrix_batch_info<E, FORM, CT, uint16_t, 4>();
-  return off_bytes == 1 ? matrix_batch_info<E, FORM, CT, uint8_t, 8>() : matrix_batch_info<E, FORM, CT, uint16_t, 8>();
+  if (waves == 16) return matrix_batch_info<E, FORM, CT, 16>(off_bytes);
+  if (waves == 12) return matrix_batch_info<E, FORM, CT, 12>(off_bytes);
+  return matrix_batch_info<E, FORM, CT, 8>(off_bytes);
 }
 
 template <typename E, int FORM, typename CT>

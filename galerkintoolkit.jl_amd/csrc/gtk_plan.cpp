@@ -225,7 +225,7 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
       int slots = kBatchSlots + 1, ncolb = 0;
       for (size_t i = 0; i < cur_cols.size(); i++) {
         const int cnt = (int)(inc.ptr[cur_cols[i] + 1] - inc.ptr[cur_cols[i]]);
-        if (slots + cnt > kBatchSlots) {
+        if (slots + cnt > kBatchSlots || ncolb == kBatchCols) {
           out.batch_col.push_back((int32_t)(out.col_id.size() + i));
           slots = 0;
           ncolb = 0;
@@ -321,7 +321,9 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
     }
   if (out.nbatches > 0) out.max_batch_acc = *std::max_element(bacc.begin(), bacc.end());
   if (out.max_batch_acc >= 0xFFFF) return "batch accumulator exceeds 16-bit offsets";
-  out.endmask_words = std::max(1, (out.max_batch_acc + 63) / 64);
+  out.rounds = (nd + 3) / 4;  // 16 slots x nd rows fill at most this many rounds of 64
+  if (out.max_batch_acc > 64 * out.rounds) return "batch accumulator exceeds its write-out rounds";
+  out.endmask_words = out.rounds;
   out.off_offset = BatchPlan::kEndmask + 8 * out.endmask_words;
   const int off_row = kBatchSlots * out.off_bytes;  // bytes per local row
   out.rec_bytes = (out.off_offset + nd * off_row + 15) / 16 * 16;
@@ -329,6 +331,8 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
   std::vector<int32_t> batch_tile(std::max<int64_t>(0, out.nbatches));
   for (int tl = 0; tl < t.ntiles; tl++)
     for (int32_t b = t.tile_batch_ptr[tl]; b < t.tile_batch_ptr[tl + 1]; b++) batch_tile[b] = tl;
+  for (int64_t b = 0; b < out.nbatches; b++)
+    if (t.batch_col[b + 1] - t.batch_col[b] > kBatchCols) return "batch with too many columns";
   parallel_for(out.nbatches, nthreads, [&](int, int64_t b0, int64_t b1) {
     for (int64_t b = b0; b < b1; b++) {
       uint8_t *r = out.rec.data() + (size_t)b * out.rec_bytes;
@@ -366,6 +370,10 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
           }
         }
         abase += len;
+      }
+      if (abase < 64 * out.rounds) {  // padding column up to the last round
+        delta[ncols] = (int32_t)(p.nnz - abase);
+        endmask[out.rounds - 1] |= 1ull << 63;
       }
       reinterpret_cast<uint32_t *>(r)[0] = (uint32_t)ns | (uint32_t)tmax << 8 | (uint32_t)ncols << 16;
       reinterpret_cast<uint32_t *>(r)[1] = (uint32_t)abase;

@@ -10,6 +10,7 @@ namespace gtk {
 constexpr uint16_t kNoRow = 0xFFFF;  // contributor row not in the row set
 constexpr int kColGroup = 16;        // columns batched together by the MFMA kernels
 constexpr int kBatchSlots = 16;      // contributors per matrix-core batch (16 = MFMA N)
+constexpr int kBatchCols = 15;       // columns per batch (the 16th column entry is the padding column)
 
 // CSC pattern of one (rows, cols) block, 1-based like SparseMatrixCSC.
 struct Pattern {
@@ -71,8 +72,10 @@ struct OffsetPlan {
 //   u32  slot_at[16] abase | t << 16      (offset of the slot's column in the batch
 //                    accumulator, rank t; t = 0xFFFF for empty slots)
 //   i32  delta[16]   seg0 - abase of column ci (seg0: 0-based start of the column's
-//                    segment in the value array)
-//   u64  endmask[endmask_words]  bit e set iff accumulator entry e ends a column
+//                    segment in the value array); after the ncols columns, a padding
+//                    column fills the accumulator up to 64*rounds entries and maps
+//                    them past the values (to the value array's tail, nnz onwards)
+//   u64  endmask[rounds]  bit e set iff accumulator entry e ends a column
 //   off  [dofs per cell][16]  row offset of local row a of each slot in its column
 //                    segment (off_bytes each), all ones for rows outside the row set
 //                    and for empty slots
@@ -81,7 +84,8 @@ struct BatchPlan {
   int64_t nbatches = 0;
   int rec_bytes = 0;
   int off_bytes = 1;
-  int endmask_words = 0;
+  int rounds = 0;         // write-out rounds of 64 entries: ceil(dofs per cell / 4)
+  int endmask_words = 0;  // = rounds
   int off_offset = 0;  // byte offset of the row offsets in a record
   int max_batch_acc = 0;
   static constexpr int kSlotSb = 16, kSlotAt = 80, kDelta = 144, kEndmask = 208;

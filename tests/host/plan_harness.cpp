@@ -85,12 +85,19 @@ static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const g
       for (int k = 0; k < acc_total; k++) ends += (endmask[k / 64] >> (k % 64)) & 1;
       if (ends != ncols || !((endmask[(acc_total - 1) / 64] >> ((acc_total - 1) % 64)) & 1))
         return fail("end mask does not close every column");
+      if (ncols > gtk::kBatchCols || bp.rounds != (nd + 3) / 4) return fail("batch shape");
+      // the kernel's write-out: 64*rounds entries, padding entries go past nnz
       int c = 0;
-      for (int k = 0; k < acc_total; k++) {
+      for (int k = 0; k < 64 * bp.rounds; k++) {
+        if (c > 15) return fail("column index past the delta table");
         const int64_t z = (int64_t)k + delta[c];
-        if (z < 0 || z >= p.nnz) return fail("segment out of range");
-        nzval[z] = acc[k];
-        written[z]++;
+        if (k < acc_total) {
+          if (z < 0 || z >= p.nnz) return fail("segment out of range");
+          nzval[z] = acc[k];
+          written[z]++;
+        } else if (z < p.nnz || z >= p.nnz + 64 * bp.rounds) {
+          return fail("padding entry outside the value array's tail");
+        }
         c += (endmask[k / 64] >> (k % 64)) & 1;
       }
     }
