@@ -53,6 +53,7 @@ struct ExecPlan {
   DevBuf tile_cell_ptr, tile_cells, tile_col_ptr, col_id, col_ctr_ptr, ctr_slot, ctr_b, ctr_off, tile_batch_ptr, rec;
   size_t lds = 0;
   int lds_geo = 0, acc_offset = 0, acc_per_wave = 0, stage_cells = 0, lds_claim = 0;
+  int resident = 1;  // workgroups of the kernel resident on the device at once
 };
 
 // Device work space of the solve on a (FREE, FREE) block.
@@ -539,6 +540,12 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
   }
   if (ep->lds > 64 * 1024)
     HIPCHK(ctx, hipFuncSetAttribute(ep->k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ep->lds));
+  if (ep->k.mfma) {
+    int per_cu = 0, cus = 0;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ep->k.fn, ep->k.block, ep->lds));
+    HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    ep->resident = std::max(1, per_cu) * std::max(1, cus);
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   out = ep.get();
   plan.exec[key] = std::move(ep);
@@ -590,8 +597,12 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   a.acc_per_wave = ep->acc_per_wave;
   a.stage_cells = ep->stage_cells;
   a.lds_claim = ep->lds_claim;
+  a.ntiles = ep->tiles.ntiles;
   void *args[] = {&a};
-  HIPCHK(ctx, hipLaunchKernel(ep->k.fn, dim3(ep->tiles.ntiles), dim3(ep->k.block), args, ep->lds, ctx->stream));
+  // matrix-core kernels are persistent: as many workgroups as fit at once,
+  // each looping over tiles
+  const int grid = ep->k.mfma ? std::min(ep->tiles.ntiles, ep->resident) : ep->tiles.ntiles;
+  HIPCHK(ctx, hipLaunchKernel(ep->k.fn, dim3(grid), dim3(ep->k.block), args, ep->lds, ctx->stream));
   return GTK_OK;
 }
 

@@ -31,6 +31,7 @@ struct MatrixArgs {
   int acc_per_wave;  // FP64 accumulator slots per wave (matrix-core kernels)
   int stage_cells;   // cells whose coordinates fit the accumulator space (matrix-core kernels)
   int lds_claim;     // byte offset in LDS of the batch counter (matrix-core kernels)
+  int ntiles;        // tiles of the plan (matrix-core kernels loop over them)
 };
 
 struct VectorArgs {

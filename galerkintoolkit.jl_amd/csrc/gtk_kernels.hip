@@ -401,6 +401,7 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   constexpr int MB = (NS + 15) / 16;
   constexpr int CPR = kB / NQ;  // cells per geometry round
   constexpr int KR = (kBatchSlotsK * NS + 63) / 64;  // write-out rounds of the largest batch
+  constexpr int PF = 2;  // staging entries per thread prefetched for the next tile
   static_assert(KR <= 16, "end mask words are read from lanes 0..15");
   using V4 = typename Acc4<CT>::type;
   using O4 = Off4<OffT>;
@@ -409,16 +410,12 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   CT *tab_s = reinterpret_cast<CT *>(lds_raw);                     // [NQP][NS][TP]
   CT *G_s = reinterpret_cast<CT *>(lds_raw + args.lds_geo);        // [cells: GS][NQP][NGP]
   double *acc_all = reinterpret_cast<double *>(lds_raw + args.acc_offset);
-  double *X_s = acc_all;  // phase A: [stage cells][NV][D]
+  double *X_s = acc_all;                                            // phase A: [stage cells][NV][D]
+  double *K_s = X_s + (size_t)args.stage_cells * NV * D;            // phase A: coefficient per staged cell
   int *claim_ctr = reinterpret_cast<int *>(lds_raw + args.lds_claim);
 
-  const int tid = threadIdx.x;
-  if (tid == 0) *claim_ctr = NW;  // read after the geometry phase's barriers
-  const int tile = blockIdx.x;
-  const int cell0 = args.tile_cell_ptr[tile];
-  const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
-
-  for (int i = tid; i < NQP * NS * TP; i += kB) {
+  // once per workgroup: the reference table
+  for (int i = threadIdx.x; i < NQP * NS * TP; i += kB) {
     const int q = i / (NS * TP), rem = i - q * NS * TP, b = rem / TP, d = rem - b * TP;
     double v = 0.0;
     if (q < NQ) {
@@ -428,10 +425,60 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
     tab_s[i] = (CT)v;
   }
 
-  // phase A: geometry
-  {
+  // the next tile's first staging entries: cell and node ids and the cell
+  // coefficients, gathered into registers during the current tile's batches
+  // (one gather level per batch); the coordinates follow at the tile's start
+  int32_t pc[PF], pn[PF];
+  double pk[PF];
+  int pre_tile = -1, npre = 0;
+
+  for (int tile = blockIdx.x; tile < args.ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile's batches are done with G, the accumulators and the counter
+    // per-thread indices are rederived each tile (an opaque copy of the
+    // thread id) so that their address arithmetic is not held in registers
+    // across the whole tile loop
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int wave = tid >> 6, lane = tid & 63;
+    const int nl = lane & 15, g = lane >> 4;
+    double *acc = acc_all + (size_t)wave * args.acc_per_wave;
+    const int sink = args.acc_per_wave - 64 + lane;
     const int tq = tid % NQ, tcs = tid / NQ;
     const bool gl = tcs < CPR;
+    // staging of the (cell -> node -> coordinates, coefficient) gather of a
+    // chunk, entries [from, n): SU entries per thread and pass, each level
+    // issued for all of them at once
+    auto stage = [&](int first_cell, int from, int n) {
+      constexpr int SU = 4;
+      for (int base = from; base < n; base += kB * SU) {
+        int64_t cellv[SU], nodev[SU];
+  #pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const int i = base + u * kB + tid;
+          cellv[u] = i < n ? args.tile_cells[first_cell + i / NV] : 0;
+        }
+  #pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const int i = base + u * kB + tid;
+          nodev[u] = i < n ? args.cell_nodes[cellv[u] * NV + (i % NV)] : 0;
+          if (i < n && i % NV == 0) K_s[i / NV] = args.coef ? args.coef[cellv[u]] : 1.0;
+        }
+  #pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const int i = base + u * kB + tid;
+          if (i < n) {
+  #pragma unroll
+            for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[nodev[u] * D + d];
+          }
+        }
+      }
+    };
+    const int cell0 = args.tile_cell_ptr[tile];
+    const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
+    if (tid == 0) *claim_ctr = NW;  // read after the geometry phase's barriers
+
+    // phase A: geometry; each thread keeps one quadrature point's geometry
+    // gradients in registers for this phase
     double gq[NV][D];
 #pragma unroll
     for (int n = 0; n < NV; n++)
@@ -440,32 +487,20 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
     const double wq = gl ? args.w[tq] : 0.0;
     for (int c0 = 0; c0 < ncell; c0 += args.stage_cells) {
       const int nch = min(args.stage_cells, ncell - c0);
-      double *K_s = X_s + (size_t)args.stage_cells * NV * D;  // coefficient per staged cell
-      // staging: SU entries per thread and pass, each level of the
-      // (cell -> node -> coordinates) gather issued for all of them at once
-      constexpr int SU = 4;
-      for (int base = 0; base < nch * NV; base += kB * SU) {
-        int64_t cellv[SU], nodev[SU];
+      int from = 0;
+      if (c0 == 0 && pre_tile == tile) {
 #pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const int i = base + u * kB + tid;
-          cellv[u] = i < nch * NV ? args.tile_cells[cell0 + c0 + i / NV] : 0;
-        }
+        for (int u = 0; u < PF; u++) {
+          const int i = u * kB + tid;
+          if (i < npre) {
 #pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const int i = base + u * kB + tid;
-          nodev[u] = i < nch * NV ? args.cell_nodes[cellv[u] * NV + (i % NV)] : 0;
-          if (i < nch * NV && i % NV == 0) K_s[i / NV] = args.coef ? args.coef[cellv[u]] : 1.0;
-        }
-#pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const int i = base + u * kB + tid;
-          if (i < nch * NV) {
-#pragma unroll
-            for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[nodev[u] * D + d];
+            for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[(int64_t)pn[u] * D + d];
+            if (i % NV == 0) K_s[i / NV] = pk[u];
           }
         }
+        from = npre;
       }
+      stage(cell0 + c0, from, nch * NV);
       __syncthreads();
       if (gl) {
         for (int sl = tcs; sl < nch; sl += CPR) {
@@ -512,126 +547,148 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
       }
       __syncthreads();
     }
-  }
 
-  // phase B: batches
-  const int wave = tid >> 6, lane = tid & 63;
-  const int nl = lane & 15, g = lane >> 4;
-  double *acc = acc_all + (size_t)wave * args.acc_per_wave;
-  for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
-  CT bfrag[MB][KS];
+    // phase B: batches; the B operand (reference gradients) in registers
+    for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
+    CT bfrag[MB][KS];
 #pragma unroll
-  for (int mb = 0; mb < MB; mb++)
+    for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-      const int a = mb * 16 + nl, q = g * QG + ks / KD, d = ks % KD;
-      bfrag[mb][ks] = (a < NS && q < NQ) ? tab_s[(q * NS + a) * TP + d] : (CT)0;
+      for (int ks = 0; ks < KS; ks++) {
+        const int a = mb * 16 + nl, q = g * QG + ks / KD, d = ks % KD;
+        bfrag[mb][ks] = (a < NS && q < NQ) ? tab_s[(q * NS + a) * TP + d] : (CT)0;
+      }
+    const int ntile = tile + gridDim.x;
+    int pf_cell0 = 0, pf_n = 0;
+    if (ntile < args.ntiles) {
+      pf_cell0 = args.tile_cell_ptr[ntile];
+      pf_n = min(min(args.tile_cell_ptr[ntile + 1] - pf_cell0, args.stage_cells) * NV, PF * kB);
+#pragma unroll
+      for (int u = 0; u < PF; u++) {
+        const int i = u * kB + tid;
+        pc[u] = i < pf_n ? args.tile_cells[pf_cell0 + i / NV] : 0;
+      }
     }
-  const int sink = args.acc_per_wave - 64 + lane;
-  const int bt0 = args.tile_batch_ptr[tile], bt1 = args.tile_batch_ptr[tile + 1];
-  // batches are claimed from a per-tile counter (wave w starts with batch w);
-  // the result does not depend on which wave runs a batch
-  auto claim = [&]() -> int {
-    int v = 0;
-    if (lane == 0) v = atomicAdd(claim_ctr, 1);
-    return bt0 + (int)__builtin_amdgcn_readfirstlane(v);
-  };
-  int bi = bt0 + wave, bn = bt1;
-  BatchRegs<OffT, MB> nxt;
-  if (bi < bt1) {
-    load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
-    bn = claim();
-  }
-  while (bi < bt1) {
-    const BatchRegs<OffT, MB> cur = nxt;
-    bi = bn;
+    bool pf_nodes = false;  // second gather level issued
+    auto pf_step = [&]() {
+#pragma unroll
+      for (int u = 0; u < PF; u++) {
+        const int i = u * kB + tid;
+        pn[u] = i < pf_n ? args.cell_nodes[(int64_t)pc[u] * NV + (i % NV)] : 0;
+        pk[u] = (i < pf_n && i % NV == 0) ? (args.coef ? args.coef[pc[u]] : 1.0) : 0.0;
+      }
+      pf_nodes = true;
+    };
+
+    const int bt0 = args.tile_batch_ptr[tile], bt1 = args.tile_batch_ptr[tile + 1];
+    // batches are claimed from a per-tile counter (wave w starts with batch
+    // w); the result does not depend on which wave runs a batch
+    auto claim = [&]() -> int {
+      int v = 0;
+      if (lane == 0) v = atomicAdd(claim_ctr, 1);
+      return bt0 + (int)__builtin_amdgcn_readfirstlane(v);
+    };
+    int bi = bt0 + wave, bn = bt1;
+    BatchRegs<OffT, MB> nxt;
     if (bi < bt1) {
       load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
       bn = claim();
     }
-    const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
-    const int tmax = (info >> 8) & 0xFF;
-    // W column of slot a_row(nl); empty slots read slot 0 (finite values,
-    // never scattered) and padding rows are zero in both operands
-    const int s = cur.sb & 0xFFFF, b = cur.sb >> 16;
-    const CT *Gc = G_s + s * GS + g * QG * NGP;
-    const CT *Tb = tab_s + (g * QG * NS + b) * TP;
-    V4 c[MB];
+    while (bi < bt1) {
+      const BatchRegs<OffT, MB> cur = nxt;
+      bi = bn;
+      if (bi < bt1) {
+        load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
+        bn = claim();
+      }
+      const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
+      const int tmax = (info >> 8) & 0xFF;
+      // W column of slot a_row(nl); empty slots read slot 0 (finite values,
+      // never scattered) and padding rows are zero in both operands
+      const int s = cur.sb & 0xFFFF, b = cur.sb >> 16;
+      const CT *Gc = G_s + s * GS + g * QG * NGP;
+      const CT *Tb = tab_s + (g * QG * NS + b) * TP;
+      V4 c[MB];
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
+      for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
 #pragma unroll
-    for (int j = 0; j < QG; j++) {
-      CT G[NGP], T[TP];
-      lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
-      lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
-      CT W[KD];
-      if constexpr (FORM == kLaplace) {
-        if constexpr (D == 3) {
-          W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
-          W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
-          W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+      for (int j = 0; j < QG; j++) {
+        CT G[NGP], T[TP];
+        lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
+        lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
+        CT W[KD];
+        if constexpr (FORM == kLaplace) {
+          if constexpr (D == 3) {
+            W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
+            W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
+            W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+          } else {
+            W[0] = G[0] * T[0] + G[1] * T[1];
+            W[1] = G[1] * T[0] + G[2] * T[1];
+          }
         } else {
-          W[0] = G[0] * T[0] + G[1] * T[1];
-          W[1] = G[1] * T[0] + G[2] * T[1];
+          W[0] = G[0] * T[0];
         }
-      } else {
-        W[0] = G[0] * T[0];
+#pragma unroll
+        for (int d = 0; d < KD; d++) {
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
+        }
       }
-#pragma unroll
-      for (int d = 0; d < KD; d++) {
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
-      }
-    }
-    // destinations of the 4 slots x MB rows this lane holds; rows outside
-    // the row set and padding go to a private per-lane slot past the
-    // accumulators
-    const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
-    int rank[4];
-    int dst[MB][4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      rank[r] = (int)(at4[r] >> 16);
-      const int ab = (int)(at4[r] & 0xFFFF);
-#pragma unroll
-      for (int mb = 0; mb < MB; mb++) {
-        const int off = O4::get(cur.off[mb], r);
-        dst[mb][r] = off != (int)O4::none ? ab + off : sink;
-      }
-    }
-    for (int t = 0; t < tmax; t++) {
+      // destinations of the 4 slots x MB rows this lane holds; rows outside
+      // the row set and padding go to a private per-lane slot past the
+      // accumulators
+      const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
+      int rank[4];
+      int dst[MB][4];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        if (rank[r] == t) {
+        rank[r] = (int)(at4[r] >> 16);
+        const int ab = (int)(at4[r] & 0xFFFF);
 #pragma unroll
-          for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+        for (int mb = 0; mb < MB; mb++) {
+          const int off = O4::get(cur.off[mb], r);
+          dst[mb][r] = off != (int)O4::none ? ab + off : sink;
         }
       }
-    }
-    // flat write-out of the batch's KR rounds of 64 accumulator entries:
-    // entry e goes to e + delta[c], c = column ends before e (the padding
-    // column sends the entries past the batch's columns to the value array's
-    // tail); all reads first, then the clears, then the stores, so every
-    // batch issues the same number of stores and the next batch's
-    // prefetched record is waited for without waiting for them
-    {
-      int ends = 0;
-      double v[KR];
-      int32_t z[KR];
+      for (int t = 0; t < tmax; t++) {
 #pragma unroll
-      for (int k = 0; k < KR; k++) {
-        const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);  // k < 16: end-mask words
-        const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
-        const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-        ends += __builtin_popcount(lo) + __builtin_popcount(hi);
-        v[k] = acc[k * 64 + lane];
-        z[k] = k * 64 + lane + __shfl(cur.delta, cidx);
+        for (int r = 0; r < 4; r++) {
+          if (rank[r] == t) {
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+          }
+        }
       }
+      // flat write-out of the batch's KR rounds of 64 accumulator entries:
+      // entry e goes to e + delta[c], c = column ends before e (the padding
+      // column sends the entries past the batch's columns to the value
+      // array's tail); all reads first, then the clears, then the stores, so
+      // every batch issues the same number of stores and the next batch's
+      // prefetched record is waited for without waiting for them
+      {
+        int ends = 0;
+        double v[KR];
+        int32_t z[KR];
 #pragma unroll
-      for (int k = 0; k < KR; k++) acc[k * 64 + lane] = 0.0;
+        for (int k = 0; k < KR; k++) {
+          const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);  // k < 16: end-mask words
+          const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
+          const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+          ends += __builtin_popcount(lo) + __builtin_popcount(hi);
+          v[k] = acc[k * 64 + lane];
+          z[k] = k * 64 + lane + __shfl(cur.delta, cidx);
+        }
 #pragma unroll
-      for (int k = 0; k < KR; k++) args.nzval[(int64_t)z[k]] = v[k];
+        for (int k = 0; k < KR; k++) acc[k * 64 + lane] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KR; k++) args.nzval[(int64_t)z[k]] = v[k];
+      }
+      if (!pf_nodes) pf_step();
     }
+    if (!pf_nodes) pf_step();
+    pre_tile = ntile;
+    npre = pf_n;
   }
 }
 
