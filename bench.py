@@ -47,6 +47,21 @@ def survey_bytes_per_cell_q2(n):
     return 8 * 3 * nodes + 4 * 8 * cells + 4 * 27 * cells + 4 * 729 * cells + 8 * nnz
 
 
+def measured_traffic(n):
+    """HBM bytes per launch from the committed counter summary of this
+    workload (rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain copy), or
+    None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_q2hex64.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("cells_per_dir") != n:
+        return None
+    return d["bytes_per_launch"]
+
+
 def copy_calibration(nbytes):
     """One plain device-to-device copy of nbytes (the HIP runtime's copy
     kernel) so hardware byte counters can be scaled against known traffic."""
@@ -88,11 +103,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=64, help="cells per direction")
-    ap.add_argument("--cpu-sample", type=int, default=40, help="cells per direction of the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="also run one plain device-to-device copy of this many MiB (torch) so hardware "
+                    help="also run one plain device-to-device copy of this many MiB (HIP runtime) so hardware "
                          "byte counters can be scaled against a copy of known size")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both",
                     help="time only one arithmetic form (profiling runs)")
@@ -139,6 +154,7 @@ def main():
     if args.only in ("both", "mixed"):
         wallmx, kernmx = run(L.GTK_MIXED)
     cells = mesh.cells.shape[0]
+    traffic = measured_traffic(n)
     bytes_alg = survey_bytes_per_cell_q2(n)
     flops_alg = cells * 27 * (2 * 3 * 27 * 27 + 2 * 9 * 27 + 2 * 8 * 9)
     achieved = bytes_alg / (kernmx * 1e-3) / 1e9
@@ -159,8 +175,10 @@ def main():
                                f"U(-0.2h,0.2h) (seed {args.seed}), 27-point Gauss, kappa=1",
                    "cells": cells, "dofs": int(V.nfree), "nnz": int(A64.nnz), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic or {}).get("mixed")},
         "fp64": {"cells_per_s": cells / (wall64 * 1e-3), "ms_per_step": wall64, "kernel_ms": kern64,
+                 "traffic": (traffic or {}).get("fp64"),
                  "hbm_gbs_alg": bytes_alg / (kern64 * 1e-3) / 1e9,
                  "fp64_tflops_alg": flops_alg / (kern64 * 1e-3) / 1e12},
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err,
