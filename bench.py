@@ -37,16 +37,6 @@ def load_package():
     return mod
 
 
-def survey_bytes_per_cell_q2(n):
-    """Algorithmic HBM bytes of one Q2-hex matrix assembly launch on an n^3
-    mesh: coordinates + cell->node + cell->DOF + local-entry->nnz map (4 B
-    per local entry) + FP64 values, each moved once."""
-    nodes = (n + 1) ** 3
-    cells = n ** 3
-    nnz = (8 * n + 1) ** 3
-    return 8 * 3 * nodes + 4 * 8 * cells + 4 * 27 * cells + 4 * 729 * cells + 8 * nnz
-
-
 def measured_traffic(n):
     """HBM bytes per launch from the committed counter summary of this
     workload (rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain copy), or
@@ -97,12 +87,154 @@ def cpu_baseline(n_sample, seed):
                       f"COO->CSC in the reference loop order, single thread, {dt:.2f} s"}
 
 
+def alg_bytes(D, n_geo, cells, nv, nd, ncoef, nnz):
+    """Algorithmic HBM bytes of one matrix launch: coordinates, cell->node,
+    cell->DOF, per-cell coefficients, a 4-byte local-entry->nnz map and the
+    FP64 values, each moved once."""
+    return 8 * D * n_geo + 4 * nv * cells + 4 * nd * cells + 8 * ncoef * cells + 4 * nd * nd * cells + 8 * nnz
+
+
+def time_form(ctx, L, form, prec, steps, warmup):
+    """Wall time per launch over `steps` queued launches, and the kernel's own
+    time from HIP events on the context stream."""
+    for _ in range(warmup):
+        ctx.assemble_matrix_async(form, prec, L.GTK_FREE, L.GTK_FREE)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble_matrix_async(form, prec, L.GTK_FREE, L.GTK_FREE)
+    ctx.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    return wall_ms, ctx.time_matrix(form, prec, L.GTK_FREE, L.GTK_FREE, steps)
+
+
+def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
+    """Laplace assembly on an n^3 hex mesh: FP64 and mixed forms."""
+    from gtk_amd import _lib as L
+    t_setup = time.perf_counter()
+    mesh = GT.cartesian_mesh((0.0, 1.0) * 3, (n, n, n))
+    if perturb:
+        mesh.coords = GT.perturb_interior_nodes(mesh, perturb, args.seed)
+    Om = GT.interior(mesh)
+    V = GT.lagrange_space(Om, order)
+    dO = GT.measure(Om, degree)
+    cells = mesh.cells.shape[0]
+    kappa = GT.inputs.lognormal(args.seed, cells) if lognormal_kappa else None
+    form = GT.Laplace(dO, GT.face_constant_field(kappa, Om) if lognormal_kappa else None)
+    A64, cache = GT.assemble_matrix(form, float, V, V, precision=GT.FP64, reuse=True)
+    ref = A64.nzval.copy()
+    Amx = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+    rel_err = float(np.linalg.norm(Amx.nzval - ref) / np.linalg.norm(ref))
+    setup_s = time.perf_counter() - t_setup
+    ctx = cache[0].ctx
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
+    wall64 = kern64 = wallmx = kernmx = float("nan")
+    if args.only in ("both", "fp64"):
+        wall64, kern64 = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_FP64, args.steps, args.warmup)
+    if args.only in ("both", "mixed"):
+        wallmx, kernmx = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_MIXED, args.steps, args.warmup)
+    nd = V.fe.nnodes
+    nq = len(dO.w)
+    traffic = measured_traffic(n) if (order, degree, perturb, lognormal_kappa) == (2, 4, 0.2, False) else None
+    bytes_alg = alg_bytes(3, mesh.coords.shape[0], cells, 8, nd, 1 if lognormal_kappa else 0, A64.nnz)
+    flops_alg = cells * nq * (2 * 3 * nd * nd + 2 * 9 * nd + 2 * 8 * 9)
+    achieved = bytes_alg / (kernmx * 1e-3) / 1e9
+    kdesc = "lognormal kappa per cell" if lognormal_kappa else "kappa=1"
+    pdesc = f"interior vertices perturbed U(-{perturb}h,{perturb}h) (seed {args.seed}), " if perturb else ""
+    return {
+        "metric": "cells/s + HBM GB/s, Q2 hex Laplace assembly; mixed vs FP64 at stated rel err",
+        "value": cells / (wallmx * 1e-3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wallmx,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 contraction, fp64 geometry + accumulation",
+        "data": "synthetic",
+        "config": {"workload": f"3D Laplace, {n}^3 Q{order} hex cells on the unit cube, {pdesc}"
+                               f"{nq}-point Gauss, {kdesc}",
+                   "name": label, "cells": cells, "dofs": int(V.nfree), "nnz": int(A64.nnz),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic or {}).get("mixed")},
+        "fp64": {"cells_per_s": cells / (wall64 * 1e-3), "ms_per_step": wall64, "kernel_ms": kern64,
+                 "traffic": (traffic or {}).get("fp64"),
+                 "hbm_gbs_alg": bytes_alg / (kern64 * 1e-3) / 1e9,
+                 "roofline_frac": bytes_alg / (kern64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "fp64_tflops_alg": flops_alg / (kern64 * 1e-3) / 1e12},
+        "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err,
+                  "speedup_vs_fp64": kern64 / kernmx},
+        "dofs_per_s": V.nfree / (wallmx * 1e-3),
+        "bytes_alg_per_launch": bytes_alg,
+        "setup_s": setup_s,
+    }
+
+
+def bench_elasticity(GT, args, n):
+    """P2-tet linear elasticity on n^3 cubes x 6: FP32 assembly, body force,
+    clamped on x = 0, solved by FP32 Jacobi-PCG inside FP64 refinement to
+    relative residual 1e-10."""
+    from gtk_amd import _lib as L
+    t_setup = time.perf_counter()
+    mesh = GT.cartesian_mesh((0.0, 1.0) * 3, (n, n, n), simplexify=True)
+    Om = GT.interior(mesh)
+    V = GT.lagrange_space(Om, 2, tensor_size=(3,), dirichlet_boundary=GT.boundary(mesh, group_names=["2-face-5"]))
+    dO = GT.measure(Om, 5)
+    cells = mesh.cells.shape[0]
+    lam = 0.5 + 1.5 * GT.inputs.uniform(args.seed, cells)
+    mu = 0.5 + 1.5 * GT.inputs.uniform(args.seed + 1, cells)
+    form = GT.Elasticity(dO, GT.face_constant_field(lam, Om), GT.face_constant_field(mu, Om))
+    A64 = GT.assemble_matrix(form, float, V, V, precision=GT.FP64)
+    ref = A64.nzval.copy()
+    A = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+    rel_err = float(np.linalg.norm(A.nzval - ref) / np.linalg.norm(ref))
+    b = GT.assemble_vector(GT.Load(dO, (0.0, 0.0, -1.0)), float, V)
+    setup_s = time.perf_counter() - t_setup
+    ctx = A.device[0].ctx
+    wall64, kern64 = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_FP64, args.steps, args.warmup)
+    wallmx, kernmx = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_MIXED, args.steps, args.warmup)
+    # the FP32 assembly is the one solved
+    A = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+    x, st = GT.solve(A, b, precision=GT.MIXED, rtol=1e-10)
+    return {
+        "metric": "cells/s, P2 tet elasticity FP32 assembly + mixed-precision refined solve",
+        "value": cells / (wallmx * 1e-3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wallmx,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 contraction, fp64 geometry + accumulation; fp32 CG in fp64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"3D linear elasticity, {n}^3 cubes x 6 P2 tets, 14-point Strang, lambda and mu "
+                               f"U[0.5,2] per cell (seed {args.seed}), clamped on x=0, body force (0,0,-1)",
+                   "name": "p2tet", "cells": cells, "dofs": int(V.nfree), "nnz": int(A.nnz),
+                   "parallelism": "single GPU"},
+        "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3)},
+        "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err, "speedup_vs_fp64": kern64 / kernmx},
+        "solve": {"rel_residual": st.rel_residual, "outer_iterations": st.outer_iterations,
+                  "inner_iterations": st.inner_iterations, "ms": st.solve_ms},
+        "setup_s": setup_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells", type=int, default=64, help="cells per direction")
+    ap.add_argument("--workload", choices=["q2hex", "q1hex", "p2tet"], default="q2hex",
+                    help="q2hex: Q2 hex 64^3 perturbed Laplace (the headline); q1hex: Q1 hex 128^3 with lognormal "
+                         "kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve")
+    ap.add_argument("--cells", type=int, default=0, help="cells per direction (default per workload)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
@@ -117,78 +249,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     GT = load_package()
-    n = args.cells
-    t_setup = time.perf_counter()
-    mesh = GT.cartesian_mesh((0.0, 1.0) * 3, (n, n, n))
-    mesh.coords = GT.perturb_interior_nodes(mesh, 0.2, args.seed)
-    Om = GT.interior(mesh)
-    V = GT.lagrange_space(Om, 2)
-    dO = GT.measure(Om, 4)
-    form = GT.Laplace(dO)
-    A64, cache = GT.assemble_matrix(form, float, V, V, precision=GT.FP64, reuse=True)
-    ref = A64.nzval.copy()
-    Amx = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
-    rel_err = float(np.linalg.norm(Amx.nzval - ref) / np.linalg.norm(ref))
-    setup_s = time.perf_counter() - t_setup
-    asm = cache[0]
-    ctx = asm.ctx
-    from gtk_amd import _lib as L
-
-    def run(prec):
-        for _ in range(args.warmup):
-            ctx.assemble_matrix_async(L.GTK_LAPLACE, prec, L.GTK_FREE, L.GTK_FREE)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ctx.assemble_matrix_async(L.GTK_LAPLACE, prec, L.GTK_FREE, L.GTK_FREE)
-        ctx.sync()
-        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        kern_ms = ctx.time_matrix(L.GTK_LAPLACE, prec, L.GTK_FREE, L.GTK_FREE, args.steps)
-        return wall_ms, kern_ms
-
-    if args.copy_calibration_mib:
-        copy_calibration(args.copy_calibration_mib << 20)
-    wall64 = kern64 = wallmx = kernmx = float("nan")
-    if args.only in ("both", "fp64"):
-        wall64, kern64 = run(L.GTK_FP64)
-    if args.only in ("both", "mixed"):
-        wallmx, kernmx = run(L.GTK_MIXED)
-    cells = mesh.cells.shape[0]
-    traffic = measured_traffic(n)
-    bytes_alg = survey_bytes_per_cell_q2(n)
-    flops_alg = cells * 27 * (2 * 3 * 27 * 27 + 2 * 9 * 27 + 2 * 8 * 9)
-    achieved = bytes_alg / (kernmx * 1e-3) / 1e9
-    out = {
-        "metric": "cells/s + HBM GB/s, Q2 hex Laplace assembly; mixed vs FP64 at stated rel err",
-        "value": cells / (wallmx * 1e-3),
-        "unit": "cells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wallmx,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 contraction, fp64 geometry + accumulation",
-        "data": "synthetic",
-        "config": {"workload": f"3D Laplace, {n}^3 Q2 hex cells on the unit cube, interior vertices perturbed "
-                               f"U(-0.2h,0.2h) (seed {args.seed}), 27-point Gauss, kappa=1",
-                   "cells": cells, "dofs": int(V.nfree), "nnz": int(A64.nnz), "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (traffic or {}).get("mixed")},
-        "fp64": {"cells_per_s": cells / (wall64 * 1e-3), "ms_per_step": wall64, "kernel_ms": kern64,
-                 "traffic": (traffic or {}).get("fp64"),
-                 "hbm_gbs_alg": bytes_alg / (kern64 * 1e-3) / 1e9,
-                 "fp64_tflops_alg": flops_alg / (kern64 * 1e-3) / 1e12},
-        "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err,
-                  "speedup_vs_fp64": kern64 / kernmx},
-        "dofs_per_s": V.nfree / (wallmx * 1e-3),
-        "bytes_alg_per_launch": bytes_alg,
-        "setup_s": setup_s,
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.seed)
+    if args.workload == "q2hex":
+        out = bench_laplace(GT, args, args.cells or 64, 2, 4, 0.2, False, "q2hex")
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.seed)
+    elif args.workload == "q1hex":
+        out = bench_laplace(GT, args, args.cells or 128, 1, 2, 0.0, True, "q1hex")
+    else:
+        out = bench_elasticity(GT, args, args.cells or 48)
     print(json.dumps(out))
 
 
