@@ -10,13 +10,13 @@ from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField
                        assemble_vector, face_constant_field, linear_problem, measure, solve, update_matrix)
 from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
 from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space
+from .batch import MatrixBatch, assemble_matrices, autotune
 from . import inputs, refcell
 
 __all__ = [
     "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "Laplace", "Load",
     "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
     "assemble_vector", "face_constant_field", "linear_problem", "measure", "solve", "update_matrix", "Mesh",
-    "cartesian_mesh",
-    "perturb_interior_nodes", "Domain", "LagrangeSpace", "boundary", "interior", "lagrange_space", "inputs",
-    "refcell",
+    "cartesian_mesh", "perturb_interior_nodes", "MatrixBatch", "assemble_matrices", "autotune", "Domain",
+    "LagrangeSpace", "boundary", "interior", "lagrange_space", "inputs", "refcell",
 ]

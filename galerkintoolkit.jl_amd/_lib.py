@@ -21,7 +21,8 @@ EXPORTS = [
     "gtk_create", "gtk_destroy", "gtk_last_error", "gtk_version", "gtk_set_mesh", "gtk_update_coordinates",
     "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
-    "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve",
+    "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
+    "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_build", "gtk_batch_run",
 ]
 
 
@@ -75,6 +76,14 @@ def load():
     L.gtk_sync.argtypes = [vp]
     L.gtk_time_matrix.argtypes = [vp, i, i, i, i, i, C.POINTER(C.c_double)]
     L.gtk_solve.argtypes = [vp, i, f64, f64, C.c_double, i, i, C.c_double, C.POINTER(GtkSolveStats)]
+    L.gtk_batch_create.argtypes = [i, i, i, C.POINTER(vp)]
+    L.gtk_batch_destroy.argtypes = [vp]
+    L.gtk_batch_last_error.argtypes = [vp]
+    L.gtk_batch_last_error.restype = C.c_char_p
+    L.gtk_batch_add.argtypes = [vp, vp]
+    L.gtk_batch_set_tile_cells.argtypes = [vp, i, i]
+    L.gtk_batch_build.argtypes = [vp]
+    L.gtk_batch_run.argtypes = [vp, C.POINTER(GtkStats)]
     _lib = L
     return L
 
@@ -176,7 +185,55 @@ class Context:
                    "gtk_solve")
         return x, st
 
+    def get_values(self, rows, cols, nnz):
+        out = np.zeros(max(nnz, 1), np.float64)
+        self.check(self.L.gtk_get_values(self.h, rows, cols, out), "gtk_get_values")
+        return out[:nnz]
+
     def values_device_ptr(self, rows, cols):
         p = C.c_void_p()
         self.check(self.L.gtk_values_device_ptr(self.h, rows, cols, C.byref(p)), "gtk_values_device_ptr")
         return p.value
+
+
+class Batch:
+    """Owner of one gtk_batch: problems (contexts) assembled by one
+    persistent launch captured in a HIP graph."""
+
+    def __init__(self, device, form, precision):
+        self.L = load()
+        self.h = C.c_void_p()
+        self.ctxs = []  # keeps the contexts alive while the batch refers to them
+        rc = self.L.gtk_batch_create(device, form, precision, C.byref(self.h))
+        if rc != 0:
+            raise GtkError(f"gtk_batch_create failed ({rc})")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise GtkError(f"{what} failed ({rc}): {self.L.gtk_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.L.gtk_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, ctx):
+        self.check(self.L.gtk_batch_add(self.h, ctx.h), "gtk_batch_add")
+        self.ctxs.append(ctx)
+
+    def set_tile_cells(self, elem_type, max_cells):
+        self.check(self.L.gtk_batch_set_tile_cells(self.h, elem_type, max_cells), "gtk_batch_set_tile_cells")
+
+    def build(self):
+        self.check(self.L.gtk_batch_build(self.h), "gtk_batch_build")
+
+    def run(self):
+        st = GtkStats()
+        self.check(self.L.gtk_batch_run(self.h, C.byref(st)), "gtk_batch_run")
+        return st

@@ -1,0 +1,193 @@
+"""Batched assembly: many independent problems of uneven size (meshes,
+orders, coefficient fields) assembled by one persistent kernel launch,
+captured once in a HIP graph, with the tile size of each element type picked
+by an autotuner that caches its choices.
+
+The reference assembles one problem per GT.assemble_matrix call
+(src/problems.jl:319-350); MatrixBatch stands for a loop of such calls over
+(form, space) pairs and returns the same SparseMatrixCSC per problem.
+"""
+import json
+import os
+import tempfile
+
+import numpy as np
+
+from . import _lib as L
+from . import refcell as R
+from .assembly import FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _coefficients, _form_code
+
+# element types of the batched kernel: scalar quadrilaterals of order 1, 2, 3
+# with (order+1)^2 Gauss points
+ELEM_TYPES = {1: 0, 2: 1, 3: 2}
+ELEM_NAMES = {0: "quad-q1", 1: "quad-q2", 2: "quad-q3"}
+
+# tile sizes the autotuner tries (cells per tile; 0 = as many as fit in LDS)
+TILE_CANDIDATES = (0, 256, 160, 96, 64, 40, 24)
+
+
+def elem_type(space, dOmega=None):
+    """Element type of the batched kernel for a space, or raise."""
+    if space.mesh.cellkind != R.QUAD or space.ncomp != 1 or space.order not in ELEM_TYPES:
+        raise NotImplementedError("batched problems are scalar Q1/Q2/Q3 Lagrange spaces on quadrilaterals")
+    if dOmega is not None and len(dOmega.w) != (space.order + 1) ** 2:
+        raise NotImplementedError("batched problems use (order+1)^2 Gauss points: GT.measure(Omega, 2*order)")
+    return ELEM_TYPES[space.order]
+
+
+def default_cache_path():
+    p = os.environ.get("GTK_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "galerkintoolkit_jl_amd", "batch_tiles.json")
+
+
+def _cache_key(form_code, precision, et):
+    ver = L.load().gtk_version().decode()
+    return f"{ver}|form{form_code}|prec{precision}|{ELEM_NAMES[et]}"
+
+
+def _read_cache(path):
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _write_cache(path, d):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    fd, tmp = tempfile.mkstemp(dir=os.path.dirname(os.path.abspath(path)), suffix=".tmp")
+    with os.fdopen(fd, "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def _time_batch(batch, repeats):
+    batch.run()  # first launch of a fresh graph
+    return min(batch.run().kernel_ms for _ in range(repeats))
+
+
+def autotune(problems, form_code, precision, device=0, cache=None, candidates=TILE_CANDIDATES, sample=8,
+             repeats=3, retune=False):
+    """Cells per tile for each element type present in `problems` (a list of
+    (assembler, elem_type, cells)).  A type without a cached choice is tuned
+    on a sample of its problems (evenly spread over their sizes): each
+    candidate is built into a batch of the sample and timed by its kernel
+    time per cell; the fastest is cached under (library build, form,
+    precision, element type).  Returns ({type: max_cells}, report)."""
+    path = default_cache_path() if cache is None else cache
+    table = _read_cache(path)
+    choice, report = {}, {}
+    for et in sorted({p[1] for p in problems}):
+        key = _cache_key(form_code, precision, et)
+        if not retune and key in table:
+            choice[et] = int(table[key]["max_cells"])
+            report[ELEM_NAMES[et]] = {"max_cells": choice[et], "cached": True}
+            continue
+        mine = sorted((p for p in problems if p[1] == et), key=lambda p: p[2])
+        pick = np.unique(np.linspace(0, len(mine) - 1, min(sample, len(mine))).round().astype(int))
+        subset = [mine[i] for i in pick]
+        cells = sum(p[2] for p in subset)
+        times = {}
+        for cap in candidates:
+            b = L.Batch(device, form_code, precision)
+            for a, _, _ in subset:
+                b.add(a.ctx)
+            b.set_tile_cells(et, cap)
+            b.build()
+            times[cap] = _time_batch(b, repeats)
+            b.close()
+        best = min(times, key=lambda c: times[c])
+        choice[et] = best
+        table[key] = {"max_cells": best, "ns_per_cell": {str(c): 1e6 * t / cells for c, t in times.items()},
+                      "sample_cells": cells}
+        report[ELEM_NAMES[et]] = {"max_cells": best, "cached": False,
+                                  "ns_per_cell": {c: round(1e6 * t / cells, 4) for c, t in times.items()}}
+    _write_cache(path, table)
+    return choice, report
+
+
+class MatrixBatch:
+    """The (FREE, FREE) matrices of many problems, one bilinear form kind
+    (GT Laplace or mass, with per-problem coefficients) and one precision,
+    assembled together.
+
+    forms[i] pairs with spaces[i]; forms[i].dOmega is that problem's
+    GT.measure.  tuning: "auto" (autotuner with its cache), a dict
+    {order: max_cells}, or None (tiles as large as LDS allows)."""
+
+    def __init__(self, forms, spaces, precision=FP64, device=0, tuning="auto", cache=None):
+        if len(forms) != len(spaces) or not forms:
+            raise ValueError("one form per space, at least one problem")
+        kinds = {type(f) for f in forms}
+        if len(kinds) != 1 or not kinds <= {Laplace, Mass}:
+            raise NotImplementedError("a batch holds one form kind, Laplace or Mass")
+        self.form_code = _form_code(forms[0])
+        self.precision = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+        self.device = device
+        self.forms = list(forms)
+        self.spaces = list(spaces)
+        self.problems = []
+        for form, V in zip(forms, spaces):
+            et = elem_type(V, form.dOmega)
+            # a context of its own: coefficients set through single-problem
+            # calls on the same space do not reach the batch
+            a = _Assembler(V, form.dOmega, device)
+            a.pattern(FREE, FREE)
+            coef, ncoef = _coefficients(form, V.mesh.cells.shape[0])
+            a.ctx.set_cell_coefficients(coef, ncoef)
+            self.problems.append((a, et, V.mesh.cells.shape[0]))
+        if tuning == "auto":
+            self.tiles, self.tuning_report = autotune(self.problems, self.form_code, self.precision, device, cache)
+        elif tuning is None:
+            self.tiles, self.tuning_report = {}, None
+        else:
+            self.tiles = {ELEM_TYPES[o]: int(c) for o, c in dict(tuning).items()}
+            self.tuning_report = None
+        self.batch = L.Batch(device, self.form_code, self.precision)
+        for a, _, _ in self.problems:
+            self.batch.add(a.ctx)
+        for et, cap in self.tiles.items():
+            self.batch.set_tile_cells(et, cap)
+        self.batch.build()
+        self.cells = sum(p[2] for p in self.problems)
+
+    def set_forms(self, forms):
+        """New coefficients for the same problems (same form kind); the next
+        run assembles with them (the reference's update_matrix! per problem,
+        src/assembly.jl:592-893)."""
+        if len(forms) != len(self.problems) or {type(f) for f in forms} != {type(self.forms[0])}:
+            raise ValueError("same number of problems and the same form kind")
+        for form, (a, _, n) in zip(forms, self.problems):
+            coef, ncoef = _coefficients(form, n)
+            a.ctx.set_cell_coefficients(coef, ncoef)
+        self.forms = list(forms)
+
+    def run(self):
+        """One launch assembling every problem; returns its gtk_stats."""
+        return self.batch.run()
+
+    def matrices(self):
+        """The assembled SparseMatrixCSC of every problem (values copied
+        from the device)."""
+        out = []
+        for a, _, _ in self.problems:
+            nnz, ncols, nrows, colptr, rowval = a.pattern(FREE, FREE)
+            A = SparseMatrixCSC(nrows, ncols, colptr, rowval, a.ctx.get_values(FREE, FREE, nnz))
+            A.device = (a, FREE, FREE)
+            out.append(A)
+        return out
+
+    def close(self):
+        self.batch.close()
+
+
+def assemble_matrices(forms, spaces, precision=FP64, device=0, tuning="auto", cache=None):
+    """GT.assemble_matrix(forms[i], Float64, spaces[i], spaces[i]) for every
+    i, in one launch; returns (matrices, batch) so the batch can be run again
+    after new coefficients (MatrixBatch.set_forms)."""
+    b = MatrixBatch(forms, spaces, precision=precision, device=device, tuning=tuning, cache=cache)
+    b.run()
+    return b.matrices(), b

@@ -462,13 +462,10 @@ int gtk_get_pattern(gtk_ctx *ctx, int rows, int cols, int32_t *colptr, int32_t *
 // Work plan of (form, precision) over a block, built on first use: the
 // matrix-core kernel when the space is scalar and no column has more than
 // kBatchSlots contributors, the VALU kernel otherwise.
-static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, ExecPlan *&out) {
-  const int key = form * 2 + precision;
-  auto it = plan.exec.find(key);
-  if (it != plan.exec.end()) {
-    out = it->second.get();
-    return GTK_OK;
-  }
+// Builds the work plan of (form, precision) over a block for a kernel of
+// `waves` wavefronts; max_cells > 0 caps the cells of a tile (tuning).
+static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, int waves, int max_cells_cap,
+                      std::unique_ptr<ExecPlan> &out) {
   if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
   auto ep = std::make_unique<ExecPlan>();
   const gtk::MeshView m = view(ctx);
@@ -478,12 +475,12 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
   const gtk::ElemKey key_e = elem_key(ctx);
   const int nd = ctx->dofs_per_cell;
   std::string e;
-  const int waves = ctx->waves ? ctx->waves : (precision == GTK_FP64 ? 8 : 12);
   bool mfma = gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, waves, ep->k) && ep->k.mfma;
   if (mfma) {
     // accumulator bound: a batch adds at most nd rows per slot
     const size_t acc_bound = (size_t)waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
-    const int max_cells = tile_cells_for(ep->k, acc_bound, ctx->lds_budget);
+    int max_cells = tile_cells_for(ep->k, acc_bound, ctx->lds_budget);
+    if (max_cells_cap > 0) max_cells = std::max(8, std::min(max_cells, max_cells_cap));
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     if (ep->tiles.max_ctr > gtk::kBatchSlots) mfma = false;
@@ -547,22 +544,33 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
     ep->resident = std::max(1, per_cu) * std::max(1, cus);
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  out = std::move(ep);
+  return GTK_OK;
+}
+
+// Work plan of (form, precision) over a block, built on first use with the
+// default workgroup shape of the precision.
+static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, ExecPlan *&out) {
+  const int key = form * 2 + precision;
+  auto it = plan.exec.find(key);
+  if (it != plan.exec.end()) {
+    out = it->second.get();
+    return GTK_OK;
+  }
+  const int waves = ctx->waves ? ctx->waves : (precision == GTK_FP64 ? 8 : 12);
+  std::unique_ptr<ExecPlan> ep;
+  int rc = build_exec(ctx, plan, form, precision, waves, 0, ep);
+  if (rc) return rc;
   out = ep.get();
   plan.exec[key] = std::move(ep);
   return GTK_OK;
 }
 
-static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision) {
-  if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
-  if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (form < GTK_LAPLACE || form > GTK_ELASTICITY) FAIL(ctx, GTK_EINVAL, "unknown form");
-  if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
-  if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
-  ExecPlan *ep = nullptr;
-  int rc = get_exec(ctx, plan, form, precision, ep);
-  if (rc) return rc;
-  if (ep->tiles.ntiles == 0) return GTK_OK;
-  gtk::MatrixArgs a = {};
+// Kernel arguments of a work plan.
+static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, const ExecPlan &e) {
+  const ExecPlan *ep = &e;
+  gtk::MatrixArgs a;
+  std::memset(&a, 0, sizeof(a));  // padding too: gtk_batch_run compares records bytewise
   a.xyz = ctx->d_xyz.as<double>();
   a.cell_nodes = ctx->d_cell_nodes.as<int32_t>();
   a.coef = ctx->ncoef ? ctx->d_coef.as<double>() : nullptr;
@@ -598,6 +606,20 @@ static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision
   a.stage_cells = ep->stage_cells;
   a.lds_claim = ep->lds_claim;
   a.ntiles = ep->tiles.ntiles;
+  return a;
+}
+
+static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision) {
+  if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
+  if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
+  if (form < GTK_LAPLACE || form > GTK_ELASTICITY) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
+  if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
+  ExecPlan *ep = nullptr;
+  int rc = get_exec(ctx, plan, form, precision, ep);
+  if (rc) return rc;
+  if (ep->tiles.ntiles == 0) return GTK_OK;
+  gtk::MatrixArgs a = matrix_args(ctx, plan, *ep);
   void *args[] = {&a};
   // matrix-core kernels are persistent: as many workgroups as fit at once,
   // each looping over tiles
@@ -891,6 +913,219 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
     stats->rel_residual = bn > 0 ? rn / bn : rn;
     stats->converged = rn <= target ? 1 : 0;
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return GTK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches of problems in one launch
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct gtk_batch {
+  int device = 0, form = 0, precision = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::vector<gtk_ctx *> ctxs;
+  std::vector<std::unique_ptr<ExecPlan>> plans;  // one per problem
+  int tile_cells[3] = {0, 0, 0};                 // cap on the cells of a tile per element type (0: LDS bound)
+  std::vector<gtk::MatrixArgs> args;  // host copy of d_problems
+  DevBuf d_problems, d_types, d_items;
+  gtk::KernelInfo k;
+  size_t lds = 0;
+  int nitems = 0, grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t cells = 0, nnz = 0;
+};
+
+static void batch_release_graph(gtk_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+#define BFAIL(b, code, msg) \
+  do {                      \
+    (b)->err = (msg);       \
+    return (code);          \
+  } while (0)
+
+#define BHIPCHK(b, call)                                                       \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+      return GTK_EDEVICE;                                                      \
+    }                                                                          \
+  } while (0)
+
+template <typename T>
+static int batch_upload(gtk_batch *b, DevBuf &buf, const T *src, size_t count) {
+  buf.release();
+  const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+  hipError_t e = hipMalloc(&buf.p, bytes);
+  if (e != hipSuccess) {
+    buf.p = nullptr;
+    BFAIL(b, GTK_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  buf.n = bytes;
+  if (count) BHIPCHK(b, hipMemcpy(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return GTK_OK;
+}
+
+extern "C" {
+
+const char *gtk_batch_last_error(const gtk_batch *b) { return b ? b->err.c_str() : "batch is NULL"; }
+
+int gtk_batch_create(int device, int form, int precision, gtk_batch **out) {
+  if (!out) return GTK_EINVAL;
+  *out = nullptr;
+  if (form != GTK_LAPLACE && form != GTK_MASS) return GTK_EUNSUP;
+  if (precision != GTK_FP64 && precision != GTK_MIXED) return GTK_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return GTK_EDEVICE;
+  auto b = std::make_unique<gtk_batch>();
+  b->device = device;
+  b->form = form;
+  b->precision = precision;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return GTK_EDEVICE;
+  *out = b.release();
+  return GTK_OK;
+}
+
+int gtk_batch_destroy(gtk_batch *b) {
+  if (!b) return GTK_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release_graph(b);
+  b->plans.clear();
+  b->d_problems.release();
+  b->d_types.release();
+  b->d_items.release();
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return GTK_OK;
+}
+
+int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx) {
+  if (!b || !ctx) return GTK_EINVAL;
+  if (ctx->device != b->device) BFAIL(b, GTK_EINVAL, "the problem's context is on another device");
+  if (!find_plan(ctx, GTK_FREE, GTK_FREE)) BFAIL(b, GTK_ESTATE, "gtk_build_pattern(GTK_FREE, GTK_FREE) first");
+  if (!ctx->has_ref) BFAIL(b, GTK_ESTATE, "gtk_set_reference first");
+  if (ctx->ncomp != 1 || gtk::batched_type(elem_key(ctx)) < 0)
+    BFAIL(b, GTK_EUNSUP, "batched problems are scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2 points");
+  if (ctx->ncoef > 1) BFAIL(b, GTK_EINVAL, "scalar forms take one coefficient per cell");
+  batch_release_graph(b);
+  b->ctxs.push_back(ctx);
+  return GTK_OK;
+}
+
+int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells) {
+  if (!b) return GTK_EINVAL;
+  if (elem_type < 0 || elem_type > 2 || max_cells < 0) BFAIL(b, GTK_EINVAL, "elem_type 0..2, max_cells >= 0");
+  b->tile_cells[elem_type] = max_cells;
+  batch_release_graph(b);
+  return GTK_OK;
+}
+
+int gtk_batch_build(gtk_batch *b) {
+  if (!b) return GTK_EINVAL;
+  if (b->ctxs.empty()) BFAIL(b, GTK_ESTATE, "gtk_batch_add problems first");
+  BHIPCHK(b, hipSetDevice(b->device));
+  batch_release_graph(b);
+  if (!gtk::find_batched_kernel(b->form, b->precision, b->k)) BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
+  b->plans.clear();
+  std::vector<gtk::MatrixArgs> args;
+  std::vector<int32_t> types;
+  std::vector<int2> items;
+  b->lds = 0;
+  b->cells = b->nnz = 0;
+  for (size_t p = 0; p < b->ctxs.size(); p++) {
+    gtk_ctx *ctx = b->ctxs[p];
+    MatrixPlan *P = find_plan(ctx, GTK_FREE, GTK_FREE);
+    const int type = gtk::batched_type(elem_key(ctx));
+    std::unique_ptr<ExecPlan> ep;
+    int rc = build_exec(ctx, *P, b->form, b->precision, gtk::kBatchedWaves, b->tile_cells[type], ep);
+    if (rc) BFAIL(b, rc, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
+    if (!ep->k.mfma || ep->bat.off_bytes != 1)
+      BFAIL(b, GTK_EUNSUP, "a problem does not fit the matrix-core batches with byte offsets");
+    b->lds = std::max(b->lds, ep->lds);
+    args.push_back(matrix_args(ctx, *P, *ep));
+    types.push_back(type);
+    for (int t = 0; t < ep->tiles.ntiles; t++) items.push_back(make_int2((int)p, t));
+    b->cells += ctx->n_cells;
+    b->nnz += P->pat.nnz;
+    b->plans.push_back(std::move(ep));
+  }
+  int rc;
+  b->args = args;
+  if ((rc = batch_upload(b, b->d_problems, args.data(), args.size()))) return rc;
+  if ((rc = batch_upload(b, b->d_types, types.data(), types.size()))) return rc;
+  if ((rc = batch_upload(b, b->d_items, items.data(), items.size()))) return rc;
+  b->nitems = (int)items.size();
+  if (b->lds > 160 * 1024) BFAIL(b, GTK_EUNSUP, "a problem's tile does not fit in LDS");
+  if (b->lds > 64 * 1024)
+    BHIPCHK(b, hipFuncSetAttribute(b->k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
+  int per_cu = 0, cus = 0;
+  BHIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->k.fn, b->k.block, b->lds));
+  BHIPCHK(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  b->grid = std::max(1, std::min(b->nitems, std::max(1, per_cu) * std::max(1, cus)));
+  // the whole batch is one persistent launch, captured once as a graph
+  gtk::BatchedArgs ba;
+  ba.problems = b->d_problems.as<gtk::MatrixArgs>();
+  ba.types = b->d_types.as<int32_t>();
+  ba.items = b->d_items.as<int2>();
+  ba.nitems = b->nitems;
+  void *kargs[] = {&ba};
+  BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipLaunchKernel(b->k.fn, dim3(b->grid), dim3(b->k.block), kargs, b->lds, b->stream);
+  hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
+  if (le != hipSuccess) BFAIL(b, GTK_EDEVICE, std::string("hipLaunchKernel: ") + hipGetErrorString(le));
+  if (ce != hipSuccess) BFAIL(b, GTK_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  BHIPCHK(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  return GTK_OK;
+}
+
+int gtk_batch_run(gtk_batch *b, gtk_stats *stats) {
+  if (!b) return GTK_EINVAL;
+  if (!b->gexec) BFAIL(b, GTK_ESTATE, "gtk_batch_build first");
+  auto t0 = std::chrono::steady_clock::now();
+  BHIPCHK(b, hipSetDevice(b->device));
+  // the graph reads the per-problem records through d_problems: refresh
+  // them when a context moved a buffer (new coefficients, kappa reset)
+  bool changed = false;
+  for (size_t p = 0; p < b->ctxs.size(); p++) {
+    MatrixPlan *P = find_plan(b->ctxs[p], GTK_FREE, GTK_FREE);
+    if (!P) BFAIL(b, GTK_ESTATE, "a problem's pattern is gone: gtk_batch_build again");
+    const gtk::MatrixArgs a = matrix_args(b->ctxs[p], *P, *b->plans[p]);
+    if (std::memcmp(&a, &b->args[p], sizeof(a)) != 0) {
+      b->args[p] = a;
+      changed = true;
+    }
+  }
+  if (changed)
+    BHIPCHK(b, hipMemcpy(b->d_problems.p, b->args.data(), b->args.size() * sizeof(gtk::MatrixArgs),
+                         hipMemcpyHostToDevice));
+  BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+  BHIPCHK(b, hipGraphLaunch(b->gexec, b->stream));
+  BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  if (stats) {
+    float ms = 0;
+    BHIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    stats->kernel_ms = ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->cells = b->cells;
+    stats->nnz = b->nnz;
+    stats->bytes_alg = 0;
+    stats->flops_alg = 0;
   }
   return GTK_OK;
 }

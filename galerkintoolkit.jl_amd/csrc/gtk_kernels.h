@@ -1,6 +1,8 @@
 // gtk_kernels.h -- launch interface between the C ABI (gtk_abi.cpp) and the
 // gfx950 kernels (gtk_kernels.hip).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstddef>
 #include <cstdint>
 
@@ -32,6 +34,15 @@ struct MatrixArgs {
   int stage_cells;   // cells whose coordinates fit the accumulator space (matrix-core kernels)
   int lds_claim;     // byte offset in LDS of the batch counter (matrix-core kernels)
   int ntiles;        // tiles of the plan (matrix-core kernels loop over them)
+};
+
+// Many problems in one launch (matrix-core kernels of 2D quadrilaterals):
+// per-problem arguments and element type, and the flattened work queue.
+struct BatchedArgs {
+  const MatrixArgs *problems;
+  const int32_t *types;  // 0: Q1, 1: Q2, 2: Q3 quadrilaterals
+  const int2 *items;     // (problem, tile)
+  int nitems;
 };
 
 struct VectorArgs {
@@ -66,5 +77,11 @@ struct KernelInfo {
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out);
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);
+// Batched kernel over 2D Q1/Q2/Q3 quadrilaterals (type index of a problem's
+// element: batched_type; -1 if the element is not one of them).  One launch
+// shape: 8 waves per workgroup.
+constexpr int kBatchedWaves = 8;
+int batched_type(const ElemKey &key);
+bool find_batched_kernel(int form, int precision, KernelInfo &out);
 
 }  // namespace gtk

@@ -358,7 +358,8 @@ __device__ inline void load_batch(const unsigned char *__restrict__ rec, const M
   }
 }
 
-// Scalar forms on the matrix cores, one workgroup per tile, NW wavefronts.
+// Scalar forms on the matrix cores: one tile per workgroup of NW wavefronts
+// at a time.
 //
 // Phase A (all waves): node coordinates of the tile's cells are staged in
 // LDS (in chunks, in the space the accumulators use later); each thread
@@ -367,10 +368,10 @@ __device__ inline void load_batch(const unsigned char *__restrict__ rec, const M
 // and the factor G = kappa |det J| w_q J^-1 J^-T (mass: kappa |det J| w_q),
 // stored in LDS in the contraction precision CT.
 //
-// Phase B (each wave independently): batches of the tile, strided over the
-// waves.  A batch packs whole columns whose contributors (cell slot s,
-// local column b) total at most 16; for its 16 slots one product gives the
-// local columns
+// Phase B (each wave independently): batches of the tile, claimed from a
+// per-tile counter.  A batch packs whole columns whose contributors (cell
+// slot s, local column b) total at most 16; for its 16 slots one product
+// gives the local columns
 //   A[n, a] = sum_k W[n, k] Bt[k, a],  k = (q, d),
 //   W[n, (q,d)]  = (G_{s_n,q} grad phi_{b_n}(q))_d  (built per batch)
 //   Bt[(q,d), a] = d_d phi_a(q)                    (reference table, registers)
@@ -388,103 +389,315 @@ __device__ inline void load_batch(const unsigned char *__restrict__ rec, const M
 // column ends before e (an end mask per batch), and cleared.  The record of
 // the next batch is loaded while the current one is computed.
 template <int D, int NV, int NS, int NQ, int FORM, typename CT, typename OffT, int NW>
-__global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) {
-  constexpr int kB = NW * 64;
-  constexpr int NG = NGeo<D, FORM>::value;
-  constexpr int NGP = NG == 3 ? 4 : NG;  // staged factors per (cell, point)
-  constexpr int KD = FORM == kLaplace ? D : 1;
-  constexpr int TP = FORM == kLaplace ? (D == 3 ? 4 : 2) : 1;  // table entries per (q, b)
-  constexpr int QG = (NQ + 3) / 4;
-  constexpr int NQP = 4 * QG;  // quadrature rows padded with zero rows to the four lane groups
-  constexpr int GS = GeoStride<NQP, NGP, CT>::value;  // CT elements per cell of staged factors
-  constexpr int KS = QG * KD;
-  constexpr int MB = (NS + 15) / 16;
-  constexpr int CPR = kB / NQ;  // cells per geometry round
-  constexpr int KR = (kBatchSlotsK * NS + 63) / 64;  // write-out rounds of the largest batch
-  constexpr int PF = 2;  // staging entries per thread prefetched for the next tile
+struct BatchTile {
+  static constexpr int kB = NW * 64;
+  static constexpr int NG = NGeo<D, FORM>::value;
+  static constexpr int NGP = NG == 3 ? 4 : NG;  // staged factors per (cell, point)
+  static constexpr int KD = FORM == kLaplace ? D : 1;
+  static constexpr int TP = FORM == kLaplace ? (D == 3 ? 4 : 2) : 1;  // table entries per (q, b)
+  static constexpr int QG = (NQ + 3) / 4;
+  static constexpr int NQP = 4 * QG;  // quadrature rows padded with zero rows to the four lane groups
+  static constexpr int GS = GeoStride<NQP, NGP, CT>::value;  // CT elements per cell of staged factors
+  static constexpr int KS = QG * KD;
+  static constexpr int MB = (NS + 15) / 16;
+  static constexpr int CPR = kB / NQ;  // cells per geometry round
+  static constexpr int KR = (kBatchSlotsK * NS + 63) / 64;  // write-out rounds of the largest batch
   static_assert(KR <= 16, "end mask words are read from lanes 0..15");
+  static_assert(CPR >= 1, "block too small for the quadrature");
   using V4 = typename Acc4<CT>::type;
   using O4 = Off4<OffT>;
-  static_assert(CPR >= 1, "block too small for the quadrature");
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  CT *tab_s = reinterpret_cast<CT *>(lds_raw);                     // [NQP][NS][TP]
-  CT *G_s = reinterpret_cast<CT *>(lds_raw + args.lds_geo);        // [cells: GS][NQP][NGP]
-  double *acc_all = reinterpret_cast<double *>(lds_raw + args.acc_offset);
-  double *X_s = acc_all;                                            // phase A: [stage cells][NV][D]
-  double *K_s = X_s + (size_t)args.stage_cells * NV * D;            // phase A: coefficient per staged cell
-  int *claim_ctr = reinterpret_cast<int *>(lds_raw + args.lds_claim);
 
-  // once per workgroup: the reference table
-  for (int i = threadIdx.x; i < NQP * NS * TP; i += kB) {
-    const int q = i / (NS * TP), rem = i - q * NS * TP, b = rem / TP, d = rem - b * TP;
-    double v = 0.0;
-    if (q < NQ) {
-      if constexpr (FORM == kLaplace) v = d < D ? args.gphi[(q * D + d) * NS + b] : 0.0;
-      else v = args.phi[q * NS + b];
-    }
-    tab_s[i] = (CT)v;
+  struct Lds {
+    CT *tab;     // [NQP][NS][TP]
+    CT *G;       // [cells: GS][NQP][NGP]
+    double *acc;  // accumulators; phase A: staged coordinates and coefficients
+    double *X, *K;
+    int *claim;
+  };
+  __device__ static Lds lds(unsigned char *raw, const MatrixArgs &a) {
+    Lds l;
+    l.tab = reinterpret_cast<CT *>(raw);
+    l.G = reinterpret_cast<CT *>(raw + a.lds_geo);
+    l.acc = reinterpret_cast<double *>(raw + a.acc_offset);
+    l.X = l.acc;
+    l.K = l.X + (size_t)a.stage_cells * NV * D;
+    l.claim = reinterpret_cast<int *>(raw + a.lds_claim);
+    return l;
   }
 
-  // the next tile's first staging entries: cell and node ids and the cell
-  // coefficients, gathered into registers during the current tile's batches
-  // (one gather level per batch); the coordinates follow at the tile's start
+  // reference table into LDS (rows q >= NQ zero)
+  __device__ static void table(const MatrixArgs &a, CT *tab, int tid) {
+    for (int i = tid; i < NQP * NS * TP; i += kB) {
+      const int q = i / (NS * TP), rem = i - q * NS * TP, b = rem / TP, d = rem - b * TP;
+      double v = 0.0;
+      if (q < NQ) {
+        if constexpr (FORM == kLaplace) v = d < D ? a.gphi[(q * D + d) * NS + b] : 0.0;
+        else v = a.phi[q * NS + b];
+      }
+      tab[i] = (CT)v;
+    }
+  }
+
+  // staging of the (cell -> node -> coordinates, coefficient) gather of a
+  // chunk, entries [from, n): SU entries per thread and pass, each level
+  // issued for all of them at once
+  __device__ static void stage(const MatrixArgs &a, const Lds &l, int first_cell, int from, int n, int tid) {
+    constexpr int SU = 4;
+    for (int base = from; base < n; base += kB * SU) {
+      int64_t cellv[SU], nodev[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int i = base + u * kB + tid;
+        cellv[u] = i < n ? a.tile_cells[first_cell + i / NV] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int i = base + u * kB + tid;
+        nodev[u] = i < n ? a.cell_nodes[cellv[u] * NV + (i % NV)] : 0;
+        if (i < n && i % NV == 0) l.K[i / NV] = a.coef ? a.coef[cellv[u]] : 1.0;
+      }
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int i = base + u * kB + tid;
+        if (i < n) {
+#pragma unroll
+          for (int d = 0; d < D; d++) l.X[i * D + d] = a.xyz[nodev[u] * D + d];
+        }
+      }
+    }
+  }
+
+  // geometry factors of the nch staged cells (slots c0 ..) from the staged
+  // coordinates; thread tid takes quadrature point tid % NQ
+  __device__ static void geometry(const MatrixArgs &a, const Lds &l, int c0, int nch, int tid) {
+    const int tq = tid % NQ, tcs = tid / NQ;
+    if (tcs >= CPR) return;
+    double gq[NV][D];
+#pragma unroll
+    for (int n = 0; n < NV; n++)
+#pragma unroll
+      for (int d = 0; d < D; d++) gq[n][d] = a.gN[(tq * NV + n) * D + d];
+    const double wq = a.w[tq];
+    for (int sl = tcs; sl < nch; sl += CPR) {
+      const double *X = l.X + sl * NV * D;
+      double J[D][D];
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) J[i][j] = 0.0;
+#pragma unroll
+      for (int n = 0; n < NV; n++)
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+          for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
+      double Ji[D][D];
+      const double det = inverse<D>(J, Ji);
+      const double kdV = l.K[sl] * fabs(det) * wq;
+      CT *G = l.G + (c0 + sl) * GS + tq * NGP;
+      if constexpr (NQP > NQ) {  // padding rows of the cell: zero
+        if (tq < NQP - NQ) {
+#pragma unroll
+          for (int i = 0; i < NGP; i++) l.G[(c0 + sl) * GS + (NQ + tq) * NGP + i] = (CT)0;
+        }
+      }
+      if constexpr (FORM == kLaplace) {
+        CT gv[NGP];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+          for (int j = i; j < D; j++) {
+            double gg = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; m++) gg = fma(Ji[i][m], Ji[j][m], gg);
+            gv[sym_index<D>(i, j)] = (CT)(kdV * gg);
+          }
+        if constexpr (NGP > NG) gv[NGP - 1] = (CT)0;
+#pragma unroll
+        for (int i = 0; i < NGP; i++) G[i] = gv[i];
+      } else {
+        G[0] = (CT)kdV;
+      }
+    }
+  }
+
+  // the B operand of the batches (reference gradients or values)
+  __device__ static void bfrag_load(const CT *tab, int nl, int g, CT (&bfrag)[MB][KS]) {
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int a = mb * 16 + nl, q = g * QG + ks / KD, d = ks % KD;
+        bfrag[mb][ks] = (a < NS && q < NQ) ? tab[(q * NS + a) * TP + d] : (CT)0;
+      }
+  }
+
+  // one batch: W, the matrix-core product, the rank-step additions and the
+  // flat write-out
+  __device__ static void batch(const MatrixArgs &a, const Lds &l, double *acc, const CT (&bfrag)[MB][KS],
+                               const BatchRegs<OffT, MB> &cur, int lane, int sink) {
+    const int g = lane >> 4;
+    const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
+    const int tmax = (info >> 8) & 0xFF;
+    // W column of slot a_row(nl); empty slots read slot 0 (finite values,
+    // never scattered) and padding rows are zero in both operands
+    const int s = cur.sb & 0xFFFF, b = cur.sb >> 16;
+    const CT *Gc = l.G + s * GS + g * QG * NGP;
+    const CT *Tb = l.tab + (g * QG * NS + b) * TP;
+    V4 c[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < QG; j++) {
+      CT G[NGP], T[TP];
+      lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
+      lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
+      CT W[KD];
+      if constexpr (FORM == kLaplace) {
+        if constexpr (D == 3) {
+          W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
+          W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
+          W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+        } else {
+          W[0] = G[0] * T[0] + G[1] * T[1];
+          W[1] = G[1] * T[0] + G[2] * T[1];
+        }
+      } else {
+        W[0] = G[0] * T[0];
+      }
+#pragma unroll
+      for (int d = 0; d < KD; d++) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
+      }
+    }
+    // destinations of the 4 slots x MB rows this lane holds; rows outside
+    // the row set and padding go to a private per-lane slot past the
+    // accumulators
+    const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
+    int rank[4];
+    int dst[MB][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      rank[r] = (int)(at4[r] >> 16);
+      const int ab = (int)(at4[r] & 0xFFFF);
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+        const int off = O4::get(cur.off[mb], r);
+        dst[mb][r] = off != (int)O4::none ? ab + off : sink;
+      }
+    }
+    for (int t = 0; t < tmax; t++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (rank[r] == t) {
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+        }
+      }
+    }
+    // flat write-out of the batch's KR rounds of 64 accumulator entries:
+    // entry e goes to e + delta[c], c = column ends before e (the padding
+    // column sends the entries past the batch's columns to the value array's
+    // tail); all reads first, then the clears, then the stores, so every
+    // batch issues the same number of stores and the next batch's
+    // prefetched record is waited for without waiting for them
+    int ends = 0;
+    double v[KR];
+    int32_t z[KR];
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);  // k < 16: end-mask words
+      const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
+      const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+      ends += __builtin_popcount(lo) + __builtin_popcount(hi);
+      v[k] = acc[k * 64 + lane];
+      z[k] = k * 64 + lane + __shfl(cur.delta, cidx);
+    }
+#pragma unroll
+    for (int k = 0; k < KR; k++) acc[k * 64 + lane] = 0.0;
+#pragma unroll
+    for (int k = 0; k < KR; k++) a.nzval[(int64_t)z[k]] = v[k];
+  }
+
+  // the tile's batches, claimed from the per-tile counter (wave w starts
+  // with batch w; the result does not depend on which wave runs a batch);
+  // `between` runs after each batch of this wave
+  template <typename F>
+  __device__ static void batches(const MatrixArgs &a, const Lds &l, int tile, int tid, F &&between) {
+    const int wave = tid >> 6, lane = tid & 63;
+    double *acc = l.acc + (size_t)wave * a.acc_per_wave;
+    const int sink = a.acc_per_wave - 64 + lane;
+    for (int i = lane; i < a.acc_per_wave; i += 64) acc[i] = 0.0;
+    CT bfrag[MB][KS];
+    bfrag_load(l.tab, lane & 15, lane >> 4, bfrag);
+    const int bt0 = a.tile_batch_ptr[tile], bt1 = a.tile_batch_ptr[tile + 1];
+    auto claim = [&]() -> int {
+      int v = 0;
+      if (lane == 0) v = atomicAdd(l.claim, 1);
+      return bt0 + (int)__builtin_amdgcn_readfirstlane(v);
+    };
+    int bi = bt0 + wave, bn = bt1;
+    BatchRegs<OffT, MB> nxt;
+    if (bi < bt1) {
+      load_batch<CT, OffT, NS, MB>(a.rec, a, bi, lane, nxt);
+      bn = claim();
+    }
+    while (bi < bt1) {
+      const BatchRegs<OffT, MB> cur = nxt;
+      bi = bn;
+      if (bi < bt1) {
+        load_batch<CT, OffT, NS, MB>(a.rec, a, bi, lane, nxt);
+        bn = claim();
+      }
+      batch(a, l, acc, bfrag, cur, lane, sink);
+      between();
+    }
+  }
+
+  // one whole tile (table already in LDS): geometry then batches; the
+  // caller provides a barrier-separated start (nothing of the previous tile
+  // in use)
+  __device__ static void tile_simple(const MatrixArgs &a, const Lds &l, int tile, int tid) {
+    const int cell0 = a.tile_cell_ptr[tile];
+    const int ncell = a.tile_cell_ptr[tile + 1] - cell0;
+    if (tid == 0) *l.claim = NW;  // read after the geometry phase's barriers
+    for (int c0 = 0; c0 < ncell; c0 += a.stage_cells) {
+      const int nch = min(a.stage_cells, ncell - c0);
+      stage(a, l, cell0 + c0, 0, nch * NV, tid);
+      __syncthreads();
+      geometry(a, l, c0, nch, tid);
+      __syncthreads();
+    }
+    batches(a, l, tile, tid, [] {});
+  }
+};
+
+// One problem: a persistent grid, each workgroup looping over tiles; the
+// next tile's cell/node ids and coefficients are gathered into registers
+// during the current tile's batches (one gather level per batch), its
+// coordinates at the tile's start.
+template <int D, int NV, int NS, int NQ, int FORM, typename CT, typename OffT, int NW>
+__global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) {
+  using BT = BatchTile<D, NV, NS, NQ, FORM, CT, OffT, NW>;
+  constexpr int kB = BT::kB;
+  constexpr int PF = 2;  // staging entries per thread prefetched for the next tile
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const typename BT::Lds l = BT::lds(lds_raw, args);
+  BT::table(args, l.tab, threadIdx.x);
   int32_t pc[PF], pn[PF];
   double pk[PF];
   int pre_tile = -1, npre = 0;
-
   for (int tile = blockIdx.x; tile < args.ntiles; tile += gridDim.x) {
-    __syncthreads();  // the previous tile's batches are done with G, the accumulators and the counter
+    __syncthreads();  // the table is in place; the previous tile's batches are done
     // per-thread indices are rederived each tile (an opaque copy of the
     // thread id) so that their address arithmetic is not held in registers
     // across the whole tile loop
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    const int wave = tid >> 6, lane = tid & 63;
-    const int nl = lane & 15, g = lane >> 4;
-    double *acc = acc_all + (size_t)wave * args.acc_per_wave;
-    const int sink = args.acc_per_wave - 64 + lane;
-    const int tq = tid % NQ, tcs = tid / NQ;
-    const bool gl = tcs < CPR;
-    // staging of the (cell -> node -> coordinates, coefficient) gather of a
-    // chunk, entries [from, n): SU entries per thread and pass, each level
-    // issued for all of them at once
-    auto stage = [&](int first_cell, int from, int n) {
-      constexpr int SU = 4;
-      for (int base = from; base < n; base += kB * SU) {
-        int64_t cellv[SU], nodev[SU];
-  #pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const int i = base + u * kB + tid;
-          cellv[u] = i < n ? args.tile_cells[first_cell + i / NV] : 0;
-        }
-  #pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const int i = base + u * kB + tid;
-          nodev[u] = i < n ? args.cell_nodes[cellv[u] * NV + (i % NV)] : 0;
-          if (i < n && i % NV == 0) K_s[i / NV] = args.coef ? args.coef[cellv[u]] : 1.0;
-        }
-  #pragma unroll
-        for (int u = 0; u < SU; u++) {
-          const int i = base + u * kB + tid;
-          if (i < n) {
-  #pragma unroll
-            for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[nodev[u] * D + d];
-          }
-        }
-      }
-    };
     const int cell0 = args.tile_cell_ptr[tile];
     const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
-    if (tid == 0) *claim_ctr = NW;  // read after the geometry phase's barriers
-
-    // phase A: geometry; each thread keeps one quadrature point's geometry
-    // gradients in registers for this phase
-    double gq[NV][D];
-#pragma unroll
-    for (int n = 0; n < NV; n++)
-#pragma unroll
-      for (int d = 0; d < D; d++) gq[n][d] = gl ? args.gN[(tq * NV + n) * D + d] : 0.0;
-    const double wq = gl ? args.w[tq] : 0.0;
+    if (tid == 0) *l.claim = NW;  // read after the geometry phase's barriers
     for (int c0 = 0; c0 < ncell; c0 += args.stage_cells) {
       const int nch = min(args.stage_cells, ncell - c0);
       int from = 0;
@@ -494,70 +707,17 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
           const int i = u * kB + tid;
           if (i < npre) {
 #pragma unroll
-            for (int d = 0; d < D; d++) X_s[i * D + d] = args.xyz[(int64_t)pn[u] * D + d];
-            if (i % NV == 0) K_s[i / NV] = pk[u];
+            for (int d = 0; d < D; d++) l.X[i * D + d] = args.xyz[(int64_t)pn[u] * D + d];
+            if (i % NV == 0) l.K[i / NV] = pk[u];
           }
         }
         from = npre;
       }
-      stage(cell0 + c0, from, nch * NV);
+      BT::stage(args, l, cell0 + c0, from, nch * NV, tid);
       __syncthreads();
-      if (gl) {
-        for (int sl = tcs; sl < nch; sl += CPR) {
-          const double *X = X_s + sl * NV * D;
-          double J[D][D];
-#pragma unroll
-          for (int i = 0; i < D; i++)
-#pragma unroll
-            for (int j = 0; j < D; j++) J[i][j] = 0.0;
-#pragma unroll
-          for (int n = 0; n < NV; n++)
-#pragma unroll
-            for (int i = 0; i < D; i++)
-#pragma unroll
-              for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
-          double Ji[D][D];
-          const double det = inverse<D>(J, Ji);
-          const double kdV = K_s[sl] * fabs(det) * wq;
-          CT *G = G_s + (c0 + sl) * GS + tq * NGP;
-          if constexpr (NQP > NQ) {  // padding rows of the cell: zero
-            if (tq < NQP - NQ) {
-#pragma unroll
-              for (int i = 0; i < NGP; i++) G_s[(c0 + sl) * GS + (NQ + tq) * NGP + i] = (CT)0;
-            }
-          }
-          if constexpr (FORM == kLaplace) {
-            CT gv[NGP];
-#pragma unroll
-            for (int i = 0; i < D; i++)
-#pragma unroll
-              for (int j = i; j < D; j++) {
-                double gg = 0.0;
-#pragma unroll
-                for (int m = 0; m < D; m++) gg = fma(Ji[i][m], Ji[j][m], gg);
-                gv[sym_index<D>(i, j)] = (CT)(kdV * gg);
-              }
-            if constexpr (NGP > NG) gv[NGP - 1] = (CT)0;
-#pragma unroll
-            for (int i = 0; i < NGP; i++) G[i] = gv[i];
-          } else {
-            G[0] = (CT)kdV;
-          }
-        }
-      }
+      BT::geometry(args, l, c0, nch, tid);
       __syncthreads();
     }
-
-    // phase B: batches; the B operand (reference gradients) in registers
-    for (int i = lane; i < args.acc_per_wave; i += 64) acc[i] = 0.0;
-    CT bfrag[MB][KS];
-#pragma unroll
-    for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-      for (int ks = 0; ks < KS; ks++) {
-        const int a = mb * 16 + nl, q = g * QG + ks / KD, d = ks % KD;
-        bfrag[mb][ks] = (a < NS && q < NQ) ? tab_s[(q * NS + a) * TP + d] : (CT)0;
-      }
     const int ntile = tile + gridDim.x;
     int pf_cell0 = 0, pf_n = 0;
     if (ntile < args.ntiles) {
@@ -571,6 +731,7 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
     }
     bool pf_nodes = false;  // second gather level issued
     auto pf_step = [&]() {
+      if (pf_nodes) return;
 #pragma unroll
       for (int u = 0; u < PF; u++) {
         const int i = u * kB + tid;
@@ -579,116 +740,43 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
       }
       pf_nodes = true;
     };
-
-    const int bt0 = args.tile_batch_ptr[tile], bt1 = args.tile_batch_ptr[tile + 1];
-    // batches are claimed from a per-tile counter (wave w starts with batch
-    // w); the result does not depend on which wave runs a batch
-    auto claim = [&]() -> int {
-      int v = 0;
-      if (lane == 0) v = atomicAdd(claim_ctr, 1);
-      return bt0 + (int)__builtin_amdgcn_readfirstlane(v);
-    };
-    int bi = bt0 + wave, bn = bt1;
-    BatchRegs<OffT, MB> nxt;
-    if (bi < bt1) {
-      load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
-      bn = claim();
-    }
-    while (bi < bt1) {
-      const BatchRegs<OffT, MB> cur = nxt;
-      bi = bn;
-      if (bi < bt1) {
-        load_batch<CT, OffT, NS, MB>(args.rec, args, bi, lane, nxt);
-        bn = claim();
-      }
-      const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
-      const int tmax = (info >> 8) & 0xFF;
-      // W column of slot a_row(nl); empty slots read slot 0 (finite values,
-      // never scattered) and padding rows are zero in both operands
-      const int s = cur.sb & 0xFFFF, b = cur.sb >> 16;
-      const CT *Gc = G_s + s * GS + g * QG * NGP;
-      const CT *Tb = tab_s + (g * QG * NS + b) * TP;
-      V4 c[MB];
-#pragma unroll
-      for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
-#pragma unroll
-      for (int j = 0; j < QG; j++) {
-        CT G[NGP], T[TP];
-        lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
-        lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
-        CT W[KD];
-        if constexpr (FORM == kLaplace) {
-          if constexpr (D == 3) {
-            W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
-            W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
-            W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
-          } else {
-            W[0] = G[0] * T[0] + G[1] * T[1];
-            W[1] = G[1] * T[0] + G[2] * T[1];
-          }
-        } else {
-          W[0] = G[0] * T[0];
-        }
-#pragma unroll
-        for (int d = 0; d < KD; d++) {
-#pragma unroll
-          for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
-        }
-      }
-      // destinations of the 4 slots x MB rows this lane holds; rows outside
-      // the row set and padding go to a private per-lane slot past the
-      // accumulators
-      const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
-      int rank[4];
-      int dst[MB][4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        rank[r] = (int)(at4[r] >> 16);
-        const int ab = (int)(at4[r] & 0xFFFF);
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++) {
-          const int off = O4::get(cur.off[mb], r);
-          dst[mb][r] = off != (int)O4::none ? ab + off : sink;
-        }
-      }
-      for (int t = 0; t < tmax; t++) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          if (rank[r] == t) {
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
-          }
-        }
-      }
-      // flat write-out of the batch's KR rounds of 64 accumulator entries:
-      // entry e goes to e + delta[c], c = column ends before e (the padding
-      // column sends the entries past the batch's columns to the value
-      // array's tail); all reads first, then the clears, then the stores, so
-      // every batch issues the same number of stores and the next batch's
-      // prefetched record is waited for without waiting for them
-      {
-        int ends = 0;
-        double v[KR];
-        int32_t z[KR];
-#pragma unroll
-        for (int k = 0; k < KR; k++) {
-          const uint32_t lo = __builtin_amdgcn_readlane(cur.em.x, k);  // k < 16: end-mask words
-          const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
-          const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-          ends += __builtin_popcount(lo) + __builtin_popcount(hi);
-          v[k] = acc[k * 64 + lane];
-          z[k] = k * 64 + lane + __shfl(cur.delta, cidx);
-        }
-#pragma unroll
-        for (int k = 0; k < KR; k++) acc[k * 64 + lane] = 0.0;
-#pragma unroll
-        for (int k = 0; k < KR; k++) args.nzval[(int64_t)z[k]] = v[k];
-      }
-      if (!pf_nodes) pf_step();
-    }
-    if (!pf_nodes) pf_step();
+    BT::batches(args, l, tile, tid, pf_step);
+    pf_step();
     pre_tile = ntile;
     npre = pf_n;
+  }
+}
+
+// Many problems in one launch: a persistent grid over a flattened queue of
+// (problem, tile) items; each problem's arguments (its arrays, plan and LDS
+// layout) are in `problems`, its element type selects the tile body.  The
+// workgroup reloads the reference table when the problem changes.
+template <typename E, int FORM, typename CT, int NW>
+__device__ inline void batched_tile(unsigned char *raw, const MatrixArgs &a, int tile, bool new_problem) {
+  using BT = BatchTile<E::d, E::nv, E::ns, E::nq, FORM, CT, uint8_t, NW>;
+  const typename BT::Lds l = BT::lds(raw, a);
+  if (new_problem) {
+    BT::table(a, l.tab, threadIdx.x);
+    __syncthreads();
+  }
+  BT::tile_simple(a, l, tile, threadIdx.x);
+}
+
+template <int FORM, typename CT, int NW, typename E0, typename E1, typename E2>
+__global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  int prev = -1;
+  for (int item = blockIdx.x; item < ba.nitems; item += gridDim.x) {
+    __syncthreads();  // the previous tile (or problem) is done with LDS
+    const int2 it = ba.items[item];  // (problem, tile)
+    const MatrixArgs &a = ba.problems[it.x];
+    const bool np = it.x != prev;
+    prev = it.x;
+    switch (ba.types[it.x]) {
+      case 0: batched_tile<E0, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+      case 1: batched_tile<E1, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+      default: batched_tile<E2, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+    }
   }
 }
 
@@ -864,6 +952,29 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<HexQ1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexQ2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, waves, out);
+}
+
+int batched_type(const ElemKey &key) {
+  if (match<QuadQ1>(key)) return 0;
+  if (match<QuadQ2>(key)) return 1;
+  if (match<QuadQ3>(key)) return 2;
+  return -1;
+}
+
+template <int FORM, typename CT>
+static KernelInfo batched_info() {
+  KernelInfo k;
+  k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, kBatchedWaves, QuadQ1, QuadQ2, QuadQ3>);
+  k.mfma = true;
+  k.block = kBatchedWaves * 64;
+  return k;
+}
+
+bool find_batched_kernel(int form, int precision, KernelInfo &out) {
+  if (form == kLaplace) out = precision == 0 ? batched_info<kLaplace, double>() : batched_info<kLaplace, float>();
+  else if (form == kMass) out = precision == 0 ? batched_info<kMass, double>() : batched_info<kMass, float>();
+  else return false;
+  return true;
 }
 
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {

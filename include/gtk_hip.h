@@ -169,6 +169,32 @@ typedef struct gtk_solve_stats {
 int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rtol, int max_outer, int max_inner,
               double inner_rtol, gtk_solve_stats *stats);
 
+/* Batched assembly: many independent problems of uneven size assembled by
+ * one persistent kernel launch over a flattened (problem, tile) queue,
+ * captured once in a HIP graph.  A problem is a context on the batch's
+ * device with a (GTK_FREE, GTK_FREE) pattern built and its reference tables
+ * and coefficients set, on scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2
+ * Gauss points; form GTK_LAPLACE or GTK_MASS.  Each problem's values land
+ * in its own context (gtk_get_values, gtk_values_device_ptr).  Contexts
+ * must outlive the batch and have no *_async work pending at a run; new
+ * coefficients or coordinates set between runs are picked up, a new mesh,
+ * space or pattern needs gtk_batch_build again.
+ * No counterpart in the reference, which assembles one problem per
+ * GT.assemble_matrix call (src/problems.jl:319-350); a batch replaces a loop
+ * of such calls.  elem_type of gtk_batch_set_tile_cells: 0 Q1, 1 Q2, 2 Q3;
+ * max_cells caps the cells of a tile for that type (0: as many as fit in
+ * LDS), the knob the autotuner sets. */
+typedef struct gtk_batch gtk_batch;
+int gtk_batch_create(int device, int form, int precision, gtk_batch **out);
+int gtk_batch_destroy(gtk_batch *b);
+const char *gtk_batch_last_error(const gtk_batch *b);
+int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx);
+int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells);
+int gtk_batch_build(gtk_batch *b);
+/* One graph launch assembling every problem; stats->kernel_ms is its
+ * device time. */
+int gtk_batch_run(gtk_batch *b, gtk_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
