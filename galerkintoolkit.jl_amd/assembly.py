@@ -197,19 +197,22 @@ def update_matrix(A, cache, form=None):
     return A
 
 
-def assemble_vector(form, T, space, free_or_dirichlet=FREE, device=0):
-    """GT.assemble_vector(l, Float64, V; free_or_dirichlet) on the GPU."""
+def _load_data(form, space):
+    """(kind, values) of the source of a Load form."""
     if not isinstance(form, Load):
         raise TypeError("unsupported linear form")
-    a = _assembler(space, form.dOmega, device)
     f = form.f
-    nc = space.ncomp
     if isinstance(f, DiscreteField):
-        kind, data = L.GTK_F_NODAL, f.values
-    elif isinstance(f, FaceConstantField):
-        kind, data = L.GTK_F_CELL, f.values.reshape(-1)
-    else:
-        kind, data = L.GTK_F_CONST, np.broadcast_to(np.asarray(f, np.float64), (nc,)).copy()
+        return L.GTK_F_NODAL, f.values
+    if isinstance(f, FaceConstantField):
+        return L.GTK_F_CELL, f.values.reshape(-1)
+    return L.GTK_F_CONST, np.broadcast_to(np.asarray(f, np.float64), (space.ncomp,)).copy()
+
+
+def assemble_vector(form, T, space, free_or_dirichlet=FREE, device=0):
+    """GT.assemble_vector(l, Float64, V; free_or_dirichlet) on the GPU."""
+    kind, data = _load_data(form, space)
+    a = _assembler(space, form.dOmega, device)
     n = space.nfree if free_or_dirichlet == FREE else space.ndiri
     b, stats = a.ctx.assemble_vector(kind, data, free_or_dirichlet, n)
     return b

@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _lib as L
 from . import refcell as R
-from .assembly import FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _coefficients, _form_code
+from .assembly import (FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _coefficients, _form_code,
+                       _load_data)
 
 # element types of the batched kernel: scalar quadrilaterals of order 1, 2, 3
 # with (order+1)^2 Gauss points
@@ -115,10 +116,12 @@ class MatrixBatch:
     assembled together.
 
     forms[i] pairs with spaces[i]; forms[i].dOmega is that problem's
-    GT.measure.  tuning: "auto" (autotuner with its cache), a dict
-    {order: max_cells}, or None (tiles as large as LDS allows)."""
+    GT.measure.  loads: optional GT Load forms, one per problem (or None
+    entries), whose free-row vectors the same launch assembles.  tuning:
+    "auto" (autotuner with its cache), a dict {order: max_cells}, or None
+    (tiles as large as LDS allows)."""
 
-    def __init__(self, forms, spaces, precision=FP64, device=0, tuning="auto", cache=None):
+    def __init__(self, forms, spaces, precision=FP64, device=0, tuning="auto", cache=None, loads=None):
         if len(forms) != len(spaces) or not forms:
             raise ValueError("one form per space, at least one problem")
         kinds = {type(f) for f in forms}
@@ -149,6 +152,11 @@ class MatrixBatch:
         self.batch = L.Batch(device, self.form_code, self.precision)
         for a, _, _ in self.problems:
             self.batch.add(a.ctx)
+        self.loads = [None] * len(self.problems)
+        if loads is not None:
+            if len(loads) != len(self.problems):
+                raise ValueError("one load (or None) per problem")
+            self.set_loads(loads)
         for et, cap in self.tiles.items():
             self.batch.set_tile_cells(et, cap)
         self.batch.build()
@@ -165,9 +173,34 @@ class MatrixBatch:
             a.ctx.set_cell_coefficients(coef, ncoef)
         self.forms = list(forms)
 
+    def set_loads(self, loads):
+        """Load forms per problem (None: no vector); values of an existing
+        kind take effect at the next run, a new kind rebuilds the batch."""
+        rebuild = False
+        for i, (l, V) in enumerate(zip(loads, self.spaces)):
+            if l is None:
+                if self.loads[i] is not None:
+                    raise ValueError("a problem's load cannot be removed")
+                continue
+            if l.dOmega is not self.forms[i].dOmega:
+                raise ValueError("a load integrates over its problem's measure")
+            kind, data = _load_data(l, V)
+            old = self.loads[i]
+            rebuild |= old is None or _load_data(old, V)[0] != kind
+            self.batch.set_load(i, kind, data)
+            self.loads[i] = l
+        if rebuild and hasattr(self, "cells"):
+            self.batch.build()
+
     def run(self):
         """One launch assembling every problem; returns its gtk_stats."""
         return self.batch.run()
+
+    def vectors(self):
+        """The assembled free-row load vector of every problem with a load
+        (None for the others), from the last run."""
+        return [None if l is None else self.batch.get_vector(i, V.nfree)
+                for i, (l, V) in enumerate(zip(self.loads, self.spaces))]
 
     def matrices(self):
         """The assembled SparseMatrixCSC of every problem (values copied

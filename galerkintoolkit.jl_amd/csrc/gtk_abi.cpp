@@ -567,6 +567,57 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
 }
 
 // Kernel arguments of a work plan.
+// Tile plan and output of the load vector over a rows set, built on first use.
+static int get_vector_plan(gtk_ctx *ctx, int rows, VectorPlan *&out) {
+  auto &slot = ctx->vplans[rows];
+  if (!slot) {
+    auto vp = std::make_unique<VectorPlan>();
+    int max_cells, chunk;
+    vector_tile_limits(ctx, max_cells, chunk);
+    std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, 1 << 20, chunk, vp->tiles, nthreads_host());
+    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    int rc = upload_tiles(ctx, vp->tiles, vp->tile_cell_ptr, vp->tile_cells, vp->tile_col_ptr, vp->col_id,
+                          vp->col_ctr_ptr, vp->ctr_slot, vp->ctr_b);
+    if (rc) return rc;
+    vp->nrows = rows == GTK_FREE ? ctx->n_free : ctx->n_diri;
+    if ((rc = alloc(ctx, vp->out, (size_t)std::max<int64_t>(1, vp->nrows) * sizeof(double)))) return rc;
+    vp->valid = true;
+    slot = std::move(vp);
+  }
+  out = slot.get();
+  return GTK_OK;
+}
+
+// Values of a load of kind fkind: one per component, per cell and component,
+// or per DOF (free then Dirichlet).
+static size_t load_count(const gtk_ctx *ctx, int fkind) {
+  return fkind == GTK_F_CONST ? (size_t)ctx->ncomp
+                              : (fkind == GTK_F_CELL ? (size_t)ctx->n_cells * ctx->ncomp
+                                                     : (size_t)(ctx->n_free + ctx->n_diri));
+}
+
+static gtk::VectorArgs vector_args(const gtk_ctx *ctx, const VectorPlan &vp, const double *f) {
+  gtk::VectorArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.xyz = ctx->d_xyz.as<double>();
+  a.cell_nodes = ctx->d_cell_nodes.as<int32_t>();
+  a.cell_dofs = ctx->d_cell_dofs.as<int32_t>();
+  a.f = f;
+  a.n_free = ctx->n_free;
+  a.w = ctx->d_w.as<double>();
+  a.phi = ctx->d_phi.as<double>();
+  a.gN = ctx->d_gN.as<double>();
+  a.tile_cell_ptr = vp.tile_cell_ptr.as<int32_t>();
+  a.tile_cells = vp.tile_cells.as<int32_t>();
+  a.tile_col_ptr = vp.tile_col_ptr.as<int32_t>();
+  a.col_id = vp.col_id.as<int32_t>();
+  a.col_ctr_ptr = vp.col_ctr_ptr.as<int32_t>();
+  a.ctr_slot = vp.ctr_slot.as<uint16_t>();
+  a.ctr_b = vp.ctr_b.as<uint16_t>();
+  a.out = vp.out.as<double>();
+  return a;
+}
+
 static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, const ExecPlan &e) {
   const ExecPlan *ep = &e;
   gtk::MatrixArgs a;
@@ -742,46 +793,15 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
   gtk::KernelInfo k;
   if (!gtk::find_vector_kernel(elem_key(ctx), fkind, k)) FAIL(ctx, GTK_EUNSUP, "no vector kernel for this element");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  auto &slot = ctx->vplans[rows];
-  if (!slot) {
-    auto vp = std::make_unique<VectorPlan>();
-    int max_cells, chunk;
-    vector_tile_limits(ctx, max_cells, chunk);
-    std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, 1 << 20, chunk, vp->tiles, nthreads_host());
-    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
-    int rc = upload_tiles(ctx, vp->tiles, vp->tile_cell_ptr, vp->tile_cells, vp->tile_col_ptr, vp->col_id,
-                          vp->col_ctr_ptr, vp->ctr_slot, vp->ctr_b);
-    if (rc) return rc;
-    vp->nrows = rows == GTK_FREE ? ctx->n_free : ctx->n_diri;
-    if ((rc = alloc(ctx, vp->out, (size_t)std::max<int64_t>(1, vp->nrows) * sizeof(double)))) return rc;
-    vp->valid = true;
-    slot = std::move(vp);
-  }
-  VectorPlan &vp = *slot;
-  size_t nf = fkind == GTK_F_CONST ? (size_t)ctx->ncomp
-                                   : (fkind == GTK_F_CELL ? (size_t)ctx->n_cells * ctx->ncomp : (size_t)(ctx->n_free + ctx->n_diri));
-  int rc = upload(ctx, vp.f, f, nf);
+  VectorPlan *vpp = nullptr;
+  int rc = get_vector_plan(ctx, rows, vpp);
   if (rc) return rc;
+  VectorPlan &vp = *vpp;
+  if ((rc = upload(ctx, vp.f, f, load_count(ctx, fkind)))) return rc;
   size_t lds = k.table_bytes + (size_t)vp.tiles.max_tile_cells * k.geo_bytes_per_cell;
   if (lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
   if (lds > 64 * 1024) HIPCHK(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  gtk::VectorArgs a;
-  a.xyz = ctx->d_xyz.as<double>();
-  a.cell_nodes = ctx->d_cell_nodes.as<int32_t>();
-  a.cell_dofs = ctx->d_cell_dofs.as<int32_t>();
-  a.f = vp.f.as<double>();
-  a.n_free = ctx->n_free;
-  a.w = ctx->d_w.as<double>();
-  a.phi = ctx->d_phi.as<double>();
-  a.gN = ctx->d_gN.as<double>();
-  a.tile_cell_ptr = vp.tile_cell_ptr.as<int32_t>();
-  a.tile_cells = vp.tile_cells.as<int32_t>();
-  a.tile_col_ptr = vp.tile_col_ptr.as<int32_t>();
-  a.col_id = vp.col_id.as<int32_t>();
-  a.col_ctr_ptr = vp.col_ctr_ptr.as<int32_t>();
-  a.ctr_slot = vp.ctr_slot.as<uint16_t>();
-  a.ctr_b = vp.ctr_b.as<uint16_t>();
-  a.out = vp.out.as<double>();
+  gtk::VectorArgs a = vector_args(ctx, vp, vp.f.as<double>());
   void *args[] = {&a};
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (vp.tiles.ntiles)
@@ -932,7 +952,14 @@ struct gtk_batch {
   std::vector<std::unique_ptr<ExecPlan>> plans;  // one per problem
   int tile_cells[3] = {0, 0, 0};                 // cap on the cells of a tile per element type (0: LDS bound)
   std::vector<gtk::MatrixArgs> args;  // host copy of d_problems
-  DevBuf d_problems, d_types, d_items;
+  struct Load {
+    int fkind = -1;  // none
+    size_t count = 0;
+    DevBuf f;
+  };
+  std::vector<std::unique_ptr<Load>> loads;  // per problem
+  std::vector<gtk::VectorArgs> vargs;       // host copy of d_loads
+  DevBuf d_problems, d_loads, d_types, d_fkinds, d_items;
   gtk::KernelInfo k;
   size_t lds = 0;
   int nitems = 0, grid = 0;
@@ -1004,8 +1031,11 @@ int gtk_batch_destroy(gtk_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_release_graph(b);
   b->plans.clear();
+  b->loads.clear();
   b->d_problems.release();
+  b->d_loads.release();
   b->d_types.release();
+  b->d_fkinds.release();
   b->d_items.release();
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1024,6 +1054,40 @@ int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx) {
   if (ctx->ncoef > 1) BFAIL(b, GTK_EINVAL, "scalar forms take one coefficient per cell");
   batch_release_graph(b);
   b->ctxs.push_back(ctx);
+  b->loads.push_back(std::make_unique<gtk_batch::Load>());
+  return GTK_OK;
+}
+
+int gtk_batch_set_load(gtk_batch *b, int problem, int fkind, const double *f) {
+  if (!b) return GTK_EINVAL;
+  if (problem < 0 || problem >= (int)b->ctxs.size()) BFAIL(b, GTK_EINVAL, "problem index out of range");
+  if (fkind < GTK_F_CONST || fkind > GTK_F_NODAL || !f) BFAIL(b, GTK_EINVAL, "invalid load");
+  BHIPCHK(b, hipSetDevice(b->device));
+  gtk_batch::Load &l = *b->loads[problem];
+  const size_t count = load_count(b->ctxs[problem], fkind);
+  if (l.fkind != fkind || l.count != count || !l.f.p) {
+    // a new kind of load changes the work items: the batch is built again
+    batch_release_graph(b);
+    int rc = batch_upload(b, l.f, f, count);
+    if (rc) return rc;
+    l.fkind = fkind;
+    l.count = count;
+    return GTK_OK;
+  }
+  BHIPCHK(b, hipMemcpy(l.f.p, f, count * sizeof(double), hipMemcpyHostToDevice));
+  return GTK_OK;
+}
+
+int gtk_batch_get_vector(gtk_batch *b, int problem, double *out) {
+  if (!b || !out) return GTK_EINVAL;
+  if (problem < 0 || problem >= (int)b->ctxs.size()) BFAIL(b, GTK_EINVAL, "problem index out of range");
+  if (b->loads[problem]->fkind < 0) BFAIL(b, GTK_ESTATE, "the problem has no load (gtk_batch_set_load)");
+  gtk_ctx *ctx = b->ctxs[problem];
+  auto it = ctx->vplans.find(GTK_FREE);
+  if (it == ctx->vplans.end() || !it->second) BFAIL(b, GTK_ESTATE, "gtk_batch_build first");
+  BHIPCHK(b, hipSetDevice(b->device));
+  if (it->second->nrows)
+    BHIPCHK(b, hipMemcpy(out, it->second->out.p, it->second->nrows * sizeof(double), hipMemcpyDeviceToHost));
   return GTK_OK;
 }
 
@@ -1043,7 +1107,8 @@ int gtk_batch_build(gtk_batch *b) {
   if (!gtk::find_batched_kernel(b->form, b->precision, b->k)) BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
   b->plans.clear();
   std::vector<gtk::MatrixArgs> args;
-  std::vector<int32_t> types;
+  std::vector<gtk::VectorArgs> vargs(b->ctxs.size());
+  std::vector<int32_t> types, fkinds(b->ctxs.size(), -1);
   std::vector<int2> items;
   b->lds = 0;
   b->cells = b->nnz = 0;
@@ -1064,9 +1129,26 @@ int gtk_batch_build(gtk_batch *b) {
     b->nnz += P->pat.nnz;
     b->plans.push_back(std::move(ep));
   }
+  // load-vector tiles after every matrix tile: short items fill the tail
+  for (size_t p = 0; p < b->ctxs.size(); p++) {
+    const gtk_batch::Load &l = *b->loads[p];
+    if (l.fkind < 0) continue;
+    gtk_ctx *ctx = b->ctxs[p];
+    gtk::KernelInfo vk;
+    if (!gtk::find_vector_kernel(elem_key(ctx), l.fkind, vk)) BFAIL(b, GTK_EUNSUP, "no vector kernel for a problem");
+    VectorPlan *vp = nullptr;
+    if (get_vector_plan(ctx, GTK_FREE, vp)) BFAIL(b, GTK_EINVAL, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
+    b->lds = std::max(b->lds, vk.table_bytes + (size_t)vp->tiles.max_tile_cells * vk.geo_bytes_per_cell);
+    vargs[p] = vector_args(ctx, *vp, l.f.as<double>());
+    fkinds[p] = l.fkind;
+    for (int t = 0; t < vp->tiles.ntiles; t++) items.push_back(make_int2((int)p, ~t));
+  }
   int rc;
   b->args = args;
+  b->vargs = vargs;
   if ((rc = batch_upload(b, b->d_problems, args.data(), args.size()))) return rc;
+  if ((rc = batch_upload(b, b->d_loads, vargs.data(), vargs.size()))) return rc;
+  if ((rc = batch_upload(b, b->d_fkinds, fkinds.data(), fkinds.size()))) return rc;
   if ((rc = batch_upload(b, b->d_types, types.data(), types.size()))) return rc;
   if ((rc = batch_upload(b, b->d_items, items.data(), items.size()))) return rc;
   b->nitems = (int)items.size();
@@ -1080,7 +1162,9 @@ int gtk_batch_build(gtk_batch *b) {
   // the whole batch is one persistent launch, captured once as a graph
   gtk::BatchedArgs ba;
   ba.problems = b->d_problems.as<gtk::MatrixArgs>();
+  ba.loads = b->d_loads.as<gtk::VectorArgs>();
   ba.types = b->d_types.as<int32_t>();
+  ba.fkinds = b->d_fkinds.as<int32_t>();
   ba.items = b->d_items.as<int2>();
   ba.nitems = b->nitems;
   void *kargs[] = {&ba};
@@ -1112,6 +1196,22 @@ int gtk_batch_run(gtk_batch *b, gtk_stats *stats) {
   }
   if (changed)
     BHIPCHK(b, hipMemcpy(b->d_problems.p, b->args.data(), b->args.size() * sizeof(gtk::MatrixArgs),
+                         hipMemcpyHostToDevice));
+  changed = false;
+  for (size_t p = 0; p < b->ctxs.size(); p++) {
+    const gtk_batch::Load &l = *b->loads[p];
+    if (l.fkind < 0) continue;
+    auto it = b->ctxs[p]->vplans.find(GTK_FREE);
+    if (it == b->ctxs[p]->vplans.end() || !it->second)
+      BFAIL(b, GTK_ESTATE, "a problem's load-vector plan is gone: gtk_batch_build again");
+    const gtk::VectorArgs v = vector_args(b->ctxs[p], *it->second, l.f.as<double>());
+    if (std::memcmp(&v, &b->vargs[p], sizeof(v)) != 0) {
+      b->vargs[p] = v;
+      changed = true;
+    }
+  }
+  if (changed)
+    BHIPCHK(b, hipMemcpy(b->d_loads.p, b->vargs.data(), b->vargs.size() * sizeof(gtk::VectorArgs),
                          hipMemcpyHostToDevice));
   BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
   BHIPCHK(b, hipGraphLaunch(b->gexec, b->stream));

@@ -38,12 +38,7 @@ struct MatrixArgs {
 
 // Many problems in one launch (matrix-core kernels of 2D quadrilaterals):
 // per-problem arguments and element type, and the flattened work queue.
-struct BatchedArgs {
-  const MatrixArgs *problems;
-  const int32_t *types;  // 0: Q1, 1: Q2, 2: Q3 quadrilaterals
-  const int2 *items;     // (problem, tile)
-  int nitems;
-};
+
 
 struct VectorArgs {
   const double *xyz;
@@ -55,6 +50,17 @@ struct VectorArgs {
   const int32_t *tile_cell_ptr, *tile_cells, *tile_col_ptr, *col_id, *col_ctr_ptr;
   const uint16_t *ctr_slot, *ctr_b;
   double *out;
+};
+
+// A batch of independent problems in one launch: work items (problem, t)
+// are matrix tile t for t >= 0 and load-vector tile ~t for t < 0.
+struct BatchedArgs {
+  const MatrixArgs *problems;
+  const VectorArgs *loads;  // per problem (read for vector items only)
+  const int32_t *types;     // 0: Q1, 1: Q2, 2: Q3 quadrilaterals
+  const int32_t *fkinds;    // load kind per problem (kFConst, kFCell, kFNodal)
+  const int2 *items;
+  int nitems;
 };
 
 struct ElemKey {

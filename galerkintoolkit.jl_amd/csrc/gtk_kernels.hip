@@ -762,39 +762,22 @@ __device__ inline void batched_tile(unsigned char *raw, const MatrixArgs &a, int
   BT::tile_simple(a, l, tile, threadIdx.x);
 }
 
-template <int FORM, typename CT, int NW, typename E0, typename E1, typename E2>
-__global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  int prev = -1;
-  for (int item = blockIdx.x; item < ba.nitems; item += gridDim.x) {
-    __syncthreads();  // the previous tile (or problem) is done with LDS
-    const int2 it = ba.items[item];  // (problem, tile)
-    const MatrixArgs &a = ba.problems[it.x];
-    const bool np = it.x != prev;
-    prev = it.x;
-    switch (ba.types[it.x]) {
-      case 0: batched_tile<E0, FORM, CT, NW>(lds_raw, a, it.y, np); break;
-      case 1: batched_tile<E1, FORM, CT, NW>(lds_raw, a, it.y, np); break;
-      default: batched_tile<E2, FORM, CT, NW>(lds_raw, a, it.y, np); break;
-    }
-  }
-}
-
-template <int D, int NV, int NS, int NQ, int NC, int FKIND>
-__global__ __launch_bounds__(kBlock) void vector_tile_kernel(VectorArgs args) {
+// One tile of the load vector: f . dV at every (cell, point) of the tile
+// staged in LDS, then each owned row sums its contributions in cell order.
+// BS threads; the caller synchronises before LDS is reused.
+template <int D, int NV, int NS, int NQ, int NC, int FKIND, int BS>
+__device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned char *lds_raw) {
   constexpr int NDOF = NS * NC;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   double *phi_s = reinterpret_cast<double *>(lds_raw);  // [NQ][NS]
   double *F_s = phi_s + NQ * NS;                        // [cells][NQ][NC] = f . dV
 
-  const int tile = blockIdx.x;
   const int cell0 = args.tile_cell_ptr[tile];
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
   const int col0 = args.tile_col_ptr[tile];
   const int ncol = args.tile_col_ptr[tile + 1] - col0;
-  for (int i = threadIdx.x; i < NQ * NS; i += kBlock) phi_s[i] = args.phi[i];
+  for (int i = threadIdx.x; i < NQ * NS; i += BS) phi_s[i] = args.phi[i];
   __syncthreads();
-  for (int idx = threadIdx.x; idx < ncell * NQ; idx += kBlock) {
+  for (int idx = threadIdx.x; idx < ncell * NQ; idx += BS) {
     const int s = idx / NQ, q = idx - s * NQ;
     const int64_t cell = args.tile_cells[cell0 + s];
     double Ji[D][D];
@@ -819,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void vector_tile_kernel(VectorArgs args) {
     }
   }
   __syncthreads();
-  for (int ci = threadIdx.x; ci < ncol; ci += kBlock) {
+  for (int ci = threadIdx.x; ci < ncol; ci += BS) {
     const int pc = col0 + ci;
     const int64_t j = args.col_id[pc];
     double acc = 0.0;
@@ -833,6 +816,47 @@ __global__ __launch_bounds__(kBlock) void vector_tile_kernel(VectorArgs args) {
       acc += v;
     }
     args.out[j] = acc;
+  }
+}
+
+template <int D, int NV, int NS, int NQ, int NC, int FKIND>
+__global__ __launch_bounds__(kBlock) void vector_tile_kernel(VectorArgs args) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  vector_tile<D, NV, NS, NQ, NC, FKIND, kBlock>(args, blockIdx.x, lds_raw);
+}
+
+template <typename E, int BS>
+__device__ inline void batched_vector_tile(const VectorArgs &v, int tile, int fkind, unsigned char *raw) {
+  if (fkind == kFConst) vector_tile<E::d, E::nv, E::ns, E::nq, E::nc, kFConst, BS>(v, tile, raw);
+  else if (fkind == kFCell) vector_tile<E::d, E::nv, E::ns, E::nq, E::nc, kFCell, BS>(v, tile, raw);
+  else vector_tile<E::d, E::nv, E::ns, E::nq, E::nc, kFNodal, BS>(v, tile, raw);
+}
+
+template <int FORM, typename CT, int NW, typename E0, typename E1, typename E2>
+__global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  int prev = -1;
+  for (int item = blockIdx.x; item < ba.nitems; item += gridDim.x) {
+    __syncthreads();  // the previous tile (or problem) is done with LDS
+    const int2 it = ba.items[item];  // (problem, matrix tile t >= 0 or load-vector tile ~t)
+    if (it.y < 0) {
+      const int fk = ba.fkinds[it.x];
+      switch (ba.types[it.x]) {
+        case 0: batched_vector_tile<E0, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+        case 1: batched_vector_tile<E1, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+        default: batched_vector_tile<E2, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+      }
+      prev = -1;  // the reference tables in LDS are overwritten
+      continue;
+    }
+    const MatrixArgs &a = ba.problems[it.x];
+    const bool np = it.x != prev;
+    prev = it.x;
+    switch (ba.types[it.x]) {
+      case 0: batched_tile<E0, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+      case 1: batched_tile<E1, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+      default: batched_tile<E2, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+    }
   }
 }
 

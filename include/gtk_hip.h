@@ -180,8 +180,8 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
  * coefficients or coordinates set between runs are picked up, a new mesh,
  * space or pattern needs gtk_batch_build again.
  * No counterpart in the reference, which assembles one problem per
- * GT.assemble_matrix call (src/problems.jl:319-350); a batch replaces a loop
- * of such calls.  elem_type of gtk_batch_set_tile_cells: 0 Q1, 1 Q2, 2 Q3;
+ * GT.assemble_matrix / GT.assemble_vector call (src/problems.jl:244-274,
+ * :319-350); a batch replaces a loop of such calls.  elem_type of gtk_batch_set_tile_cells: 0 Q1, 1 Q2, 2 Q3;
  * max_cells caps the cells of a tile for that type (0: as many as fit in
  * LDS), the knob the autotuner sets. */
 typedef struct gtk_batch gtk_batch;
@@ -190,6 +190,13 @@ int gtk_batch_destroy(gtk_batch *b);
 const char *gtk_batch_last_error(const gtk_batch *b);
 int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx);
 int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells);
+/* Load vector of problem i (in gtk_batch_add order) over its free rows,
+ * assembled by the same launch: f as in gtk_assemble_vector.  New values of
+ * the same kind take effect at the next run; a new kind needs
+ * gtk_batch_build again.  gtk_batch_get_vector copies the n_free values of
+ * the last run to the host. */
+int gtk_batch_set_load(gtk_batch *b, int problem, int fkind, const double *f);
+int gtk_batch_get_vector(gtk_batch *b, int problem, double *out);
 int gtk_batch_build(gtk_batch *b);
 /* One graph launch assembling every problem; stats->kernel_ms is its
  * device time. */

@@ -40,6 +40,28 @@ def make_problems(GT, O, count, seed, nmin=2, nmax=14, with_oracle=True):
     return out
 
 
+def loads_of(GT, probs, seed):
+    """A load per problem, cycling through the three source kinds: seeded
+    Gaussian nodal values (free then Dirichlet), per-cell values, a
+    constant.  Returns the forms and the oracle's (kind, data)."""
+    forms, ref = [], []
+    for i, q in enumerate(probs):
+        V, dO = q["V"], q["dO"]
+        k = i % 3
+        if k == 0:
+            f = GT.inputs.normal(seed + i, V.nfree + V.ndiri)
+            forms.append(GT.Load(dO, GT.DiscreteField(V, f[:V.nfree], f[V.nfree:])))
+            ref.append((2, f))
+        elif k == 1:
+            f = GT.inputs.uniform(seed + i, q["m"].cells.shape[0]) - 0.5
+            forms.append(GT.Load(dO, GT.face_constant_field(f, q["Om"])))
+            ref.append((1, f))
+        else:
+            forms.append(GT.Load(dO, 1.5))
+            ref.append((0, np.array([1.5])))
+    return forms, ref
+
+
 def forms_of(GT, probs, kind, scale=1.0):
     F = GT.Laplace if kind == "laplace" else GT.Mass
     return [F(q["dO"], GT.face_constant_field(scale * q["kappa"], q["Om"])) for q in probs]
@@ -52,13 +74,19 @@ def test_batch_matches_oracle(GT, O, kind, precision):
     probs = make_problems(GT, O, 12, seed=41)
     prec = GT.FP64 if precision == "fp64" else GT.MIXED
     forms = forms_of(GT, probs, kind)
-    mats, b = GT.assemble_matrices(forms, [q["V"] for q in probs], precision=prec, tuning=None)
+    loads, lref = loads_of(GT, probs, 900)
+    b = GT.MatrixBatch(forms, [q["V"] for q in probs], precision=prec, tuning=None, loads=loads)
+    b.run()
+    mats = b.matrices()
     tol = TOL_FP64 if precision == "fp64" else TOL_MIXED
     code = O.LAPLACE if kind == "laplace" else O.MASS
     for q, A in zip(probs, mats):
         cp, rv, nz = q["p"].assemble_matrix(code, coef=q["kappa"])
         assert np.array_equal(A.colptr, cp) and np.array_equal(A.rowval, rv)
         assert relfro(A.nzval, nz) <= tol, (q["order"], q["m"].cells.shape)
+    # load vectors stay FP64 in both precisions
+    for q, v, (fk, fd) in zip(probs, b.vectors(), lref):
+        assert relfro(v, q["p"].assemble_vector(fk, fd)) <= TOL_FP64, (q["order"], fk)
     # the same problems one call each: identical per-cell arithmetic and
     # cell-ascending sums per nonzero, so the values agree to rounding
     for f, q, A in zip(forms, probs, mats):
@@ -97,13 +125,23 @@ def test_batch_new_coefficients(GT, O):
     print("case batch set_forms", flush=True)
     probs = make_problems(GT, O, 6, seed=77)
     spaces = [q["V"] for q in probs]
-    b = GT.MatrixBatch(forms_of(GT, probs, "laplace"), spaces, precision=GT.FP64, tuning=None)
+    loads, _ = loads_of(GT, probs, 300)
+    b = GT.MatrixBatch(forms_of(GT, probs, "laplace"), spaces, precision=GT.FP64, tuning=None, loads=loads)
     b.run()
     b.set_forms(forms_of(GT, probs, "laplace", scale=2.5))
+    loads, lref = loads_of(GT, probs, 301)  # same kinds, new values: no rebuild
+    b.set_loads(loads)
     b.run()
     for q, A in zip(probs, b.matrices()):
         cp, rv, nz = q["p"].assemble_matrix(O.LAPLACE, coef=2.5 * q["kappa"])
         assert relfro(A.nzval, nz) <= TOL_FP64
+    for q, v, (fk, fd) in zip(probs, b.vectors(), lref):
+        assert relfro(v, q["p"].assemble_vector(fk, fd)) <= TOL_FP64
+    # a new kind of load (constant everywhere) rebuilds the work items
+    b.set_loads([GT.Load(q["dO"], -2.0) for q in probs])
+    b.run()
+    for q, v in zip(probs, b.vectors()):
+        assert relfro(v, q["p"].assemble_vector(0, [-2.0])) <= TOL_FP64
     # kappa reset to 1 (no coefficient buffer): picked up as well
     b.set_forms([GT.Laplace(q["dO"]) for q in probs])
     b.run()
