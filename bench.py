@@ -226,14 +226,133 @@ def bench_elasticity(GT, args, n):
     }
 
 
+def batch_problems(GT, count, seed, nmin=16, nmax=256):
+    """`count` unit-square Poisson problems: n per side uniform in
+    [nmin, nmax], Lagrange order uniform in {1, 2, 3}, Gauss with order+1
+    points per direction, Dirichlet on the whole boundary, lognormal kappa
+    per cell and Gaussian nodal f (seeded per problem)."""
+    u = GT.inputs.uniform(seed, 2 * count)
+    probs = []
+    for i in range(count):
+        n = nmin + int(u[2 * i] * (nmax - nmin + 1))
+        order = 1 + int(u[2 * i + 1] * 3)
+        m = GT.cartesian_mesh((0.0, 1.0, 0.0, 1.0), (n, n))
+        Om = GT.interior(m)
+        V = GT.lagrange_space(Om, order, dirichlet_boundary=GT.boundary(m))
+        dO = GT.measure(Om, 2 * order)
+        kappa = GT.inputs.lognormal(seed + 1 + i, m.cells.shape[0])
+        f = GT.inputs.normal(seed + 100_000 + i, V.nfree + V.ndiri)
+        probs.append(dict(n=n, order=order, m=m, Om=Om, V=V, dO=dO, kappa=kappa, f=f))
+    return probs
+
+
+def cpu_baseline_batch(probs, stride):
+    """The oracle (single thread) on every `stride`-th problem: matrix and
+    load vector, cells/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker, timed here only as the CPU baseline
+    sample = probs[::stride]
+    cells, dt = 0, 0.0
+    for q in sample:
+        p = O.Problem((q["n"], q["n"])).space(q["order"], dirichlet=1)
+        p.quadrature(2 * q["order"])
+        t0 = time.perf_counter()
+        p.assemble_matrix(O.LAPLACE, coef=q["kappa"])
+        p.assemble_vector(2, q["f"])
+        dt += time.perf_counter() - t0
+        cells += q["m"].cells.shape[0]
+    return {"value": cells / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"every {stride}th problem of the batch ({len(sample)} problems, {cells} cells): FP64 matrix "
+                      f"+ nodal load vector + COO->CSC in the reference loop order, single thread, {dt:.2f} s"}
+
+
+def bench_batch(GT, args, count):
+    """Batched uneven 2D Poisson: every problem's matrix and load vector in
+    one persistent launch replayed from a HIP graph."""
+    t_setup = time.perf_counter()
+    probs = batch_problems(GT, count, args.seed)
+    forms = [GT.Laplace(q["dO"], GT.face_constant_field(q["kappa"], q["Om"])) for q in probs]
+    loads = [GT.Load(q["dO"], GT.DiscreteField(q["V"], q["f"][:q["V"].nfree], q["f"][q["V"].nfree:]))
+             for q in probs]
+    cache = args.tune_cache or None
+    bm = GT.MatrixBatch(forms, [q["V"] for q in probs], precision=GT.MIXED, tuning="auto", cache=cache, loads=loads)
+    b64 = bm.with_precision(GT.FP64)
+    setup_s = time.perf_counter() - t_setup
+    cells = bm.cells
+    nnz = 0
+    nbytes = 0
+    for q, (a, _, nc) in zip(probs, bm.problems):
+        pnnz = a.pattern(GT.FREE, GT.FREE)[0]
+        V = q["V"]
+        nd = V.cell_dofs.shape[1]
+        nnz += pnnz
+        nbytes += alg_bytes(2, q["m"].coords.shape[0], nc, 4, nd, 1, pnnz)
+        nbytes += 8 * (V.nfree + V.ndiri) + 8 * V.nfree  # nodal f in, load vector out
+
+    def timed(b):
+        for _ in range(args.warmup):
+            b.run()
+        kern = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            kern.append(b.run().kernel_ms)
+        return (time.perf_counter() - t0) * 1e3 / args.steps, float(np.mean(kern))
+
+    wall64, kern64 = timed(b64)
+    ref = [A.nzval.copy() for A in b64.matrices()]
+    vref = b64.vectors()
+    wallmx, kernmx = timed(bm)
+    errs = np.array([np.linalg.norm(A.nzval - r) / np.linalg.norm(r) for A, r in zip(bm.matrices(), ref)])
+    verrs = np.array([np.linalg.norm(v - r) / np.linalg.norm(r) for v, r in zip(bm.vectors(), vref)])
+    orders = [q["order"] for q in probs]
+    achieved = nbytes / (kernmx * 1e-3) / 1e9
+    out = {
+        "metric": "aggregate cells/s, batched uneven 2D Poisson (matrix + load vector per problem), one graph launch",
+        "value": cells / (wallmx * 1e-3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wallmx,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 contraction, fp64 geometry + accumulation",
+        "data": "synthetic",
+        "config": {"workload": f"{count} unit-square problems, n per side U[16,256], Lagrange order U{{1,2,3}}, "
+                               f"(order+1)^2 Gauss, lognormal kappa per cell, Gaussian nodal f, Dirichlet boundary "
+                               f"(seed {args.seed})",
+                   "name": "batch", "problems": count, "cells": int(cells), "nnz": int(nnz),
+                   "orders": {str(o): orders.count(o) for o in (1, 2, 3)}, "parallelism": "single GPU",
+                   "launch": {name: {"waves": b.waves,
+                                     "tiles": {GT.batch.ELEM_NAMES[k]: v for k, v in b.tiles.items()}}
+                              for name, b in (("mixed", bm), ("fp64", b64))}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": int(nbytes)},
+        "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3)},
+        "mixed": {"kernel_ms": kernmx, "speedup_vs_fp64": kern64 / kernmx,
+                  "rel_fro_err_vs_fp64": {"max": float(errs.max()), "median": float(np.median(errs))},
+                  "load_vector_rel_err_vs_fp64_max": float(verrs.max())},
+        "tuning": {"mixed": bm.tuning_report, "fp64": b64.tuning_report},
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(probs, args.batch_cpu_stride)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["q2hex", "q1hex", "p2tet"], default="q2hex",
+    ap.add_argument("--workload", choices=["q2hex", "q1hex", "p2tet", "batch"], default="q2hex",
                     help="q2hex: Q2 hex 64^3 perturbed Laplace (the headline); q1hex: Q1 hex 128^3 with lognormal "
-                         "kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve")
+                         "kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve; batch: 256 uneven 2D "
+                         "Poisson problems in one graph launch")
+    ap.add_argument("--problems", type=int, default=256, help="problems of the batch workload")
+    ap.add_argument("--batch-cpu-stride", type=int, default=2, help="CPU baseline: every k-th problem of the batch")
+    ap.add_argument("--tune-cache", default="", help="autotuner cache file (default: the user cache)")
     ap.add_argument("--cells", type=int, default=0, help="cells per direction (default per workload)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -255,8 +374,10 @@ def main():
             out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.seed)
     elif args.workload == "q1hex":
         out = bench_laplace(GT, args, args.cells or 128, 1, 2, 0.0, True, "q1hex")
-    else:
+    elif args.workload == "p2tet":
         out = bench_elasticity(GT, args, args.cells or 48)
+    else:
+        out = bench_batch(GT, args, args.problems)
     print(json.dumps(out))
 
 

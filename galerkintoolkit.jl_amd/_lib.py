@@ -22,7 +22,7 @@ EXPORTS = [
     "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
-    "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_load", "gtk_batch_get_vector",
+    "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_waves", "gtk_batch_set_load", "gtk_batch_get_vector",
     "gtk_batch_build", "gtk_batch_run",
 ]
 
@@ -83,6 +83,7 @@ def load():
     L.gtk_batch_last_error.restype = C.c_char_p
     L.gtk_batch_add.argtypes = [vp, vp]
     L.gtk_batch_set_tile_cells.argtypes = [vp, i, i]
+    L.gtk_batch_set_waves.argtypes = [vp, i]
     L.gtk_batch_set_load.argtypes = [vp, i, i, f64]
     L.gtk_batch_get_vector.argtypes = [vp, i, f64]
     L.gtk_batch_build.argtypes = [vp]
@@ -233,6 +234,9 @@ class Batch:
     def set_tile_cells(self, elem_type, max_cells):
         self.check(self.L.gtk_batch_set_tile_cells(self.h, elem_type, max_cells), "gtk_batch_set_tile_cells")
 
+    def set_waves(self, waves):
+        self.check(self.L.gtk_batch_set_waves(self.h, waves), "gtk_batch_set_waves")
+
     def set_load(self, problem, fkind, f):
         self.check(self.L.gtk_batch_set_load(self.h, problem, fkind, np.ascontiguousarray(f, np.float64)),
                    "gtk_batch_set_load")
@@ -244,6 +248,7 @@ class Batch:
 
     def build(self):
         self.check(self.L.gtk_batch_build(self.h), "gtk_batch_build")
+        self.built = True
 
     def run(self):
         st = GtkStats()

@@ -24,7 +24,9 @@ ELEM_TYPES = {1: 0, 2: 1, 3: 2}
 ELEM_NAMES = {0: "quad-q1", 1: "quad-q2", 2: "quad-q3"}
 
 # tile sizes the autotuner tries (cells per tile; 0 = as many as fit in LDS)
-TILE_CANDIDATES = (0, 256, 160, 96, 64, 40, 24)
+TILE_CANDIDATES = (0, 256, 160, 96, 64, 40)
+# waves per workgroup it tries
+WAVE_CANDIDATES = (8, 12)
 
 
 def elem_type(space, dOmega=None):
@@ -70,44 +72,63 @@ def _time_batch(batch, repeats):
     return min(batch.run().kernel_ms for _ in range(repeats))
 
 
-def autotune(problems, form_code, precision, device=0, cache=None, candidates=TILE_CANDIDATES, sample=8,
-             repeats=3, retune=False):
-    """Cells per tile for each element type present in `problems` (a list of
-    (assembler, elem_type, cells)).  A type without a cached choice is tuned
-    on a sample of its problems (evenly spread over their sizes): each
-    candidate is built into a batch of the sample and timed by its kernel
-    time per cell; the fastest is cached under (library build, form,
-    precision, element type).  Returns ({type: max_cells}, report)."""
+def autotune(problems, form_code, precision, device=0, cache=None, candidates=TILE_CANDIDATES,
+             waves=WAVE_CANDIDATES, sample=8, repeats=3, retune=False):
+    """Launch shape of a batch of `problems` (a list of (assembler,
+    elem_type, cells)): waves per workgroup, and cells per tile for each
+    element type present.  An element type without a cached entry is tuned
+    on a sample of its problems (evenly spread over their sizes): every
+    (waves, cells) candidate is built into a batch of the sample and timed by
+    its kernel time per cell, and the best tile per wave count is cached under
+    (library build, form, precision, element type).  The batch then takes the
+    wave count with the least predicted time over its own cell mix.
+    Returns (waves, {type: max_cells}, report)."""
     path = default_cache_path() if cache is None else cache
     table = _read_cache(path)
-    choice, report = {}, {}
+    report = {}
+    entries = {}
+    cells_of = {}
     for et in sorted({p[1] for p in problems}):
-        key = _cache_key(form_code, precision, et)
-        if not retune and key in table:
-            choice[et] = int(table[key]["max_cells"])
-            report[ELEM_NAMES[et]] = {"max_cells": choice[et], "cached": True}
-            continue
         mine = sorted((p for p in problems if p[1] == et), key=lambda p: p[2])
+        cells_of[et] = sum(p[2] for p in mine)
+        key = _cache_key(form_code, precision, et)
+        e = table.get(key)
+        if not retune and isinstance(e, dict) and all(str(w) in e for w in waves):
+            entries[et] = e
+            report[ELEM_NAMES[et]] = {"cached": True}
+            continue
         pick = np.unique(np.linspace(0, len(mine) - 1, min(sample, len(mine))).round().astype(int))
         subset = [mine[i] for i in pick]
         cells = sum(p[2] for p in subset)
-        times = {}
-        for cap in candidates:
-            b = L.Batch(device, form_code, precision)
-            for a, _, _ in subset:
-                b.add(a.ctx)
-            b.set_tile_cells(et, cap)
-            b.build()
-            times[cap] = _time_batch(b, repeats)
-            b.close()
-        best = min(times, key=lambda c: times[c])
-        choice[et] = best
-        table[key] = {"max_cells": best, "ns_per_cell": {str(c): 1e6 * t / cells for c, t in times.items()},
-                      "sample_cells": cells}
-        report[ELEM_NAMES[et]] = {"max_cells": best, "cached": False,
-                                  "ns_per_cell": {c: round(1e6 * t / cells, 4) for c, t in times.items()}}
+        e = {}
+        for w in waves:
+            times = {}
+            for cap in candidates:
+                b = L.Batch(device, form_code, precision)
+                for a, _, _ in subset:
+                    b.add(a.ctx)
+                b.set_waves(w)
+                b.set_tile_cells(et, cap)
+                b.build()
+                times[cap] = 1e6 * _time_batch(b, repeats) / cells
+                b.close()
+            best = min(times, key=lambda c: times[c])
+            e[str(w)] = {"max_cells": best, "ns_per_cell": times[best],
+                         "candidates": {str(c): round(t, 4) for c, t in times.items()}}
+        e["sample_cells"] = cells
+        table[key] = entries[et] = e
+        report[ELEM_NAMES[et]] = {"cached": False}
     _write_cache(path, table)
-    return choice, report
+    predicted = {w: sum(cells_of[et] * entries[et][str(w)]["ns_per_cell"] for et in entries) for w in waves}
+    w = min(predicted, key=lambda k: predicted[k])
+    tiles = {et: int(entries[et][str(w)]["max_cells"]) for et in entries}
+    for et in entries:
+        r = report[ELEM_NAMES[et]]
+        r["max_cells"] = tiles[et]
+        r["ns_per_cell"] = {str(k): round(entries[et][str(k)]["ns_per_cell"], 4) for k in waves}
+    report["waves"] = w
+    report["predicted_ms"] = {str(k): round(v * 1e-6, 4) for k, v in predicted.items()}
+    return w, tiles, report
 
 
 class MatrixBatch:
@@ -142,13 +163,7 @@ class MatrixBatch:
             coef, ncoef = _coefficients(form, V.mesh.cells.shape[0])
             a.ctx.set_cell_coefficients(coef, ncoef)
             self.problems.append((a, et, V.mesh.cells.shape[0]))
-        if tuning == "auto":
-            self.tiles, self.tuning_report = autotune(self.problems, self.form_code, self.precision, device, cache)
-        elif tuning is None:
-            self.tiles, self.tuning_report = {}, None
-        else:
-            self.tiles = {ELEM_TYPES[o]: int(c) for o, c in dict(tuning).items()}
-            self.tuning_report = None
+        self._tune(tuning, cache)
         self.batch = L.Batch(device, self.form_code, self.precision)
         for a, _, _ in self.problems:
             self.batch.add(a.ctx)
@@ -157,10 +172,47 @@ class MatrixBatch:
             if len(loads) != len(self.problems):
                 raise ValueError("one load (or None) per problem")
             self.set_loads(loads)
+        self.batch.set_waves(self.waves)
         for et, cap in self.tiles.items():
             self.batch.set_tile_cells(et, cap)
         self.batch.build()
         self.cells = sum(p[2] for p in self.problems)
+        self.cache = cache
+        self.tuning = tuning
+
+    def _tune(self, tuning, cache):
+        """tuning: "auto", None (8 waves, tiles as large as LDS allows) or a
+        dict {order: max_cells, "waves": w}."""
+        self.tuning_report = None
+        if tuning == "auto":
+            self.waves, self.tiles, self.tuning_report = autotune(self.problems, self.form_code, self.precision,
+                                                                  self.device, cache)
+        elif tuning is None:
+            self.waves, self.tiles = 8, {}
+        else:
+            t = dict(tuning)
+            self.waves = int(t.pop("waves", 8))
+            self.tiles = {ELEM_TYPES[o]: int(c) for o, c in t.items()}
+
+    def with_precision(self, precision, tuning=None):
+        """A batch over the same problems and device contexts (patterns,
+        coefficients, loads) in another precision.  Both write the same value
+        arrays: read one's matrices before running the other."""
+        other = object.__new__(MatrixBatch)
+        other.__dict__.update(self.__dict__)
+        other.precision = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+        other.forms, other.loads = list(self.forms), [None] * len(self.problems)
+        other._tune(self.tuning if tuning is None else tuning, self.cache)
+        other.batch = L.Batch(self.device, self.form_code, other.precision)
+        for a, _, _ in self.problems:
+            other.batch.add(a.ctx)
+        if any(l is not None for l in self.loads):
+            other.set_loads(self.loads)
+        other.batch.set_waves(other.waves)
+        for et, cap in other.tiles.items():
+            other.batch.set_tile_cells(et, cap)
+        other.batch.build()
+        return other
 
     def set_forms(self, forms):
         """New coefficients for the same problems (same form kind); the next
@@ -189,7 +241,7 @@ class MatrixBatch:
             rebuild |= old is None or _load_data(old, V)[0] != kind
             self.batch.set_load(i, kind, data)
             self.loads[i] = l
-        if rebuild and hasattr(self, "cells"):
+        if rebuild and getattr(self.batch, "built", False):
             self.batch.build()
 
     def run(self):

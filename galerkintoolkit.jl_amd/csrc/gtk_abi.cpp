@@ -213,10 +213,12 @@ static int tile_cells_for(const gtk::KernelInfo &k, size_t acc_bytes, size_t bud
   return (int)std::max<size_t>(8, std::min<size_t>(4095, avail / geo));
 }
 
-// Tile limits of the vector kernel (no accumulators).
+// Tile limits of the vector kernel: per cell f . dV at the points, the node
+// coordinates and the nodal values; tables phi and the geometry gradients.
 static void vector_tile_limits(const gtk_ctx *ctx, int &max_cells, int &chunk) {
-  const size_t geo = (size_t)ctx->nq * 8 * ctx->ncomp;
-  const size_t tables = (size_t)ctx->nq * ctx->nshape * 8;
+  const size_t fs = (size_t)ctx->nq * ctx->ncomp | 1;  // odd per-cell stride of f . dV
+  const size_t geo = (fs + (size_t)ctx->nv * ctx->dim + (size_t)ctx->nshape * ctx->ncomp) * 8;
+  const size_t tables = (size_t)ctx->nq * (((size_t)ctx->nshape | 1) + ((size_t)ctx->nv * ctx->dim | 1)) * 8;
   const size_t avail = kLdsBudget > tables + 1024 ? kLdsBudget - tables - 1024 : 8 * geo;
   max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, avail / geo));
   chunk = tile_chunk(ctx, max_cells);
@@ -951,6 +953,7 @@ struct gtk_batch {
   std::vector<gtk_ctx *> ctxs;
   std::vector<std::unique_ptr<ExecPlan>> plans;  // one per problem
   int tile_cells[3] = {0, 0, 0};                 // cap on the cells of a tile per element type (0: LDS bound)
+  int waves = 8;                                 // waves per workgroup
   std::vector<gtk::MatrixArgs> args;  // host copy of d_problems
   struct Load {
     int fkind = -1;  // none
@@ -1099,12 +1102,21 @@ int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells) {
   return GTK_OK;
 }
 
+int gtk_batch_set_waves(gtk_batch *b, int waves) {
+  if (!b) return GTK_EINVAL;
+  if (waves != 8 && waves != 12) BFAIL(b, GTK_EINVAL, "waves per workgroup: 8 or 12");
+  if (waves != b->waves) batch_release_graph(b);
+  b->waves = waves;
+  return GTK_OK;
+}
+
 int gtk_batch_build(gtk_batch *b) {
   if (!b) return GTK_EINVAL;
   if (b->ctxs.empty()) BFAIL(b, GTK_ESTATE, "gtk_batch_add problems first");
   BHIPCHK(b, hipSetDevice(b->device));
   batch_release_graph(b);
-  if (!gtk::find_batched_kernel(b->form, b->precision, b->k)) BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
+  if (!gtk::find_batched_kernel(b->form, b->precision, b->waves, b->k))
+    BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
   b->plans.clear();
   std::vector<gtk::MatrixArgs> args;
   std::vector<gtk::VectorArgs> vargs(b->ctxs.size());
@@ -1117,7 +1129,7 @@ int gtk_batch_build(gtk_batch *b) {
     MatrixPlan *P = find_plan(ctx, GTK_FREE, GTK_FREE);
     const int type = gtk::batched_type(elem_key(ctx));
     std::unique_ptr<ExecPlan> ep;
-    int rc = build_exec(ctx, *P, b->form, b->precision, gtk::kBatchedWaves, b->tile_cells[type], ep);
+    int rc = build_exec(ctx, *P, b->form, b->precision, b->waves, b->tile_cells[type], ep);
     if (rc) BFAIL(b, rc, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
     if (!ep->k.mfma || ep->bat.off_bytes != 1)
       BFAIL(b, GTK_EUNSUP, "a problem does not fit the matrix-core batches with byte offsets");

@@ -84,10 +84,9 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
                         KernelInfo &out);
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);
 // Batched kernel over 2D Q1/Q2/Q3 quadrilaterals (type index of a problem's
-// element: batched_type; -1 if the element is not one of them).  One launch
-// shape: 8 waves per workgroup.
-constexpr int kBatchedWaves = 8;
+// element: batched_type; -1 if the element is not one of them), 8 or 12
+// waves per workgroup.
 int batched_type(const ElemKey &key);
-bool find_batched_kernel(int form, int precision, KernelInfo &out);
+bool find_batched_kernel(int form, int precision, int waves, KernelInfo &out);
 
 }  // namespace gtk

@@ -762,60 +762,164 @@ __device__ inline void batched_tile(unsigned char *raw, const MatrixArgs &a, int
   BT::tile_simple(a, l, tile, threadIdx.x);
 }
 
-// One tile of the load vector: f . dV at every (cell, point) of the tile
-// staged in LDS, then each owned row sums its contributions in cell order.
-// BS threads; the caller synchronises before LDS is reused.
+// per-cell stride (doubles) of the vector tile's f . dV array: the next odd
+// count, so that lanes on different cells fall on different LDS banks
+template <int N>
+struct VecCellStride {
+  static constexpr int value = N % 2 ? N : N + 1;
+};
+
+template <int D>
+__device__ inline double determinant(const double J[D][D]) {
+  if constexpr (D == 2) {
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  } else {
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    return J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  }
+}
+
+// One tile of the load vector.  Each cell's node coordinates (and, for a
+// discrete field, its nodal values) are read from HBM once into LDS; then
+// f . dV at every (cell, point) and the element vectors are formed in LDS,
+// and each owned row sums its contributions in cell order.  LDS: the tables (phi, order-1 geometry
+// gradients) and, per cell, (NQ*NC + NV*D + NS*NC) doubles.  BS threads; the
+// caller synchronises before LDS is reused.
 template <int D, int NV, int NS, int NQ, int NC, int FKIND, int BS>
 __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned char *lds_raw) {
   constexpr int NDOF = NS * NC;
-  double *phi_s = reinterpret_cast<double *>(lds_raw);  // [NQ][NS]
-  double *F_s = phi_s + NQ * NS;                        // [cells][NQ][NC] = f . dV
-
+  // odd strides: lanes on different points (or cells) fall on different LDS banks
+  constexpr int PS = VecCellStride<NS>::value;       // phi row
+  constexpr int GR = VecCellStride<NV * D>::value;   // geometry-gradient row
+  constexpr int FS = VecCellStride<NQ * NC>::value;  // f . dV of a cell
+  constexpr int U = 4;                                // staging items per thread in flight
   const int cell0 = args.tile_cell_ptr[tile];
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
   const int col0 = args.tile_col_ptr[tile];
   const int ncol = args.tile_col_ptr[tile + 1] - col0;
-  for (int i = threadIdx.x; i < NQ * NS; i += BS) phi_s[i] = args.phi[i];
+  double *phi_s = reinterpret_cast<double *>(lds_raw);  // [NQ][PS]
+  double *gN_s = phi_s + NQ * PS;                       // [NQ][GR]: [NV][D] used
+  double *F_s = gN_s + NQ * GR;                         // [cells][FS]: f . dV at the points
+  double *X_s = F_s + ncell * FS;                       // [cells][NV][D]
+  double *fc_s = X_s + ncell * NV * D;                  // [cells][NDOF] (discrete field)
+  int32_t *cid_s = reinterpret_cast<int32_t *>(F_s);    // cell ids while staging
+  for (int i = threadIdx.x; i < NQ * NS; i += BS) phi_s[(i / NS) * PS + i % NS] = args.phi[i];
+  for (int i = threadIdx.x; i < NQ * NV * D; i += BS) gN_s[(i / (NV * D)) * GR + i % (NV * D)] = args.gN[i];
+  for (int i = threadIdx.x; i < ncell; i += BS) cid_s[i] = args.tile_cells[cell0 + i];
+  __syncthreads();
+  // node coordinates (node id -> xyz) and nodal values (DOF id -> f, free
+  // then Dirichlet), each level issued for U items before the next
+  for (int i0 = threadIdx.x; i0 < ncell * NV; i0 += U * BS) {
+    int64_t node[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int idx = i0 + u * BS;
+      node[u] = idx < ncell * NV ? args.cell_nodes[(int64_t)cid_s[idx / NV] * NV + idx % NV] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int idx = i0 + u * BS;
+      if (idx < ncell * NV) {
+#pragma unroll
+        for (int i = 0; i < D; i++) X_s[idx * D + i] = args.xyz[node[u] * D + i];
+      }
+    }
+  }
+  if constexpr (FKIND == kFNodal) {
+    for (int i0 = threadIdx.x; i0 < ncell * NDOF; i0 += U * BS) {
+      int32_t dof[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = i0 + u * BS;
+        dof[u] = idx < ncell * NDOF ? args.cell_dofs[(int64_t)cid_s[idx / NDOF] * NDOF + idx % NDOF] : 1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = i0 + u * BS;
+        if (idx < ncell * NDOF) fc_s[idx] = dof[u] > 0 ? args.f[dof[u] - 1] : args.f[args.n_free - dof[u] - 1];
+      }
+    }
+  }
   __syncthreads();
   for (int idx = threadIdx.x; idx < ncell * NQ; idx += BS) {
     const int s = idx / NQ, q = idx - s * NQ;
-    const int64_t cell = args.tile_cells[cell0 + s];
-    double Ji[D][D];
-    const double dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji) * args.w[q];
+    double J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) J[i][j] = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) J[i][j] = fma(X_s[(s * NV + v) * D + i], gN_s[q * GR + v * D + j], J[i][j]);
+    const double dV = fabs(determinant<D>(J)) * args.w[q];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       double f;
       if constexpr (FKIND == kFConst) {
         f = args.f[c];
       } else if constexpr (FKIND == kFCell) {
-        f = args.f[cell * NC + c];
+        f = args.f[(int64_t)args.tile_cells[cell0 + s] * NC + c];
       } else {
         // discrete field: sum_a f[dof(cell, a)] phi_a(q)
         f = 0.0;
-        for (int n = 0; n < NS; n++) {
-          const int32_t dof = args.cell_dofs[cell * NDOF + n * NC + c];
-          const double fv = dof > 0 ? args.f[dof - 1] : args.f[args.n_free - dof - 1];
-          f = fma(fv, phi_s[q * NS + n], f);
-        }
+#pragma unroll 4
+        for (int n = 0; n < NS; n++) f = fma(fc_s[s * NDOF + n * NC + c], phi_s[q * PS + n], f);
       }
-      F_s[(s * NQ + q) * NC + c] = f * dV;
+      F_s[s * FS + q * NC + c] = f * dV;
     }
   }
   __syncthreads();
-  for (int ci = threadIdx.x; ci < ncol; ci += BS) {
-    const int pc = col0 + ci;
-    const int64_t j = args.col_id[pc];
-    double acc = 0.0;
-    const int k1 = args.col_ctr_ptr[pc + 1];
-    for (int k = args.col_ctr_ptr[pc]; k < k1; k++) {
-      const int s = args.ctr_slot[k], b = args.ctr_b[k];
-      const int bn = b / NC, bc = b - bn * NC;
-      double v = 0.0;
-#pragma unroll 4
-      for (int q = 0; q < NQ; q++) v = fma(phi_s[q * NS + bn], F_s[(s * NQ + q) * NC + bc], v);
-      acc += v;
+  // element vectors b[s][k] = sum_q phi[q][n] F[s][q][c], k = n*NC + c, KB
+  // entries per thread: a thread reads its cell's F once per point and the
+  // wave shares the phi reads.  b reuses the nodal values' LDS.
+  constexpr int KB = NDOF < 8 ? NDOF : 8;
+  constexpr int NKB = (NDOF + KB - 1) / KB;
+  double *b_s = fc_s;  // [cells][NDOF]
+  for (int idx = threadIdx.x; idx < ncell * NKB; idx += BS) {
+    const int s = idx / NKB, k0 = (idx - s * NKB) * KB;
+    double acc[KB];
+#pragma unroll
+    for (int u = 0; u < KB; u++) acc[u] = 0.0;
+#pragma unroll 2
+    for (int q = 0; q < NQ; q++) {
+#pragma unroll
+      for (int u = 0; u < KB; u++) {
+        const int k = k0 + u < NDOF ? k0 + u : NDOF - 1;
+        const int n = k / NC, c = k - n * NC;
+        acc[u] = fma(phi_s[q * PS + n], F_s[s * FS + q * NC + c], acc[u]);
+      }
     }
-    args.out[j] = acc;
+#pragma unroll
+    for (int u = 0; u < KB; u++)
+      if (k0 + u < NDOF) b_s[s * NDOF + k0 + u] = acc[u];
+  }
+  __syncthreads();
+  // rows, U per thread in flight: row id and contribution range, then the
+  // contributions in cell order
+  for (int c0 = threadIdx.x; c0 < ncol; c0 += U * BS) {
+    int j[U], k0[U], k1[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int ci = c0 + u * BS;
+      const int pc = col0 + (ci < ncol ? ci : 0);
+      j[u] = args.col_id[pc];
+      k0[u] = args.col_ctr_ptr[pc];
+      k1[u] = ci < ncol ? args.col_ctr_ptr[pc + 1] : k0[u];
+    }
+    double acc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      acc[u] = 0.0;
+      for (int k = k0[u]; k < k1[u]; k++) acc[u] += b_s[args.ctr_slot[k] * NDOF + args.ctr_b[k]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (c0 + u * BS < ncol) args.out[j[u]] = acc[u];
   }
 }
 
@@ -913,8 +1017,9 @@ template <typename E, int FK>
 static KernelInfo vector_info() {
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&vector_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FK>);
-  k.table_bytes = (size_t)E::nq * E::ns * sizeof(double);
-  k.geo_bytes_per_cell = (size_t)E::nq * E::nc * sizeof(double);
+  // same sizes as vector_tile's LDS layout (and the host's tile limits)
+  k.table_bytes = (size_t)E::nq * (VecCellStride<E::ns>::value + VecCellStride<E::nv * E::d>::value) * sizeof(double);
+  k.geo_bytes_per_cell = (size_t)(VecCellStride<E::nq * E::nc>::value + E::nv * E::d + E::ns * E::nc) * sizeof(double);
   k.cols_per_wave = 64;
   return k;
 }
@@ -985,18 +1090,25 @@ int batched_type(const ElemKey &key) {
   return -1;
 }
 
-template <int FORM, typename CT>
+template <int FORM, typename CT, int NW>
 static KernelInfo batched_info() {
   KernelInfo k;
-  k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, kBatchedWaves, QuadQ1, QuadQ2, QuadQ3>);
+  k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, QuadQ1, QuadQ2, QuadQ3>);
   k.mfma = true;
-  k.block = kBatchedWaves * 64;
+  k.block = NW * 64;
   return k;
 }
 
-bool find_batched_kernel(int form, int precision, KernelInfo &out) {
-  if (form == kLaplace) out = precision == 0 ? batched_info<kLaplace, double>() : batched_info<kLaplace, float>();
-  else if (form == kMass) out = precision == 0 ? batched_info<kMass, double>() : batched_info<kMass, float>();
+template <int FORM>
+static KernelInfo batched_info(int precision, int waves) {
+  if (waves == 12) return precision == 0 ? batched_info<FORM, double, 12>() : batched_info<FORM, float, 12>();
+  return precision == 0 ? batched_info<FORM, double, 8>() : batched_info<FORM, float, 8>();
+}
+
+bool find_batched_kernel(int form, int precision, int waves, KernelInfo &out) {
+  if (waves != 8 && waves != 12) return false;
+  if (form == kLaplace) out = batched_info<kLaplace>(precision, waves);
+  else if (form == kMass) out = batched_info<kMass>(precision, waves);
   else return false;
   return true;
 }

@@ -190,6 +190,9 @@ int gtk_batch_destroy(gtk_batch *b);
 const char *gtk_batch_last_error(const gtk_batch *b);
 int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx);
 int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells);
+/* Waves per workgroup of the launch: 8 (default) or 12, the autotuner's
+ * other knob. */
+int gtk_batch_set_waves(gtk_batch *b, int waves);
 /* Load vector of problem i (in gtk_batch_add order) over its free rows,
  * assembled by the same launch: f as in gtk_assemble_vector.  New values of
  * the same kind take effect at the next run; a new kind needs

@@ -108,7 +108,7 @@ def test_batch_tile_sizes_and_replays(GT, O):
     b0.run()
     ref = [A.nzval.copy() for A in b0.matrices()]
     b0.close()
-    b1 = GT.MatrixBatch(forms, spaces, precision=GT.MIXED, tuning={1: 24, 2: 24, 3: 24})
+    b1 = GT.MatrixBatch(forms, spaces, precision=GT.MIXED, tuning={1: 24, 2: 24, 3: 24, "waves": 12})
     other = GT.assemble_matrix(GT.Mass(probs[0]["dO"]), float, spaces[0], spaces[0])  # eager work on another form
     for rep in range(4):
         b1.run()
@@ -160,15 +160,17 @@ def test_batch_autotune_cache(GT, O, tmp_path):
     b = GT.MatrixBatch(forms, spaces, precision=GT.MIXED, tuning="auto", cache=cache)
     rep = b.tuning_report
     orders = {q["order"] for q in probs}
-    assert len(rep) == len(orders) and not any(r["cached"] for r in rep.values())
-    for r in rep.values():
-        assert r["max_cells"] in GT.batch.TILE_CANDIDATES
+    types = [GT.batch.ELEM_NAMES[GT.batch.ELEM_TYPES[o]] for o in orders]
+    assert all(not rep[t]["cached"] for t in types)
+    for t in types:
+        assert rep[t]["max_cells"] in GT.batch.TILE_CANDIDATES
+    assert rep["waves"] in GT.batch.WAVE_CANDIDATES and b.waves == rep["waves"]
     b.run()
     vals = [A.nzval.copy() for A in b.matrices()]
     b.close()
     b2 = GT.MatrixBatch(forms, spaces, precision=GT.MIXED, tuning="auto", cache=cache)
-    assert all(r["cached"] for r in b2.tuning_report.values())
-    assert b2.tiles == b.tiles
+    assert all(b2.tuning_report[t]["cached"] for t in types)
+    assert b2.tiles == b.tiles and b2.waves == b.waves
     b2.run()
     for v, A in zip(vals, b2.matrices()):
         assert np.array_equal(v, A.nzval)
