@@ -37,17 +37,17 @@ def load_package():
     return mod
 
 
-def measured_traffic(n):
-    """HBM bytes per launch from the committed counter summary of this
-    workload (rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain copy), or
-    None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_q2hex64.json")
+def measured_traffic(name, **match):
+    """HBM bytes per launch from the committed counter summary of a workload
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain copy), or None when
+    no summary matches it."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{name}.json")
     try:
         with open(path) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    if d.get("cells_per_dir") != n:
+    if any(d.get(k) != v for k, v in match.items()):
         return None
     return d["bytes_per_launch"]
 
@@ -136,7 +136,8 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         wallmx, kernmx = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_MIXED, args.steps, args.warmup)
     nd = V.fe.nnodes
     nq = len(dO.w)
-    traffic = measured_traffic(n) if (order, degree, perturb, lognormal_kappa) == (2, 4, 0.2, False) else None
+    headline = (order, degree, perturb, lognormal_kappa) == (2, 4, 0.2, False)
+    traffic = measured_traffic("q2hex64", cells_per_dir=n) if headline else None
     bytes_alg = alg_bytes(3, mesh.coords.shape[0], cells, 8, nd, 1 if lognormal_kappa else 0, A64.nnz)
     flops_alg = cells * nq * (2 * 3 * nd * nd + 2 * 9 * nd + 2 * 8 * 9)
     achieved = bytes_alg / (kernmx * 1e-3) / 1e9
@@ -278,6 +279,8 @@ def bench_batch(GT, args, count):
     bm = GT.MatrixBatch(forms, [q["V"] for q in probs], precision=GT.MIXED, tuning="auto", cache=cache, loads=loads)
     b64 = bm.with_precision(GT.FP64)
     setup_s = time.perf_counter() - t_setup
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
     cells = bm.cells
     nnz = 0
     nbytes = 0
@@ -306,6 +309,7 @@ def bench_batch(GT, args, count):
     verrs = np.array([np.linalg.norm(v - r) / np.linalg.norm(r) for v, r in zip(bm.vectors(), vref)])
     orders = [q["order"] for q in probs]
     achieved = nbytes / (kernmx * 1e-3) / 1e9
+    traffic = measured_traffic("batch256", problems=count, seed=args.seed)
     out = {
         "metric": "aggregate cells/s, batched uneven 2D Poisson (matrix + load vector per problem), one graph launch",
         "value": cells / (wallmx * 1e-3),
@@ -328,8 +332,10 @@ def bench_batch(GT, args, count):
                                      "tiles": {GT.batch.ELEM_NAMES[k]: v for k, v in b.tiles.items()}}
                               for name, b in (("mixed", bm), ("fp64", b64))}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": int(nbytes)},
-        "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic and traffic["mixed"],
+                     "alg_bytes": int(nbytes)},
+        "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3),
+                 "traffic": traffic and traffic["fp64"]},
         "mixed": {"kernel_ms": kernmx, "speedup_vs_fp64": kern64 / kernmx,
                   "rel_fro_err_vs_fp64": {"max": float(errs.max()), "median": float(np.median(errs))},
                   "load_vector_rel_err_vs_fp64_max": float(verrs.max())},
