@@ -134,6 +134,10 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         wall64, kern64 = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_FP64, args.steps, args.warmup)
     if args.only in ("both", "mixed"):
         wallmx, kernmx = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_MIXED, args.steps, args.warmup)
+    # the same mixed assembly through the host boundary: launch + the values'
+    # copy to host memory (PCIe), wall time of gtk_assemble_matrix
+    pcie_ms = min(GT.update_matrix(Amx, (cache[0], form, GT.FREE, GT.FREE, L.GTK_MIXED)).stats.total_ms
+                  for _ in range(3))
     nd = V.fe.nnodes
     nq = len(dO.w)
     headline = (order, degree, perturb, lognormal_kappa) == (2, 4, 0.2, False)
@@ -171,6 +175,8 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err,
                   "speedup_vs_fp64": kern64 / kernmx},
         "dofs_per_s": V.nfree / (wallmx * 1e-3),
+        "host_values": {"ms": pcie_ms, "cells_per_s": cells / (pcie_ms * 1e-3),
+                        "note": "mixed assembly with the values copied to host memory (not the value)"},
         "bytes_alg_per_launch": bytes_alg,
         "setup_s": setup_s,
     }
