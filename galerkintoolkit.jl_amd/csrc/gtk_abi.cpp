@@ -477,7 +477,11 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
   const gtk::ElemKey key_e = elem_key(ctx);
   const int nd = ctx->dofs_per_cell;
   std::string e;
-  bool mfma = gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, waves, ep->k) && ep->k.mfma;
+  // GTK_MATRIX_KERNEL=valu forces the VALU kernel (measurement)
+  const char *kenv = std::getenv("GTK_MATRIX_KERNEL");
+  const bool allow_mfma = !(kenv && std::strcmp(kenv, "valu") == 0);
+  bool mfma = allow_mfma && gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, waves, ep->k) &&
+              ep->k.mfma;
   if (mfma) {
     // accumulator bound: a batch adds at most nd rows per slot
     const size_t acc_bound = (size_t)waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
@@ -503,9 +507,16 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
     // batch entries + a private sink per lane; at least the write-out rounds
     ep->acc_per_wave = (std::max(ep->bat.max_batch_acc + 64, 64 * ep->bat.rounds) + 1) & ~1;
     const size_t acc_bytes = (size_t)waves * ep->acc_per_wave * sizeof(double);
-    // coordinates and coefficient of a staged cell
-    ep->stage_cells = (int)std::max<size_t>(1, acc_bytes / (((size_t)ctx->nv * ctx->dim + 1) * sizeof(double)));
-    ep->lds_claim = (int)(ep->acc_offset + acc_bytes);
+    // staged coordinates and coefficient of a cell; staging reuses the
+    // accumulators and the LDS the budget leaves after them, so that a tile
+    // is staged in as few passes (each a gather and two barriers) as fit
+    const size_t stage_per_cell = ((size_t)ctx->nv * ctx->dim + 1) * sizeof(double);
+    const size_t budget = std::max<size_t>(ctx->lds_budget, (size_t)ep->acc_offset + acc_bytes + 16);
+    const size_t stage_room = std::max(acc_bytes, budget - ep->acc_offset - 16);
+    ep->stage_cells = (int)std::max<size_t>(
+        1, std::min<size_t>(std::max(ep->tiles.max_tile_cells, 1), stage_room / stage_per_cell));
+    const size_t stage_bytes = std::max(acc_bytes, (size_t)ep->stage_cells * stage_per_cell);
+    ep->lds_claim = (int)(ep->acc_offset + align16(stage_bytes));
     ep->lds = ep->lds_claim + 16;
   } else {
     if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, waves, ep->k))
