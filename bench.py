@@ -134,6 +134,18 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         wall64, kern64 = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_FP64, args.steps, args.warmup)
     if args.only in ("both", "mixed"):
         wallmx, kernmx = time_form(ctx, L, L.GTK_LAPLACE, L.GTK_MIXED, args.steps, args.warmup)
+    # symbolic phase of the same block on host and device (fresh contexts)
+    from gtk_amd import assembly as AS
+    sym = {}
+    for where in ("host", "device"):
+        a_sym = AS._Assembler(V, dO)
+        t0 = time.perf_counter()
+        if where == "host":
+            a_sym.ctx.build_pattern(L.GTK_FREE, L.GTK_FREE)
+        else:
+            sym["device_kernels_ms"] = a_sym.ctx.build_pattern_device(L.GTK_FREE, L.GTK_FREE)[2]
+        sym[f"{where}_ms"] = (time.perf_counter() - t0) * 1e3
+        a_sym.ctx.close()
     # the same mixed assembly through the host boundary: launch + the values'
     # copy to host memory (PCIe), wall time of gtk_assemble_matrix
     pcie_ms = min(GT.update_matrix(Amx, (cache[0], form, GT.FREE, GT.FREE, L.GTK_MIXED)).stats.total_ms
@@ -175,6 +187,8 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err,
                   "speedup_vs_fp64": kern64 / kernmx},
         "dofs_per_s": V.nfree / (wallmx * 1e-3),
+        "symbolic": {**sym, "note": "pattern of the block: host C++ (threads) vs GPU kernels; device_ms includes "
+                                   "the copy of colptr/rowval to host memory"},
         "host_values": {"ms": pcie_ms, "cells_per_s": cells / (pcie_ms * 1e-3),
                         "note": "mixed assembly with the values copied to host memory (not the value)"},
         "bytes_alg_per_launch": bytes_alg,

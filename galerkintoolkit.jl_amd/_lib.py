@@ -19,7 +19,8 @@ GTK_F_CONST, GTK_F_CELL, GTK_F_NODAL = 0, 1, 2
 # every symbol the header declares
 EXPORTS = [
     "gtk_create", "gtk_destroy", "gtk_last_error", "gtk_version", "gtk_set_mesh", "gtk_update_coordinates",
-    "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_get_pattern", "gtk_set_cell_coefficients",
+    "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_build_pattern_device", "gtk_local_nnz_map",
+    "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
     "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_waves", "gtk_batch_set_load", "gtk_batch_get_vector",
@@ -68,6 +69,8 @@ def load():
     L.gtk_set_reference.argtypes = [vp, i, f64, i, f64, f64, f64]
     L.gtk_build_pattern.argtypes = [vp, i, i, C.POINTER(i64), C.POINTER(i64)]
     L.gtk_get_pattern.argtypes = [vp, i, i, i32, i32]
+    L.gtk_build_pattern_device.argtypes = [vp, i, i, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_double)]
+    L.gtk_local_nnz_map.argtypes = [vp, i, i, i32]
     L.gtk_set_cell_coefficients.argtypes = [vp, i, C.c_void_p]
     L.gtk_assemble_matrix.argtypes = [vp, i, i, i, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_get_values.argtypes = [vp, i, i, f64]
@@ -141,6 +144,17 @@ class Context:
         nnz, ncols = C.c_int64(), C.c_int64()
         self.check(self.L.gtk_build_pattern(self.h, rows, cols, C.byref(nnz), C.byref(ncols)), "gtk_build_pattern")
         return nnz.value, ncols.value
+
+    def build_pattern_device(self, rows, cols):
+        nnz, ncols, ms = C.c_int64(), C.c_int64(), C.c_double()
+        self.check(self.L.gtk_build_pattern_device(self.h, rows, cols, C.byref(nnz), C.byref(ncols), C.byref(ms)),
+                   "gtk_build_pattern_device")
+        return nnz.value, ncols.value, ms.value
+
+    def local_nnz_map(self, rows, cols, n_cells, nd):
+        out = np.zeros(max(n_cells * nd * nd, 1), np.int32)
+        self.check(self.L.gtk_local_nnz_map(self.h, rows, cols, out), "gtk_local_nnz_map")
+        return out[:n_cells * nd * nd].reshape(n_cells, nd, nd)
 
     def get_pattern(self, rows, cols, nnz, ncols):
         colptr = np.zeros(ncols + 1, np.int32)

@@ -120,10 +120,17 @@ class _Assembler:
         self.ctx.set_reference(dOmega.w, phi, gphi, gN)
         self.patterns = {}
 
-    def pattern(self, rows, cols):
+    def pattern(self, rows, cols, symbolic="host"):
+        """CSC pattern of the (rows, cols) block, built once: symbolic="host"
+        (threaded C++) or "device" (GPU kernels; bit-identical)."""
         key = (rows, cols)
         if key not in self.patterns:
-            nnz, ncols = self.ctx.build_pattern(rows, cols)
+            if symbolic == "device":
+                nnz, ncols, _ = self.ctx.build_pattern_device(rows, cols)
+            elif symbolic == "host":
+                nnz, ncols = self.ctx.build_pattern(rows, cols)
+            else:
+                raise ValueError("symbolic is 'host' or 'device'")
             colptr, rowval = self.ctx.get_pattern(rows, cols, nnz, ncols)
             nrows = self.space.nfree if rows == FREE else self.space.ndiri
             self.patterns[key] = (nnz, ncols, nrows, colptr, rowval)
@@ -164,8 +171,10 @@ def _coefficients(form, ncells):
 
 
 def assemble_matrix(form, T, trial_space, test_space, free_or_dirichlet=(FREE, FREE), precision=FP64, reuse=False,
-                    device=0):
-    """GT.assemble_matrix(a, Float64, U, V; free_or_dirichlet) on the GPU."""
+                    device=0, symbolic="host"):
+    """GT.assemble_matrix(a, Float64, U, V; free_or_dirichlet) on the GPU.
+    symbolic: where the pattern is built the first time ("host" or
+    "device")."""
     if T not in (float, np.float64):
         raise TypeError("values are Float64")
     if trial_space is not test_space:
@@ -173,7 +182,7 @@ def assemble_matrix(form, T, trial_space, test_space, free_or_dirichlet=(FREE, F
     space = test_space
     a = _assembler(space, form.dOmega, device)
     rows, cols = free_or_dirichlet
-    nnz, ncols, nrows, colptr, rowval = a.pattern(rows, cols)
+    nnz, ncols, nrows, colptr, rowval = a.pattern(rows, cols, symbolic)
     coef, ncoef = _coefficients(form, space.mesh.cells.shape[0])
     a.ctx.set_cell_coefficients(coef, ncoef)
     prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED

@@ -21,6 +21,7 @@
 #include "gtk_kernels.h"
 #include "gtk_plan.h"
 #include "gtk_solve.h"
+#include "gtk_symbolic.h"
 
 namespace {
 
@@ -65,6 +66,7 @@ struct MatrixPlan {
   gtk::Pattern pat;
   int max_seg = 0;
   DevBuf colptr, nzval;
+  DevBuf rowval;  // device copy (device symbolic phase, local map), on demand
   SolveWork sw;
   std::map<int, std::unique_ptr<ExecPlan>> exec;  // key: form * 2 + precision
   bool valid = false;
@@ -448,6 +450,101 @@ int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *n
   if (nnz) *nnz = plan->pat.nnz;
   if (ncols) *ncols = plan->pat.ncols;
   ctx->mplans[key_of(rows, cols)] = std::move(plan);
+  return GTK_OK;
+}
+
+int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *ncols, double *device_ms) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_space) FAIL(ctx, GTK_ESTATE, "gtk_set_space first");
+  if ((rows != GTK_FREE && rows != GTK_DIRICHLET) || (cols != GTK_FREE && cols != GTK_DIRICHLET))
+    FAIL(ctx, GTK_EINVAL, "rows/cols must be GTK_FREE or GTK_DIRICHLET");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto plan = std::make_unique<MatrixPlan>();
+  gtk::Pattern &pat = plan->pat;
+  pat.rows = rows;
+  pat.cols = cols;
+  pat.nrows = rows == GTK_FREE ? ctx->n_free : ctx->n_diri;
+  pat.ncols = cols == GTK_FREE ? ctx->n_free : ctx->n_diri;
+  const int64_t nc = pat.ncols, ncell = ctx->n_cells;
+  const int nd = ctx->dofs_per_cell;
+  hipStream_t st = ctx->stream;
+  const int32_t *cd = ctx->d_cell_dofs.as<int32_t>();
+  DevBuf counts, ptr, tmp, fill, inc, len, maxlist, c64;
+  int rc;
+  const size_t nc1 = (size_t)std::max<int64_t>(1, nc);
+  if ((rc = alloc(ctx, counts, nc1 * sizeof(int32_t))) || (rc = alloc(ctx, fill, nc1 * sizeof(int32_t))) ||
+      (rc = alloc(ctx, len, nc1 * sizeof(int32_t))) || (rc = alloc(ctx, ptr, (nc1 + 1) * sizeof(int64_t))) ||
+      (rc = alloc(ctx, c64, (nc1 + 1) * sizeof(int64_t))) || (rc = alloc(ctx, maxlist, sizeof(int32_t))) ||
+      (rc = alloc(ctx, tmp, gtk::symbolic_scan_tmp(nc) * sizeof(int64_t))))
+    return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
+  HIPCHK(ctx, hipMemsetAsync(counts.p, 0, nc1 * sizeof(int32_t), st));
+  HIPCHK(ctx, hipMemsetAsync(fill.p, 0, nc1 * sizeof(int32_t), st));
+  HIPCHK(ctx, hipMemsetAsync(maxlist.p, 0, sizeof(int32_t), st));
+  gtk::launch_column_count(cd, ncell, nd, cols, counts.as<int32_t>(), st);
+  gtk::launch_scan(counts.as<int32_t>(), nc, 0, ptr.as<int64_t>(), tmp.as<int64_t>(), st);
+  int64_t ninc = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&ninc, ptr.as<int64_t>() + nc, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  if ((rc = alloc(ctx, inc, (size_t)std::max<int64_t>(1, ninc) * sizeof(int32_t)))) return rc;
+  gtk::launch_column_fill(cd, ncell, nd, cols, ptr.as<int64_t>(), fill.as<int32_t>(), inc.as<int32_t>(), st);
+  gtk::launch_column_rows(cd, nd, rows, nc, ptr.as<int64_t>(), inc.as<int32_t>(), 0, len.as<int32_t>(),
+                          maxlist.as<int32_t>(), nullptr, nullptr, st);
+  gtk::launch_scan(len.as<int32_t>(), nc, 1, c64.as<int64_t>(), tmp.as<int64_t>(), st);
+  int32_t too_long = 0;
+  int64_t end = 1;
+  HIPCHK(ctx, hipMemcpyAsync(&too_long, maxlist.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(&end, c64.as<int64_t>() + nc, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  if (too_long)
+    FAIL(ctx, GTK_EUNSUP, "a column gathers " + std::to_string(too_long) + " candidate rows, more than the " +
+                              std::to_string(gtk::kSymbolicMaxList) + " the device symbolic phase sorts: use gtk_build_pattern");
+  if (end - 1 > INT32_MAX - 1) FAIL(ctx, GTK_EINVAL, "nnz exceeds the Int32 range of SparseMatrixCSC{Float64,Int32}");
+  pat.nnz = end - 1;
+  if ((rc = alloc(ctx, plan->rowval, (size_t)std::max<int64_t>(1, pat.nnz) * sizeof(int32_t))) ||
+      (rc = alloc(ctx, plan->colptr, (nc1 + 1) * sizeof(int32_t))))
+    return rc;
+  gtk::launch_column_rows(cd, nd, rows, nc, ptr.as<int64_t>(), inc.as<int32_t>(), 1, nullptr, maxlist.as<int32_t>(),
+                          c64.as<int64_t>(), plan->rowval.as<int32_t>(), st);
+  gtk::launch_colptr32(c64.as<int64_t>(), nc, plan->colptr.as<int32_t>(), st);
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
+  // host copy: the tile and batch planning and gtk_get_pattern read it
+  pat.colptr.resize(nc + 1);
+  pat.rowval.resize(pat.nnz);
+  HIPCHK(ctx, hipMemcpyAsync(pat.colptr.data(), plan->colptr.p, (nc + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (pat.nnz)
+    HIPCHK(ctx, hipMemcpyAsync(pat.rowval.data(), plan->rowval.p, pat.nnz * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if ((rc = alloc(ctx, plan->nzval, ((size_t)pat.nnz + kValueTail) * sizeof(double)))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  if (device_ms) {
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *device_ms = ms;
+  }
+  for (int64_t j = 0; j < nc; j++) plan->max_seg = std::max(plan->max_seg, pat.colptr[j + 1] - pat.colptr[j]);
+  plan->valid = true;
+  if (nnz) *nnz = pat.nnz;
+  if (ncols) *ncols = pat.ncols;
+  ctx->mplans[key_of(rows, cols)] = std::move(plan);
+  return GTK_OK;
+}
+
+int gtk_local_nnz_map(gtk_ctx *ctx, int rows, int cols, int32_t *map_out) {
+  if (!ctx || !map_out) return GTK_EINVAL;
+  auto it = ctx->mplans.find(key_of(rows, cols));
+  if (it == ctx->mplans.end()) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  MatrixPlan &P = *it->second;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if (!P.rowval.p && P.pat.nnz)
+    if ((rc = upload(ctx, P.rowval, P.pat.rowval.data(), P.pat.rowval.size()))) return rc;
+  const size_t n = (size_t)ctx->n_cells * ctx->dofs_per_cell * ctx->dofs_per_cell;
+  DevBuf map;
+  if ((rc = alloc(ctx, map, std::max<size_t>(1, n) * sizeof(int32_t)))) return rc;
+  gtk::launch_local_map(ctx->d_cell_dofs.as<int32_t>(), ctx->n_cells, ctx->dofs_per_cell, rows, cols,
+                        P.colptr.as<int32_t>(), P.rowval.as<int32_t>(), map.as<int32_t>(), ctx->stream);
+  if (n) HIPCHK(ctx, hipMemcpyAsync(map_out, map.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return GTK_OK;
 }
 

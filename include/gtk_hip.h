@@ -109,6 +109,22 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
  * Returns nnz and the column count. */
 int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *ncols);
 int gtk_get_pattern(gtk_ctx *ctx, int rows, int cols, int32_t *colptr, int32_t *rowval);
+/* The same symbolic phase on the GPU (column incidence by counting, each
+ * column's rows sorted and deduplicated by one wavefront, pointers by
+ * scans); colptr/rowval are bit-identical to gtk_build_pattern's and a host
+ * copy is kept for the work plan and gtk_get_pattern.  device_ms (may be
+ * NULL): device time of the pattern kernels.  GTK_EUNSUP if a column
+ * gathers more than 2048 candidate rows.  Replaces, on the device, the
+ * symbolic pass of allocate_matrix (src/assembly.jl:75-157) and compress
+ * (:571-575). */
+int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *ncols, double *device_ms);
+/* Cell-local-entry -> nonzero map of a built block, computed on the GPU:
+ * map_out[(cell*dofs_per_cell + b)*dofs_per_cell + a] = 1-based index in
+ * the block's values of entry (row of local DOF a, column of local DOF b),
+ * 0 when that row or column is outside the block; n_cells*dofs_per_cell^2
+ * values.  The cached map of the reference's reassembly path
+ * (src/assembly.jl:694-771). */
+int gtk_local_nnz_map(gtk_ctx *ctx, int rows, int cols, int32_t *map_out);
 
 /* Per-cell coefficients resident on the device: kappa (ncoef=1) or
  * (lambda, mu) (ncoef=2), n_cells*ncoef values, cell-major
