@@ -5,7 +5,7 @@ Host-side mirror of the reference's interface for the assembly path
 GT.assemble_matrix/assemble_vector, GT.update_matrix!) whose numeric work runs
 in libgtk_hip.so (HIP kernels for gfx950 behind the C ABI of include/gtk_hip.h).
 """
-from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, Laplace, Load, Mass, Measure,
+from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, FormSum, Laplace, Load, Mass, Measure,
                        SparseMatrixCSC, assemble_matrix, assemble_matrix_with_free_and_dirichlet_columns,
                        assemble_vector, face_constant_field, linear_problem, measure, solve, update_matrix)
 from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
@@ -14,7 +14,8 @@ from .batch import MatrixBatch, assemble_matrices, autotune
 from . import inputs, refcell
 
 __all__ = [
-    "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "Laplace", "Load",
+    "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "FormSum", "Laplace",
+    "Load",
     "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
     "assemble_vector", "face_constant_field", "linear_problem", "measure", "solve", "update_matrix", "Mesh",
     "cartesian_mesh", "perturb_interior_nodes", "MatrixBatch", "assemble_matrices", "autotune", "Domain",

@@ -19,7 +19,8 @@ GTK_F_CONST, GTK_F_CELL, GTK_F_NODAL = 0, 1, 2
 # every symbol the header declares
 EXPORTS = [
     "gtk_create", "gtk_destroy", "gtk_last_error", "gtk_version", "gtk_set_mesh", "gtk_update_coordinates",
-    "gtk_set_space", "gtk_set_reference", "gtk_build_pattern", "gtk_build_pattern_device", "gtk_local_nnz_map",
+    "gtk_set_space", "gtk_set_reference", "gtk_set_reference_faces", "gtk_build_pattern", "gtk_build_pattern_device",
+    "gtk_local_nnz_map", "gtk_add_block",
     "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
@@ -71,6 +72,8 @@ def load():
     L.gtk_get_pattern.argtypes = [vp, i, i, i32, i32]
     L.gtk_build_pattern_device.argtypes = [vp, i, i, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_double)]
     L.gtk_local_nnz_map.argtypes = [vp, i, i, i32]
+    L.gtk_set_reference_faces.argtypes = [vp, i, i, f64, i, f64, f64, f64, i32]
+    L.gtk_add_block.argtypes = [vp, i, i, vp]
     L.gtk_set_cell_coefficients.argtypes = [vp, i, C.c_void_p]
     L.gtk_assemble_matrix.argtypes = [vp, i, i, i, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_get_values.argtypes = [vp, i, i, f64]
@@ -139,6 +142,17 @@ class Context:
         self.check(self.L.gtk_set_reference(self.h, len(w), w, phi.shape[1], phi,
                                             np.ascontiguousarray(grad_phi, np.float64),
                                             np.ascontiguousarray(grad_geo, np.float64)), "gtk_set_reference")
+
+    def set_reference_faces(self, w, phi, grad_geo, face_jac, item_face):
+        phi = np.ascontiguousarray(phi, np.float64)  # (nfaces, nq, nshape)
+        self.check(self.L.gtk_set_reference_faces(self.h, phi.shape[0], phi.shape[1],
+                                                  np.ascontiguousarray(w, np.float64), phi.shape[2], phi,
+                                                  np.ascontiguousarray(grad_geo, np.float64),
+                                                  np.ascontiguousarray(face_jac, np.float64),
+                                                  np.ascontiguousarray(item_face, np.int32)), "gtk_set_reference_faces")
+
+    def add_block(self, rows, cols, src):
+        self.check(self.L.gtk_add_block(self.h, rows, cols, src.h), "gtk_add_block")
 
     def build_pattern(self, rows, cols):
         nnz, ncols = C.c_int64(), C.c_int64()

@@ -22,12 +22,25 @@ FP64, MIXED = "fp64", "mixed"
 
 
 class Measure:
-    """GT.measure(domain, degree): the reference rule of every cell."""
+    """GT.measure(domain, degree): the reference rule of every cell, or of
+    every face for a boundary domain (dimension D-1; the rule of the face's
+    reference cell, src/quadrature.jl:36-157)."""
 
     def __init__(self, domain, degree):
         self.domain = domain
         self.degree = int(degree)
-        self.x, self.w = R.quadrature(domain.mesh.cellkind, self.degree)
+        mesh = domain.mesh
+        if domain.dim == mesh.D:
+            self.kind = mesh.cellkind
+        elif domain.dim == mesh.D - 1:
+            self.kind = R.GEO[mesh.cellkind].ftype[mesh.D - 1]
+        else:
+            raise NotImplementedError("measures on cells or on (D-1)-faces")
+        self.x, self.w = R.quadrature(self.kind, self.degree)
+
+    @property
+    def on_boundary(self):
+        return self.domain.dim == self.domain.mesh.D - 1
 
 
 def measure(domain, degree):
@@ -56,22 +69,38 @@ class DiscreteField:
         self.values = np.ascontiguousarray(np.concatenate([fv, dv]))
 
 
+class FormSum:
+    """Sum of integrals over several measures (the reference's
+    DomainContribution sum): assembled into one matrix or vector."""
+
+    def __init__(self, terms):
+        self.terms = list(terms)
+
+    def __add__(self, other):
+        return FormSum(self.terms + (other.terms if isinstance(other, FormSum) else [other]))
+
+
+class _Form:
+    def __add__(self, other):
+        return FormSum([self] + (other.terms if isinstance(other, FormSum) else [other]))
+
+
 @dataclass
-class Laplace:
+class Laplace(_Form):
     """int kappa grad(u) . grad(v) dOmega."""
     dOmega: Measure
     kappa: object = None
 
 
 @dataclass
-class Mass:
+class Mass(_Form):
     """int kappa u v dOmega."""
     dOmega: Measure
     kappa: object = None
 
 
 @dataclass
-class Elasticity:
+class Elasticity(_Form):
     """int sigma(eps(u)) : eps(v) dOmega, sigma = lambda tr(eps) I + 2 mu eps."""
     dOmega: Measure
     lam: object
@@ -79,7 +108,7 @@ class Elasticity:
 
 
 @dataclass
-class Load:
+class Load(_Form):
     """int f . v dOmega; f constant (scalar/tuple), per cell, or discrete."""
     dOmega: Measure
     f: object = 1.0
@@ -137,12 +166,59 @@ class _Assembler:
         return self.patterns[key]
 
 
+class _BoundaryAssembler(_Assembler):
+    """The device context of one (space, boundary measure) pair: one item per
+    face of the domain (the face's cell and its local face), the item's DOFs
+    on that face's closure nodes, and the cell's shape functions and
+    geometry gradients at the face rule's points mapped into the cell's
+    reference coordinates, per local face."""
+
+    def __init__(self, space, dGamma, device=0):
+        mesh = space.mesh
+        D = mesh.D
+        self.space = space
+        self.dOmega = dGamma
+        faces = dGamma.domain.faces
+        cf = mesh.cell_faces[D - 1]  # (ncells, nlf) face ids
+        nlf = cf.shape[1]
+        owner = np.full(mesh.num_faces(D - 1), -1, np.int64)
+        owner[cf.ravel()] = np.arange(cf.size)  # boundary faces have one cell
+        hit = owner[faces]
+        if np.any(hit < 0):
+            raise ValueError("a face of the boundary domain has no cell")
+        cells, lface = hit // nlf, hit % nlf
+        fe = space.fe
+        fnodes = fe.face_nodes[D - 1]  # (nlf, nf) 1-based cell nodes
+        nc = space.ncomp
+        cols = ((fnodes[lface] - 1)[:, :, None] * nc + np.arange(nc)[None, None, :]).reshape(len(faces), -1)
+        item_dofs = np.take_along_axis(space.cell_dofs[cells], cols, axis=1)
+        g = R.GEO[mesh.cellkind]
+        ft = g.ftype[D - 1]
+        N = R.linear_shape(ft, dGamma.x)  # face rule -> reference cell points
+        _, dNf = R.Lagrange(ft, 1).tabulate(dGamma.x[:1])
+        geo = R.Lagrange(mesh.cellkind, 1)
+        phi, gN, A = [], [], []
+        for l, fv in enumerate(g.faces[D - 1]):
+            X = np.asarray([g.verts[v - 1] for v in fv], np.float64)
+            xc = N @ X
+            val, _ = fe.tabulate(xc)
+            phi.append(val[:, fnodes[l] - 1])
+            gN.append(geo.tabulate(xc)[1])
+            A.append(X.T @ dNf[0])  # (D, D-1)
+        self.ctx = L.Context(device)
+        self.ctx.set_mesh(D, mesh.coords, mesh.cell_nodes_1b()[cells])
+        self.ctx.set_space(nc, item_dofs, space.nfree, space.ndiri)
+        self.ctx.set_reference_faces(dGamma.w, np.stack(phi), np.stack(gN), np.stack(A), lface)
+        self.items = len(faces)
+        self.patterns = {}
+
+
 def _assembler(space, dOmega, device):
     key = (id(dOmega), device)
     cache = space.__dict__.setdefault("_assemblers", {})
     a = cache.get(key)
     if a is None or a.dOmega is not dOmega:
-        a = _Assembler(space, dOmega, device)
+        a = (_BoundaryAssembler if dOmega.on_boundary else _Assembler)(space, dOmega, device)
         cache[key] = a
     return a
 
@@ -155,6 +231,12 @@ def _form_code(form):
     if isinstance(form, Elasticity):
         return L.GTK_ELASTICITY
     raise TypeError("unsupported bilinear form")
+
+
+def _ncoef_items(form, space):
+    """Cells (or boundary items) the form's coefficients run over."""
+    dom = form.dOmega.domain
+    return len(dom.faces) if form.dOmega.on_boundary else space.mesh.cells.shape[0]
 
 
 def _coefficients(form, ncells):
@@ -180,10 +262,16 @@ def assemble_matrix(form, T, trial_space, test_space, free_or_dirichlet=(FREE, F
     if trial_space is not test_space:
         raise NotImplementedError("trial and test spaces must be the same space")
     space = test_space
+    if isinstance(form, FormSum):
+        if reuse:
+            raise NotImplementedError("reuse of a sum of integrals")
+        return _assemble_matrix_sum(form, T, space, free_or_dirichlet, precision, device, symbolic)
+    if form.dOmega.on_boundary and not isinstance(form, Mass):
+        raise NotImplementedError("boundary integrals of matrices: the mass form (Robin terms)")
     a = _assembler(space, form.dOmega, device)
     rows, cols = free_or_dirichlet
     nnz, ncols, nrows, colptr, rowval = a.pattern(rows, cols, symbolic)
-    coef, ncoef = _coefficients(form, space.mesh.cells.shape[0])
+    coef, ncoef = _coefficients(form, _ncoef_items(form, space))
     a.ctx.set_cell_coefficients(coef, ncoef)
     prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
     nzval, stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, nnz)
@@ -200,7 +288,7 @@ def update_matrix(A, cache, form=None):
     (new coefficients or coordinates); the pattern is not rebuilt."""
     a, form0, rows, cols, prec = cache
     form = form0 if form is None else form
-    coef, ncoef = _coefficients(form, a.space.mesh.cells.shape[0])
+    coef, ncoef = _coefficients(form, _ncoef_items(form, a.space))
     a.ctx.set_cell_coefficients(coef, ncoef)
     A.nzval, A.stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, A.nnz)
     return A
@@ -218,8 +306,30 @@ def _load_data(form, space):
     return L.GTK_F_CONST, np.broadcast_to(np.asarray(f, np.float64), (space.ncomp,)).copy()
 
 
+def _assemble_matrix_sum(fs, T, space, free_or_dirichlet, precision, device, symbolic):
+    """One volume integral plus boundary mass integrals: the volume block is
+    assembled, then each boundary block is added into its values on the
+    device (the boundary pattern lies inside the volume pattern)."""
+    vol = [t for t in fs.terms if not t.dOmega.on_boundary]
+    bnd = [t for t in fs.terms if t.dOmega.on_boundary]
+    if len(vol) != 1:
+        raise NotImplementedError("a sum of matrix integrals: one over the cells plus boundary integrals")
+    rows, cols = free_or_dirichlet
+    A = assemble_matrix(vol[0], T, space, space, free_or_dirichlet, precision=precision, device=device,
+                        symbolic=symbolic)
+    a_vol = A.device[0]
+    for t in bnd:
+        Ab = assemble_matrix(t, T, space, space, free_or_dirichlet, precision=precision, device=device)
+        a_vol.ctx.add_block(rows, cols, Ab.device[0].ctx)
+    A.nzval = a_vol.ctx.get_values(rows, cols, A.nnz)
+    return A
+
+
 def assemble_vector(form, T, space, free_or_dirichlet=FREE, device=0):
-    """GT.assemble_vector(l, Float64, V; free_or_dirichlet) on the GPU."""
+    """GT.assemble_vector(l, Float64, V; free_or_dirichlet) on the GPU; l may
+    be a sum of loads over cells and boundary faces."""
+    if isinstance(form, FormSum):
+        return sum(assemble_vector(t, T, space, free_or_dirichlet, device) for t in form.terms)
     kind, data = _load_data(form, space)
     a = _assembler(space, form.dOmega, device)
     n = space.nfree if free_or_dirichlet == FREE else space.ndiri

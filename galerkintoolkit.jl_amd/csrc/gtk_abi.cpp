@@ -103,8 +103,13 @@ struct gtk_ctx {
   int nq = 0, nshape = 0;
   std::vector<double> w, phi, gphi_t, gN;  // gphi_t: [q][d][s]
   int ncoef = 0;
+  // face mode: the "cells" are boundary items (cell, local face); tables per
+  // local face, d(reference cell point)/d(reference face point) per face
+  int nfaces = 0;
+  std::vector<double> face_jac;
+  std::vector<int32_t> item_face;
   // device copies
-  DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef;
+  DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef, d_face_jac, d_item_face;
   // wavefronts per workgroup of the matrix-core kernels, per precision
   // (FP64: 8, two per SIMD within 256 registers; mixed: 12, three per SIMD);
   // 0 = the default, GTK_WAVES overrides both
@@ -181,7 +186,7 @@ static gtk::MeshView view(const gtk_ctx *ctx) {
 }
 
 static gtk::ElemKey elem_key(const gtk_ctx *ctx) {
-  return gtk::ElemKey{ctx->dim, ctx->nv, ctx->nshape, ctx->nq, ctx->ncomp};
+  return gtk::ElemKey{ctx->dim, ctx->nv, ctx->nshape, ctx->nq, ctx->ncomp, ctx->nfaces};
 }
 
 // LDS available to one workgroup (gfx950: 160 KiB per CU), less headroom.
@@ -219,8 +224,10 @@ static int tile_cells_for(const gtk::KernelInfo &k, size_t acc_bytes, size_t bud
 // coordinates and the nodal values; tables phi and the geometry gradients.
 static void vector_tile_limits(const gtk_ctx *ctx, int &max_cells, int &chunk) {
   const size_t fs = (size_t)ctx->nq * ctx->ncomp | 1;  // odd per-cell stride of f . dV
-  const size_t geo = (fs + (size_t)ctx->nv * ctx->dim + (size_t)ctx->nshape * ctx->ncomp) * 8;
-  const size_t tables = (size_t)ctx->nq * (((size_t)ctx->nshape | 1) + ((size_t)ctx->nv * ctx->dim | 1)) * 8;
+  const size_t geo =
+      (fs + (size_t)ctx->nv * ctx->dim + (size_t)ctx->nshape * ctx->ncomp + (ctx->nfaces ? 1 : 0)) * 8;
+  const size_t tables = (size_t)std::max(1, ctx->nfaces) * ctx->nq *
+                        (((size_t)ctx->nshape | 1) + ((size_t)ctx->nv * ctx->dim | 1)) * 8;
   const size_t avail = kLdsBudget > tables + 1024 ? kLdsBudget - tables - 1024 : 8 * geo;
   max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, avail / geo));
   chunk = tile_chunk(ctx, max_cells);
@@ -386,12 +393,47 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
     for (int s = 0; s < nshape; s++)
       for (int d = 0; d < D; d++) ctx->gphi_t[((size_t)q * D + d) * nshape + s] = grad_phi[((size_t)q * nshape + s) * D + d];
   ctx->gN.assign(grad_geo, grad_geo + (size_t)nq * ctx->nv * D);
+  ctx->nfaces = 0;
   for (auto &mp : ctx->mplans) mp.second->exec.clear();  // kernels depend on the tables
+  ctx->vplans.clear();
   int rc;
   if ((rc = upload(ctx, ctx->d_w, ctx->w.data(), ctx->w.size()))) return rc;
   if ((rc = upload(ctx, ctx->d_phi, ctx->phi.data(), ctx->phi.size()))) return rc;
   if ((rc = upload(ctx, ctx->d_gphi, ctx->gphi_t.data(), ctx->gphi_t.size()))) return rc;
   if ((rc = upload(ctx, ctx->d_gN, ctx->gN.data(), ctx->gN.size()))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->has_ref = true;
+  return GTK_OK;
+}
+
+int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, int nshape, const double *phi,
+                            const double *grad_geo, const double *face_jac, const int32_t *item_face) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_mesh) FAIL(ctx, GTK_ESTATE, "gtk_set_mesh first");
+  if (nfaces < 1 || nfaces > 8 || nq < 1 || nq > 128 || nshape < 1 || nshape > 64 || !w || !phi || !grad_geo ||
+      !face_jac || !item_face)
+    FAIL(ctx, GTK_EINVAL, "invalid face tables");
+  const int D = ctx->dim;
+  for (int64_t i = 0; i < ctx->n_cells; i++)
+    if (item_face[i] < 0 || item_face[i] >= nfaces) FAIL(ctx, GTK_EINVAL, "item_face out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ctx->nfaces = nfaces;
+  ctx->nq = nq;
+  ctx->nshape = nshape;
+  ctx->w.assign(w, w + nq);
+  ctx->phi.assign(phi, phi + (size_t)nfaces * nq * nshape);
+  ctx->gphi_t.clear();
+  ctx->gN.assign(grad_geo, grad_geo + (size_t)nfaces * nq * ctx->nv * D);
+  ctx->face_jac.assign(face_jac, face_jac + (size_t)nfaces * D * (D - 1));
+  ctx->item_face.assign(item_face, item_face + ctx->n_cells);
+  for (auto &mp : ctx->mplans) mp.second->exec.clear();
+  ctx->vplans.clear();
+  int rc;
+  if ((rc = upload(ctx, ctx->d_w, ctx->w.data(), ctx->w.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_phi, ctx->phi.data(), ctx->phi.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_gN, ctx->gN.data(), ctx->gN.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_face_jac, ctx->face_jac.data(), ctx->face_jac.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_item_face, ctx->item_face.data(), ctx->item_face.size()))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_ref = true;
   return GTK_OK;
@@ -526,6 +568,34 @@ int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int
   if (nnz) *nnz = pat.nnz;
   if (ncols) *ncols = pat.ncols;
   ctx->mplans[key_of(rows, cols)] = std::move(plan);
+  return GTK_OK;
+}
+
+int gtk_add_block(gtk_ctx *dst, int rows, int cols, gtk_ctx *src) {
+  if (!dst || !src) return GTK_EINVAL;
+  if (dst->device != src->device) FAIL(dst, GTK_EINVAL, "the contexts are on different devices");
+  auto id = dst->mplans.find(key_of(rows, cols));
+  auto is = src->mplans.find(key_of(rows, cols));
+  if (id == dst->mplans.end() || is == src->mplans.end()) FAIL(dst, GTK_ESTATE, "both blocks must be built");
+  MatrixPlan &D = *id->second, &S = *is->second;
+  if (D.pat.ncols != S.pat.ncols || D.pat.nrows != S.pat.nrows) FAIL(dst, GTK_EINVAL, "the blocks differ in shape");
+  HIPCHK(dst, hipSetDevice(dst->device));
+  int rc;
+  if (!D.rowval.p && D.pat.nnz)
+    if ((rc = upload(dst, D.rowval, D.pat.rowval.data(), D.pat.rowval.size()))) return rc;
+  if (!S.rowval.p && S.pat.nnz)
+    if ((rc = upload(dst, S.rowval, S.pat.rowval.data(), S.pat.rowval.size()))) return rc;
+  HIPCHK(dst, hipStreamSynchronize(src->stream));  // the source values are complete
+  DevBuf missing;
+  if ((rc = alloc(dst, missing, sizeof(int32_t)))) return rc;
+  HIPCHK(dst, hipMemsetAsync(missing.p, 0, sizeof(int32_t), dst->stream));
+  gtk::launch_add_block(S.pat.ncols, S.colptr.as<int32_t>(), S.rowval.as<int32_t>(), S.nzval.as<double>(),
+                        D.colptr.as<int32_t>(), D.rowval.as<int32_t>(), D.nzval.as<double>(), missing.as<int32_t>(),
+                        dst->stream);
+  int32_t miss = 0;
+  HIPCHK(dst, hipMemcpyAsync(&miss, missing.p, sizeof(int32_t), hipMemcpyDeviceToHost, dst->stream));
+  HIPCHK(dst, hipStreamSynchronize(dst->stream));
+  if (miss) FAIL(dst, GTK_EINVAL, "the source block has entries outside the destination pattern");
   return GTK_OK;
 }
 
@@ -725,6 +795,10 @@ static gtk::VectorArgs vector_args(const gtk_ctx *ctx, const VectorPlan &vp, con
   a.ctr_slot = vp.ctr_slot.as<uint16_t>();
   a.ctr_b = vp.ctr_b.as<uint16_t>();
   a.out = vp.out.as<double>();
+  if (ctx->nfaces) {
+    a.item_face = ctx->d_item_face.as<int32_t>();
+    a.face_jac = ctx->d_face_jac.as<double>();
+  }
   return a;
 }
 
@@ -767,6 +841,10 @@ static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, c
   a.stage_cells = ep->stage_cells;
   a.lds_claim = ep->lds_claim;
   a.ntiles = ep->tiles.ntiles;
+  if (ctx->nfaces) {
+    a.item_face = ctx->d_item_face.as<int32_t>();
+    a.face_jac = ctx->d_face_jac.as<double>();
+  }
   return a;
 }
 
@@ -914,6 +992,8 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
   gtk::VectorArgs a = vector_args(ctx, vp, vp.f.as<double>());
   void *args[] = {&a};
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (vp.tiles.uncovered && vp.nrows)  // rows no cell touches stay zero
+    HIPCHK(ctx, hipMemsetAsync(vp.out.p, 0, vp.nrows * sizeof(double), ctx->stream));
   if (vp.tiles.ntiles)
     HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(vp.tiles.ntiles), dim3(gtk::kBlock), args, lds, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1258,6 +1338,7 @@ int gtk_batch_build(gtk_batch *b) {
     if (!gtk::find_vector_kernel(elem_key(ctx), l.fkind, vk)) BFAIL(b, GTK_EUNSUP, "no vector kernel for a problem");
     VectorPlan *vp = nullptr;
     if (get_vector_plan(ctx, GTK_FREE, vp)) BFAIL(b, GTK_EINVAL, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
+    if (vp->tiles.uncovered) BFAIL(b, GTK_EUNSUP, "a problem's load vector has rows no cell touches");
     b->lds = std::max(b->lds, vk.table_bytes + (size_t)vp->tiles.max_tile_cells * vk.geo_bytes_per_cell);
     vargs[p] = vector_args(ctx, *vp, l.f.as<double>());
     fkinds[p] = l.fkind;

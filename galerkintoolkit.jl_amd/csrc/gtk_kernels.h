@@ -34,6 +34,8 @@ struct MatrixArgs {
   int stage_cells;   // cells whose coordinates fit the accumulator space (matrix-core kernels)
   int lds_claim;     // byte offset in LDS of the batch counter (matrix-core kernels)
   int ntiles;        // tiles of the plan (matrix-core kernels loop over them)
+  const int32_t *item_face;  // face mode (boundary items), as in VectorArgs
+  const double *face_jac;
 };
 
 // Many problems in one launch (matrix-core kernels of 2D quadrilaterals):
@@ -50,6 +52,10 @@ struct VectorArgs {
   const int32_t *tile_cell_ptr, *tile_cells, *tile_col_ptr, *col_id, *col_ctr_ptr;
   const uint16_t *ctr_slot, *ctr_b;
   double *out;
+  // boundary items (face mode): local face of each item, d(reference cell
+  // point)/d(reference face point) of each local face, [face][D][D-1]
+  const int32_t *item_face;
+  const double *face_jac;
 };
 
 // A batch of independent problems in one launch: work items (problem, t)
@@ -63,8 +69,9 @@ struct BatchedArgs {
   int nitems;
 };
 
+// nl: local faces of the cell for boundary items (face mode), 0 for cells
 struct ElemKey {
-  int dim, nv, ns, nq, nc;
+  int dim, nv, ns, nq, nc, nl;
 };
 
 struct KernelInfo {

@@ -76,6 +76,46 @@ __device__ inline double cell_jacobian(const double *__restrict__ xyz, const int
   return fabs(inverse<D>(J, Ji));
 }
 
+// J at quadrature point q of `cell` (geometry gradients gN[q][NV][D]).
+template <int D, int NV>
+__device__ inline void cell_jacobian_only(const double *__restrict__ xyz, const int32_t *__restrict__ cell_nodes,
+                                          const double *__restrict__ gN, int64_t cell, int q, double J[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) J[i][j] = 0.0;
+#pragma unroll
+  for (int n = 0; n < NV; n++) {
+    const int64_t node = cell_nodes[cell * NV + n];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) J[i][j] = fma(xyz[node * D + i], gN[(q * NV + n) * D + j], J[i][j]);
+  }
+}
+
+// Measure factor of a boundary face at a point: sqrt(det(Jf^T Jf)) with
+// Jf = J A, A = d(reference cell point)/d(reference face point), D x (D-1)
+// row-major (the weight sqrt(det(J'J)) of the reference,
+// src/accessors.jl:1000-1007, for the face's map).
+template <int D>
+__device__ inline double surface_measure(const double J[D][D], const double *__restrict__ A) {
+  if constexpr (D == 2) {
+    const double t0 = J[0][0] * A[0] + J[0][1] * A[1];
+    const double t1 = J[1][0] * A[0] + J[1][1] * A[1];
+    return sqrt(t0 * t0 + t1 * t1);
+  } else {
+    double u[3], v[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      u[i] = J[i][0] * A[0] + J[i][1] * A[2] + J[i][2] * A[4];
+      v[i] = J[i][0] * A[1] + J[i][1] * A[3] + J[i][2] * A[5];
+    }
+    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2], c2 = u[0] * v[1] - u[1] * v[0];
+    return sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  }
+}
+
 // number of staged geometry factors per (cell, point)
 template <int D, int FORM>
 struct NGeo {
@@ -94,16 +134,21 @@ __device__ inline int sym_index(int i, int j) {
   else return i == 0 ? j : (i == 1 ? 2 + j : 5);
 }
 
-template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT>
+// NL > 0: boundary items (face mode, mass only): the tables are per local
+// face, [NL][NQ][...], and the measure is the face's.
+template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT, int NL = 0>
 __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
+  static_assert(NL == 0 || FORM == kMass, "boundary items: mass form only");
   constexpr int NG = NGeo<D, FORM>::value;
   constexpr int NDOF = NS * NC;
   constexpr int CPW = 64 / NDOF;
   constexpr int NWAVES = kBlock / 64;
+  constexpr int NT = NL > 0 ? NL : 1;            // table sets
+  constexpr int NGPHI = NL > 0 ? 0 : NQ * D * NS;  // face mode: no gradient table
   extern __shared__ __align__(16) unsigned char lds_raw[];
   CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NQ][D][NS]
-  CT *phi_s = gphi_s + NQ * D * NS;                // [NQ][NS]
-  CT *G_s = phi_s + NQ * NS;                       // [cells][NQ][NG]
+  CT *phi_s = gphi_s + NGPHI;                      // [NT][NQ][NS]
+  CT *G_s = phi_s + NT * NQ * NS;                  // [cells][NQ][NG]
   double *acc_base = reinterpret_cast<double *>(lds_raw + args.acc_offset);
 
   const int tile = blockIdx.x;
@@ -111,16 +156,27 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
   const int col0 = args.tile_col_ptr[tile];
   const int ncol = args.tile_col_ptr[tile + 1] - col0;
+  int32_t *lf_s = reinterpret_cast<int32_t *>(G_s + ncell * NQ * NG);  // [cells] local face (NL > 0)
 
-  for (int i = threadIdx.x; i < NQ * D * NS; i += kBlock) gphi_s[i] = (CT)args.gphi[i];
-  for (int i = threadIdx.x; i < NQ * NS; i += kBlock) phi_s[i] = (CT)args.phi[i];
+  for (int i = threadIdx.x; i < NGPHI; i += kBlock) gphi_s[i] = (CT)args.gphi[i];
+  for (int i = threadIdx.x; i < NT * NQ * NS; i += kBlock) phi_s[i] = (CT)args.phi[i];
+  if constexpr (NL > 0)
+    for (int i = threadIdx.x; i < ncell; i += kBlock) lf_s[i] = args.item_face[args.tile_cells[cell0 + i]];
 
   // phase A: FP64 geometry per (cell, point)
   for (int idx = threadIdx.x; idx < ncell * NQ; idx += kBlock) {
     const int s = idx / NQ, q = idx - s * NQ;
     const int64_t cell = args.tile_cells[cell0 + s];
     double Ji[D][D];
-    const double dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji) * args.w[q];
+    double dV;
+    if constexpr (NL > 0) {
+      const int l = args.item_face[cell];
+      double J[D][D];
+      cell_jacobian_only<D, NV>(args.xyz, args.cell_nodes, args.gN + (size_t)l * NQ * NV * D, cell, q, J);
+      dV = surface_measure<D>(J, args.face_jac + l * D * (D - 1)) * args.w[q];
+    } else {
+      dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji) * args.w[q];
+    }
     CT *G = G_s + (s * NQ + q) * NG;
     if constexpr (FORM == kLaplace) {
       const double k = args.coef ? args.coef[cell] : 1.0;
@@ -189,8 +245,9 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
           }
         }
       } else if constexpr (FORM == kMass) {
+        const CT *ph = phi_s + (NL > 0 ? lf_s[s] * NQ * NS : 0);
 #pragma unroll
-        for (int q = 0; q < NQ; q++) v = fma(phi_s[q * NS + an], Gs[q * NG] * phi_s[q * NS + bn], v);
+        for (int q = 0; q < NQ; q++) v = fma(ph[q * NS + an], Gs[q * NG] * ph[q * NS + bn], v);
       } else {
         // sigma(eps(phi_b e_bc)) : eps(phi_a e_ac)
         //   = lambda (g_b)_bc (g_a)_ac + mu (delta_{ac,bc} g_a.g_b + (g_a)_bc (g_b)_ac)
@@ -787,9 +844,10 @@ __device__ inline double determinant(const double J[D][D]) {
 // and each owned row sums its contributions in cell order.  LDS: the tables (phi, order-1 geometry
 // gradients) and, per cell, (NQ*NC + NV*D + NS*NC) doubles.  BS threads; the
 // caller synchronises before LDS is reused.
-template <int D, int NV, int NS, int NQ, int NC, int FKIND, int BS>
+template <int D, int NV, int NS, int NQ, int NC, int FKIND, int BS, int NL = 0>
 __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned char *lds_raw) {
   constexpr int NDOF = NS * NC;
+  constexpr int NT = NL > 0 ? NL : 1;  // table sets: one per local face for boundary items
   // odd strides: lanes on different points (or cells) fall on different LDS banks
   constexpr int PS = VecCellStride<NS>::value;       // phi row
   constexpr int GR = VecCellStride<NV * D>::value;   // geometry-gradient row
@@ -799,15 +857,20 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
   const int col0 = args.tile_col_ptr[tile];
   const int ncol = args.tile_col_ptr[tile + 1] - col0;
-  double *phi_s = reinterpret_cast<double *>(lds_raw);  // [NQ][PS]
-  double *gN_s = phi_s + NQ * PS;                       // [NQ][GR]: [NV][D] used
-  double *F_s = gN_s + NQ * GR;                         // [cells][FS]: f . dV at the points
+  double *phi_s = reinterpret_cast<double *>(lds_raw);  // [NT][NQ][PS]
+  double *gN_s = phi_s + NT * NQ * PS;                  // [NT][NQ][GR]: [NV][D] used
+  double *F_s = gN_s + NT * NQ * GR;                    // [cells][FS]: f . dV at the points
   double *X_s = F_s + ncell * FS;                       // [cells][NV][D]
   double *fc_s = X_s + ncell * NV * D;                  // [cells][NDOF] (discrete field)
+  int32_t *lf_s = reinterpret_cast<int32_t *>(fc_s + ncell * NDOF);  // [cells] local face (NL > 0)
   int32_t *cid_s = reinterpret_cast<int32_t *>(F_s);    // cell ids while staging
-  for (int i = threadIdx.x; i < NQ * NS; i += BS) phi_s[(i / NS) * PS + i % NS] = args.phi[i];
-  for (int i = threadIdx.x; i < NQ * NV * D; i += BS) gN_s[(i / (NV * D)) * GR + i % (NV * D)] = args.gN[i];
-  for (int i = threadIdx.x; i < ncell; i += BS) cid_s[i] = args.tile_cells[cell0 + i];
+  for (int i = threadIdx.x; i < NT * NQ * NS; i += BS) phi_s[(i / NS) * PS + i % NS] = args.phi[i];
+  for (int i = threadIdx.x; i < NT * NQ * NV * D; i += BS) gN_s[(i / (NV * D)) * GR + i % (NV * D)] = args.gN[i];
+  for (int i = threadIdx.x; i < ncell; i += BS) {
+    const int32_t c = args.tile_cells[cell0 + i];
+    cid_s[i] = c;
+    if constexpr (NL > 0) lf_s[i] = args.item_face[c];
+  }
   __syncthreads();
   // node coordinates (node id -> xyz) and nodal values (DOF id -> f, free
   // then Dirichlet), each level issued for U items before the next
@@ -845,6 +908,8 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
   __syncthreads();
   for (int idx = threadIdx.x; idx < ncell * NQ; idx += BS) {
     const int s = idx / NQ, q = idx - s * NQ;
+    const int l = NL > 0 ? lf_s[s] : 0;
+    const double *gq = gN_s + (l * NQ + q) * GR;
     double J[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++)
@@ -855,8 +920,11 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
 #pragma unroll
       for (int i = 0; i < D; i++)
 #pragma unroll
-        for (int j = 0; j < D; j++) J[i][j] = fma(X_s[(s * NV + v) * D + i], gN_s[q * GR + v * D + j], J[i][j]);
-    const double dV = fabs(determinant<D>(J)) * args.w[q];
+        for (int j = 0; j < D; j++) J[i][j] = fma(X_s[(s * NV + v) * D + i], gq[v * D + j], J[i][j]);
+    double dV;
+    if constexpr (NL > 0) dV = surface_measure<D>(J, args.face_jac + l * D * (D - 1)) * args.w[q];
+    else dV = fabs(determinant<D>(J)) * args.w[q];
+    const double *pq = phi_s + (l * NQ + q) * PS;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       double f;
@@ -868,7 +936,7 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
         // discrete field: sum_a f[dof(cell, a)] phi_a(q)
         f = 0.0;
 #pragma unroll 4
-        for (int n = 0; n < NS; n++) f = fma(fc_s[s * NDOF + n * NC + c], phi_s[q * PS + n], f);
+        for (int n = 0; n < NS; n++) f = fma(fc_s[s * NDOF + n * NC + c], pq[n], f);
       }
       F_s[s * FS + q * NC + c] = f * dV;
     }
@@ -882,6 +950,7 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
   double *b_s = fc_s;  // [cells][NDOF]
   for (int idx = threadIdx.x; idx < ncell * NKB; idx += BS) {
     const int s = idx / NKB, k0 = (idx - s * NKB) * KB;
+    const double *ph = phi_s + (NL > 0 ? lf_s[s] * NQ * PS : 0);
     double acc[KB];
 #pragma unroll
     for (int u = 0; u < KB; u++) acc[u] = 0.0;
@@ -891,7 +960,7 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
       for (int u = 0; u < KB; u++) {
         const int k = k0 + u < NDOF ? k0 + u : NDOF - 1;
         const int n = k / NC, c = k - n * NC;
-        acc[u] = fma(phi_s[q * PS + n], F_s[s * FS + q * NC + c], acc[u]);
+        acc[u] = fma(ph[q * PS + n], F_s[s * FS + q * NC + c], acc[u]);
       }
     }
 #pragma unroll
@@ -923,10 +992,10 @@ __device__ inline void vector_tile(const VectorArgs &args, int tile, unsigned ch
   }
 }
 
-template <int D, int NV, int NS, int NQ, int NC, int FKIND>
+template <int D, int NV, int NS, int NQ, int NC, int FKIND, int NL>
 __global__ __launch_bounds__(kBlock) void vector_tile_kernel(VectorArgs args) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  vector_tile<D, NV, NS, NQ, NC, FKIND, kBlock>(args, blockIdx.x, lds_raw);
+  vector_tile<D, NV, NS, NQ, NC, FKIND, kBlock, NL>(args, blockIdx.x, lds_raw);
 }
 
 template <typename E, int BS>
@@ -967,9 +1036,11 @@ __global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba)
 // ---------------------------------------------------------------------------
 // dispatch tables
 // ---------------------------------------------------------------------------
-template <int D, int NV, int NS, int NQ, int NC>
+// NL > 0: boundary items of a cell with NL local faces (NS shape functions
+// and NQ points of one face)
+template <int D, int NV, int NS, int NQ, int NC, int NL = 0>
 struct Elem {
-  static constexpr int d = D, nv = NV, ns = NS, nq = NQ, nc = NC;
+  static constexpr int d = D, nv = NV, ns = NS, nq = NQ, nc = NC, nl = NL;
 };
 
 template <typename E, int FORM, typename CT, typename OffT, int NW>
@@ -1003,37 +1074,47 @@ static KernelInfo matrix_batch_info(int off_bytes, int waves) {
   return matrix_batch_info<E, FORM, CT, 8>(off_bytes);
 }
 
+// LDS sizes below follow the kernels' layouts (the host's tile limits use
+// the same formulas)
 template <typename E, int FORM, typename CT>
 static KernelInfo matrix_info() {
+  constexpr int NT = E::nl > 0 ? E::nl : 1;
   KernelInfo k;
-  k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT>);
-  k.table_bytes = (size_t)(E::nq * E::d * E::ns + E::nq * E::ns) * sizeof(CT);
-  k.geo_bytes_per_cell = (size_t)E::nq * NGeo<E::d, FORM>::value * sizeof(CT);
+  k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl>);
+  k.table_bytes = (size_t)((E::nl > 0 ? 0 : E::nq * E::d * E::ns) + NT * E::nq * E::ns) * sizeof(CT);
+  k.geo_bytes_per_cell = (size_t)E::nq * NGeo<E::d, FORM>::value * sizeof(CT) + (E::nl > 0 ? sizeof(int32_t) : 0);
   k.cols_per_wave = 64 / (E::ns * E::nc);
   return k;
 }
 
 template <typename E, int FK>
 static KernelInfo vector_info() {
+  constexpr int NT = E::nl > 0 ? E::nl : 1;
   KernelInfo k;
-  k.fn = reinterpret_cast<const void *>(&vector_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FK>);
-  // same sizes as vector_tile's LDS layout (and the host's tile limits)
-  k.table_bytes = (size_t)E::nq * (VecCellStride<E::ns>::value + VecCellStride<E::nv * E::d>::value) * sizeof(double);
-  k.geo_bytes_per_cell = (size_t)(VecCellStride<E::nq * E::nc>::value + E::nv * E::d + E::ns * E::nc) * sizeof(double);
+  k.fn = reinterpret_cast<const void *>(&vector_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FK, E::nl>);
+  k.table_bytes =
+      (size_t)NT * E::nq * (VecCellStride<E::ns>::value + VecCellStride<E::nv * E::d>::value) * sizeof(double);
+  k.geo_bytes_per_cell = (size_t)(VecCellStride<E::nq * E::nc>::value + E::nv * E::d + E::ns * E::nc +
+                                  (E::nl > 0 ? 1 : 0)) * sizeof(double);
   k.cols_per_wave = 64;
   return k;
 }
 
 template <typename E>
 static bool match(const ElemKey &key) {
-  return key.dim == E::d && key.nv == E::nv && key.ns == E::ns && key.nq == E::nq && key.nc == E::nc;
+  return key.dim == E::d && key.nv == E::nv && key.ns == E::ns && key.nq == E::nq && key.nc == E::nc &&
+         key.nl == E::nl;
 }
 
 template <typename E>
 static bool try_matrix(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                        KernelInfo &out) {
   if (!match<E>(key)) return false;
-  if constexpr (E::nc == 1) {
+  if constexpr (E::nl > 0) {
+    // boundary items: the mass form on the VALU kernel
+    if (form != kMass) return false;
+    out = precision == 0 ? matrix_info<E, kMass, double>() : matrix_info<E, kMass, float>();
+  } else if constexpr (E::nc == 1) {
     if (form != kLaplace && form != kMass) return false;
     if (allow_mfma) {
       if (form == kLaplace)
@@ -1072,6 +1153,14 @@ using QuadQ3 = Elem<2, 4, 16, 16, 1>;
 using HexQ1 = Elem<3, 8, 8, 8, 1>;
 using HexQ2 = Elem<3, 8, 27, 27, 1>;
 using TetP2v = Elem<3, 4, 10, 14, 3>;
+// boundary items: edges of Q1..Q3 quadrilaterals (k+1 Gauss points), faces
+// of Q1/Q2 hexahedra ((k+1)^2), faces of P2 vector tetrahedra (9-point rule)
+using QuadEdge1 = Elem<2, 4, 2, 2, 1, 4>;
+using QuadEdge2 = Elem<2, 4, 3, 3, 1, 4>;
+using QuadEdge3 = Elem<2, 4, 4, 4, 1, 4>;
+using HexFace1 = Elem<3, 8, 4, 4, 1, 6>;
+using HexFace2 = Elem<3, 8, 9, 9, 1, 6>;
+using TetFace2v = Elem<3, 4, 6, 9, 3, 4>;
 
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out) {
@@ -1080,7 +1169,13 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<QuadQ3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexQ1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexQ2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
-         try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, waves, out);
+         try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadEdge1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadEdge2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadEdge3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexFace1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexFace2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetFace2v>(key, form, precision, off_bytes, allow_mfma, waves, out);
 }
 
 int batched_type(const ElemKey &key) {
@@ -1116,7 +1211,10 @@ bool find_batched_kernel(int form, int precision, int waves, KernelInfo &out) {
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {
   return try_vector<QuadQ1>(key, fkind, out) || try_vector<QuadQ2>(key, fkind, out) ||
          try_vector<QuadQ3>(key, fkind, out) || try_vector<HexQ1>(key, fkind, out) ||
-         try_vector<HexQ2>(key, fkind, out) || try_vector<TetP2v>(key, fkind, out);
+         try_vector<HexQ2>(key, fkind, out) || try_vector<TetP2v>(key, fkind, out) ||
+         try_vector<QuadEdge1>(key, fkind, out) || try_vector<QuadEdge2>(key, fkind, out) ||
+         try_vector<QuadEdge3>(key, fkind, out) || try_vector<HexFace1>(key, fkind, out) ||
+         try_vector<HexFace2>(key, fkind, out) || try_vector<TetFace2v>(key, fkind, out);
 }
 
 }  // namespace gtk

@@ -295,7 +295,10 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
   out.col_ctr_ptr.push_back((int32_t)out.ctr_slot.size());
   out.tile_batch_ptr.push_back((int32_t)out.batch_col.size());
   out.batch_col.push_back((int32_t)out.col_id.size());
-  if ((int64_t)out.col_id.size() != nset) return "tile plan does not cover every DOF of the set";
+  // DOFs of S that no cell touches (boundary items cover only boundary DOFs)
+  // are in no tile: their matrix columns are empty, their vector rows zero
+  if ((int64_t)out.col_id.size() > nset) return "tile plan lists a DOF twice";
+  out.uncovered = nset - (int64_t)out.col_id.size();
   for (int tl = 0; tl < out.ntiles; tl++)
     out.max_tile_batches = std::max(out.max_tile_batches, out.tile_batch_ptr[tl + 1] - out.tile_batch_ptr[tl]);
   return "";

@@ -29,6 +29,7 @@ struct TilePlan {
   int set = 0;                  // GTK_FREE / GTK_DIRICHLET
   int ndofs_cell = 0;           // local dofs per cell
   int64_t nset = 0;             // number of dofs in S
+  int64_t uncovered = 0;        // dofs of S no cell touches (in no tile)
   int ntiles = 0;
   int max_tile_cells = 0, max_tile_cols = 0, max_ctr = 0;
   int max_tile_ctr = 0, max_tile_batches = 0;

@@ -39,4 +39,10 @@ void launch_colptr32(const int64_t *colptr64, int64_t n, int32_t *colptr, hipStr
 void launch_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, int rows, int cols, const int32_t *colptr,
                       const int32_t *rowval, int32_t *map, hipStream_t st);
 
+// dst values += src values over the src pattern (each src column's rows are
+// found in the same dst column by binary search); *missing = 1 if a src
+// entry has no dst entry.
+void launch_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,
+                      const int32_t *dcolptr, const int32_t *drowval, double *dval, int32_t *missing, hipStream_t st);
+
 }  // namespace gtk

@@ -192,6 +192,25 @@ __global__ void k_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, i
   }
 }
 
+// one thread per column: every dst entry gets at most one src entry
+__global__ void k_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,
+                            const int32_t *dcolptr, const int32_t *drowval, double *dval, int32_t *missing) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x) {
+    const int d0 = dcolptr[j] - 1, d1 = dcolptr[j + 1] - 1;
+    for (int k = scolptr[j] - 1; k < scolptr[j + 1] - 1; k++) {
+      const int32_t r = srowval[k];
+      int lo = d0, hi = d1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (drowval[mid] < r) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < d1 && drowval[lo] == r) dval[lo] += sval[k];
+      else *missing = 1;
+    }
+  }
+}
+
 unsigned grid_for(int64_t n, int block) {
   const int64_t g = (n + block - 1) / block;
   return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
@@ -243,6 +262,12 @@ void launch_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, int row
   const int64_t total = n_cells * nd * nd;
   if (total == 0) return;
   k_local_map<<<grid_for(total, 256), 256, 0, st>>>(cell_dofs, n_cells, nd, rows, cols, colptr, rowval, map);
+}
+
+void launch_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,
+                      const int32_t *dcolptr, const int32_t *drowval, double *dval, int32_t *missing, hipStream_t st) {
+  if (ncols == 0) return;
+  k_add_block<<<grid_for(ncols, 256), 256, 0, st>>>(ncols, scolptr, srowval, sval, dcolptr, drowval, dval, missing);
 }
 
 }  // namespace gtk

@@ -102,6 +102,25 @@ int gtk_set_space(gtk_ctx *ctx, int ncomp, int dofs_per_cell, int64_t n_free, in
 int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const double *phi,
                       const double *grad_phi, const double *grad_geo);
 
+/* Boundary integrals (face mode).  The context's "cells" are boundary items
+ * (a cell and one of its local faces): gtk_set_mesh with each item's cell
+ * nodes, gtk_set_space with each item's DOFs on that face's closure nodes
+ * (dofs_per_cell = face nodes x ncomp), then these tables at the nq points
+ * of the face rule mapped into the cell's reference coordinates, per local
+ * face: w[nq] (weights of the reference face), phi[nfaces][nq][nshape] (the
+ * cell's shape functions of the face's nodes), grad_geo[nfaces][nq][nv][dim]
+ * (order-1 geometry gradients), face_jac[nfaces][dim][dim-1]
+ * (d reference-cell point / d reference-face point), item_face[n_items]
+ * (0-based local face of each item).  The measure is
+ * sqrt(det(Jf' Jf)) w with Jf = J face_jac (src/accessors.jl:1000-1007 on
+ * the face map).  gtk_assemble_vector then integrates f v over the items and
+ * gtk_assemble_matrix(GTK_MASS) kappa u v, kappa per item; the pattern comes
+ * from the items' DOFs.  Replaces the boundary-measure terms
+ * (GT.measure(GT.boundary(mesh), degree), src/compiler.jl:513-529) of the
+ * generated loops. */
+int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, int nshape, const double *phi,
+                            const double *grad_geo, const double *face_jac, const int32_t *item_face);
+
 /* Symbolic phase for a (rows, cols) block: builds the CSC pattern and the
  * column-owner work plan.  Equals the pattern compress() produces for the
  * COO triplets of allocate_matrix (src/assembly.jl:75-157, :571-575);
@@ -125,6 +144,12 @@ int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int
  * values.  The cached map of the reference's reassembly path
  * (src/assembly.jl:694-771). */
 int gtk_local_nnz_map(gtk_ctx *ctx, int rows, int cols, int32_t *map_out);
+/* dst's (rows, cols) values += src's, on the device, where the two contexts
+ * number the same DOFs and src's pattern lies inside dst's (a boundary block
+ * inside the volume block): the sum of integrals over several measures that
+ * the reference compresses into one matrix (src/assembly.jl:545-575).
+ * GTK_EINVAL if a src entry is outside dst's pattern. */
+int gtk_add_block(gtk_ctx *dst, int rows, int cols, gtk_ctx *src);
 
 /* Per-cell coefficients resident on the device: kappa (ncoef=1) or
  * (lambda, mu) (ncoef=2), n_cells*ncoef values, cell-major
