@@ -1044,7 +1044,7 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   if ((rc = alloc(ctx, w.dinv, nb * vb)) || (rc = alloc(ctx, w.x, nb * 8)) || (rc = alloc(ctx, w.b, nb * 8)) ||
       (rc = alloc(ctx, w.r, nb * 8)) || (rc = alloc(ctx, w.d, nb * vb)) || (rc = alloc(ctx, w.rr, nb * vb)) ||
       (rc = alloc(ctx, w.z, nb * vb)) || (rc = alloc(ctx, w.p, nb * vb)) || (rc = alloc(ctx, w.q, nb * vb)) ||
-      (rc = alloc(ctx, w.partial, gtk::kSolvePartials * 8)) || (rc = alloc(ctx, w.partial2, gtk::kSolvePartials * 8)) ||
+      (rc = alloc(ctx, w.partial, gtk::kSolvePartials * 8)) || (rc = alloc(ctx, w.partial2, 2 * gtk::kSolvePartials * 8)) ||
       (rc = alloc(ctx, w.sc, gtk::kSolveScalars * 8)) || (rc = alloc(ctx, w.bad, 16)))
     return rc;
   gtk::SolveMat M;
@@ -1052,6 +1052,10 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   M.ptr = P.colptr.as<int32_t>();
   M.idx = w.rowval.as<int32_t>();
   M.lanes = gtk::solve_rows_per_lane_group(n ? (double)nnz / n : 1.0);
+  if (const char *e = std::getenv("GTK_SPMV_LANES")) {  // measurement override
+    const int v = std::atoi(e);
+    if (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) M.lanes = v;
+  }
   hipStream_t st = ctx->stream;
   double *sc = w.sc.as<double>();
   HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
@@ -1105,7 +1109,8 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
     int it = 0;
     while (it < max_inner) {
       for (int k = 0; k < kCheck && it < max_inner; k++, it++)
-        gtk::launch_cg_iteration(M, val, v, w.partial.as<double>(), w.partial2.as<double>(), sc, st);
+        gtk::launch_cg_iteration(M, val, v, w.partial.as<double>(), w.partial2.as<double>(), sc, it,
+                                 k == kCheck - 1 || it == max_inner - 1, st);
       double rr = 0;
       if ((rc = read_sc(5, rr))) return rc;
       if (!(std::sqrt(std::max(0.0, rr)) > itol)) break;

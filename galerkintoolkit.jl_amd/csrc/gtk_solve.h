@@ -24,8 +24,8 @@ struct CgVectors {
   const void *dinv = nullptr;
 };
 
-// Scalars kept on the device in sc[]: [0] r.z, [1] p.q, [2] alpha,
-// [3] new r.z, [4] beta, [5] r.r, [6] scratch dot.
+// Scalars kept on the device in sc[]: [0] and [3] r.z (alternating by
+// iteration), [5] r.r, [6] scratch dot.
 constexpr int kSolveScalars = 8;
 
 int solve_rows_per_lane_group(double mean_row);
@@ -35,8 +35,10 @@ void launch_dot(int64_t n, const void *a, const void *b, bool f64, double *parti
                 hipStream_t st);
 void launch_cg_start(int64_t n, double s, const double *r64, const CgVectors &v, double *partial, double *sc,
                      hipStream_t st);
+// One CG iteration (it counts from 0 after launch_cg_start; partial2 holds
+// 2 x kSolvePartials); read_rr: also sum r.r into sc[5].
 void launch_cg_iteration(const SolveMat &m, const void *val, const CgVectors &v, double *partial, double *partial2,
-                         double *sc, hipStream_t st);
+                         double *sc, int it, bool read_rr, hipStream_t st);
 void launch_refine(int64_t n, double s, const void *d, bool f64, double *x, hipStream_t st);
 void launch_diag_inv(const SolveMat &m, const double *val, void *dinv, bool f64, int *bad, hipStream_t st);
 void launch_gather_values(int64_t n, const int32_t *perm, const double *a, double *v64, float *v32, hipStream_t st);

@@ -38,6 +38,19 @@ __device__ inline double block_sum(double v, double *sh) {
   return s;  // valid in thread 0
 }
 
+// sum of np partials in a fixed order, the same in every block, returned to
+// every thread (sh: kRB / 64 + 1 doubles)
+__device__ inline double block_sum_partials(int np, const double *__restrict__ partial, double *sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += kRB) v += partial[i];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) sh[kRB / 64] = v;
+  __syncthreads();
+  v = sh[kRB / 64];
+  __syncthreads();
+  return v;
+}
+
 // y = A x (or y = b - A x when b != nullptr); S lanes per row.
 template <typename AT, typename XT, typename YT, int S>
 __global__ __launch_bounds__(kRB) void spmv_kernel(int64_t n, const int32_t *__restrict__ ptr,
@@ -50,7 +63,20 @@ __global__ __launch_bounds__(kRB) void spmv_kernel(int64_t n, const int32_t *__r
   Acc acc = 0;
   if (row < n) {
     const int32_t k0 = ptr[row] - 1, k1 = ptr[row + 1] - 1;
-    for (int32_t k = k0 + sl; k < k1; k += S) acc += (Acc)val[k] * (Acc)x[idx[k] - 1];
+    // four entries per lane in flight: index loads, then the x gathers
+    int32_t k = k0 + sl;
+    for (; k + 3 * S < k1; k += 4 * S) {
+      int32_t c[4];
+      AT v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        c[u] = idx[k + u * S];
+        v[u] = val[k + u * S];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc += (Acc)v[u] * (Acc)x[c[u] - 1];
+    }
+    for (; k < k1; k += S) acc += (Acc)val[k] * (Acc)x[idx[k] - 1];
   }
 #pragma unroll
   for (int o = S / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, S);
@@ -92,23 +118,21 @@ __global__ __launch_bounds__(kRB) void sum_partials_kernel(int np, const double 
   if (threadIdx.x == 0) sc[slot] = v;
 }
 
-// CG scalars in sc: [0] rz, [1] pq, [2] alpha, [3] rz_new, [4] beta, [5] rr
-__global__ void alpha_kernel(double *sc) {
-  sc[2] = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0;
-}
-__global__ void beta_kernel(double *sc) {
-  sc[4] = sc[0] != 0.0 ? sc[3] / sc[0] : 0.0;
-  sc[0] = sc[3];
-}
-
-// d += alpha p; r -= alpha q; z = dinv r; partials of r.z and r.r
+// CG scalars in sc: r.z of the current direction in [0] or [3] (the two
+// slots alternate by iteration), [5] r.r when read for convergence.
+//
+// alpha = r.z / p.q (p.q from the partials of p . q, summed by every
+// block in the same order); d += alpha p; r -= alpha q; z = dinv r;
+// partials of the new r.z and of r.r
 template <typename T>
-__global__ __launch_bounds__(kRB) void cg_update_kernel(int64_t n, const double *__restrict__ sc, T *__restrict__ d,
-                                                        T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
-                                                        const T *__restrict__ q, const T *__restrict__ dinv,
+__global__ __launch_bounds__(kRB) void cg_update_kernel(int64_t n, const double *__restrict__ sc, int cur,
+                                                        T *__restrict__ d, T *__restrict__ r, T *__restrict__ z,
+                                                        const T *__restrict__ p, const T *__restrict__ q,
+                                                        const T *__restrict__ dinv, const double *__restrict__ part_pq,
                                                         double *__restrict__ part_rz, double *__restrict__ part_rr) {
-  __shared__ double sh[kRB / 64];
-  const T alpha = (T)sc[2];
+  __shared__ double sh[kRB / 64 + 1];
+  const double pq = block_sum_partials(kSolvePartials, part_pq, sh);
+  const T alpha = (T)(pq != 0.0 ? sc[cur] / pq : 0.0);
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
     d[i] += alpha * p[i];
@@ -125,12 +149,18 @@ __global__ __launch_bounds__(kRB) void cg_update_kernel(int64_t n, const double 
   if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
 }
 
-// p = z + beta p
+// beta = r.z(new) / r.z; p = z + beta p; block 0 stores the new r.z in the
+// other slot (read by the next iteration only)
 template <typename T>
-__global__ __launch_bounds__(kRB) void cg_direction_kernel(int64_t n, const double *__restrict__ sc,
-                                                           const T *__restrict__ z, T *__restrict__ p) {
-  const T beta = (T)sc[4];
+__global__ __launch_bounds__(kRB) void cg_direction_kernel(int64_t n, double *__restrict__ sc, int cur,
+                                                           const double *__restrict__ part_rz, const T *__restrict__ z,
+                                                           T *__restrict__ p) {
+  __shared__ double sh[kRB / 64 + 1];
+  const double rz_new = block_sum_partials(kSolvePartials, part_rz, sh);
+  const double rz = sc[cur];
+  const T beta = (T)(rz != 0.0 ? rz_new / rz : 0.0);
   for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[cur == 0 ? 3 : 0] = rz_new;
 }
 
 // start of an inner solve: d = 0, r = s * r64, z = dinv r, p = z; partials of r.z
@@ -202,6 +232,8 @@ void launch_spmv(const SolveMat &m, const void *val, bool val64, const void *x, 
   switch (m.lanes) {
     case 4: launch_spmv_s<4>(m, val, val64, x, x64, b, y, y64, st); break;
     case 8: launch_spmv_s<8>(m, val, val64, x, x64, b, y, y64, st); break;
+    case 32: launch_spmv_s<32>(m, val, val64, x, x64, b, y, y64, st); break;
+    case 64: launch_spmv_s<64>(m, val, val64, x, x64, b, y, y64, st); break;
     default: launch_spmv_s<16>(m, val, val64, x, x64, b, y, y64, st); break;
   }
 }
@@ -225,26 +257,24 @@ void launch_cg_start(int64_t n, double s, const double *r64, const CgVectors &v,
 }
 
 void launch_cg_iteration(const SolveMat &m, const void *val, const CgVectors &v, double *partial, double *partial2,
-                         double *sc, hipStream_t st) {
+                         double *sc, int it, bool read_rr, hipStream_t st) {
   const int64_t n = m.n;
+  const int cur = (it & 1) ? 3 : 0;  // slot of the current r.z
+  double *part_rz = partial2, *part_rr = partial2 + kSolvePartials;
   launch_spmv(m, val, v.f64, v.p, v.f64, nullptr, v.q, v.f64, st);
   if (v.f64) dot_partial_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, (const double *)v.p, (const double *)v.q, partial);
   else dot_partial_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, (const float *)v.p, (const float *)v.q, partial);
-  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial, sc, 1);
-  alpha_kernel<<<1, 1, 0, st>>>(sc);
   if (v.f64)
-    cg_update_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, sc, (double *)v.d, (double *)v.r, (double *)v.z,
+    cg_update_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, sc, cur, (double *)v.d, (double *)v.r, (double *)v.z,
                                                              (const double *)v.p, (const double *)v.q,
-                                                             (const double *)v.dinv, partial, partial2);
+                                                             (const double *)v.dinv, partial, part_rz, part_rr);
   else
-    cg_update_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, sc, (float *)v.d, (float *)v.r, (float *)v.z,
+    cg_update_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, sc, cur, (float *)v.d, (float *)v.r, (float *)v.z,
                                                             (const float *)v.p, (const float *)v.q,
-                                                            (const float *)v.dinv, partial, partial2);
-  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial, sc, 3);
-  sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, partial2, sc, 5);
-  beta_kernel<<<1, 1, 0, st>>>(sc);
-  if (v.f64) cg_direction_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, sc, (const double *)v.z, (double *)v.p);
-  else cg_direction_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, sc, (const float *)v.z, (float *)v.p);
+                                                            (const float *)v.dinv, partial, part_rz, part_rr);
+  if (v.f64) cg_direction_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, sc, cur, part_rz, (const double *)v.z, (double *)v.p);
+  else cg_direction_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, sc, cur, part_rz, (const float *)v.z, (float *)v.p);
+  if (read_rr) sum_partials_kernel<<<1, kRB, 0, st>>>(kSolvePartials, part_rr, sc, 5);
 }
 
 void launch_refine(int64_t n, double s, const void *d, bool f64, double *x, hipStream_t st) {
