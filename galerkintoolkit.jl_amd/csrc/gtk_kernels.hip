@@ -632,11 +632,11 @@ struct BatchTile {
     // the row set and padding go to a private per-lane slot past the
     // accumulators
     const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
-    int rank[4];
+    int win[4];  // window of 4 ranks of each held slot (empty slots: none)
     int dst[MB][4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      rank[r] = (int)(at4[r] >> 16);
+      win[r] = (int)(at4[r] >> 18);
       const int ab = (int)(at4[r] & 0xFFFF);
 #pragma unroll
       for (int mb = 0; mb < MB; mb++) {
@@ -644,10 +644,14 @@ struct BatchTile {
         dst[mb][r] = off != (int)O4::none ? ab + off : sink;
       }
     }
-    for (int t = 0; t < tmax; t++) {
+    // step (h, r) adds slot 4g+r of rank 4h..4h+3: the slots of one column
+    // in one step are 4 ranks apart, never in one window, so no two lanes of
+    // an addition touch one entry, and the placement of the columns
+    // (batch_slot_start) makes the steps of a column come in rank order
+    for (int h = 0; h < (tmax + 3) >> 2; h++) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        if (rank[r] == t) {
+        if (win[r] == h) {
 #pragma unroll
           for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
         }

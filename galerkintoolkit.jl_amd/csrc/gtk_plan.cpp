@@ -225,12 +225,13 @@ std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols,
       int slots = kBatchSlots + 1, ncolb = 0;
       for (size_t i = 0; i < cur_cols.size(); i++) {
         const int cnt = (int)(inc.ptr[cur_cols[i] + 1] - inc.ptr[cur_cols[i]]);
-        if (slots + cnt > kBatchSlots || ncolb == kBatchCols) {
+        int start = batch_slot_start(slots, cnt);
+        if (start + cnt > kBatchSlots || ncolb == kBatchCols) {
           out.batch_col.push_back((int32_t)(out.col_id.size() + i));
-          slots = 0;
+          start = 0;
           ncolb = 0;
         }
-        slots += cnt;
+        slots = start + cnt;
         ncolb++;
         out.max_batch_cols = std::max(out.max_batch_cols, ncolb);
       }
@@ -358,6 +359,7 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
         if (len > 0) endmask[(abase + len - 1) / 64] |= 1ull << ((abase + len - 1) % 64);
         const int cnt = t.col_ctr_ptr[pc + 1] - t.col_ctr_ptr[pc];
         tmax = std::max(tmax, cnt);
+        ns = batch_slot_start(ns, cnt);  // slots skipped here stay empty
         for (int tt = 0; tt < cnt; tt++, ns++) {
           const int32_t k = t.col_ctr_ptr[pc] + tt;
           sb[ns] = (uint32_t)t.ctr_slot[k] | (uint32_t)t.ctr_b[k] << 16;

@@ -13,6 +13,16 @@ constexpr int kBatchSlots = 16;      // contributors per matrix-core batch (16 =
 constexpr int kBatchCols = 15;       // columns per batch (the 16th column entry is the padding column)
 
 // CSC pattern of one (rows, cols) block, 1-based like SparseMatrixCSC.
+// First slot of a batch column of cnt contributors placed after `used`
+// slots: a column of more than 4 contributors starts at a multiple of 4, a
+// shorter one does not straddle one.  The matrix-core kernel adds slot 4g+r
+// of rank t in step (t / 4, r); with this placement the steps of one column
+// come in rank (cell) order, and two slots of one column in one step would
+// need ranks 4 apart in one window of 4, which cannot happen.
+inline int batch_slot_start(int used, int cnt) {
+  return (cnt > 4 || used % 4 + cnt > 4) ? (used + 3) & ~3 : used;
+}
+
 struct Pattern {
   int rows = 0, cols = 0;
   int64_t nrows = 0, ncols = 0, nnz = 0;

@@ -10,9 +10,10 @@
 #include "gtk_plan.h"
 
 // Replays the batch records exactly as the matrix-core kernel consumes
-// them: per batch, in rank steps t < tmax, every slot of rank t adds its rows
-// into the batch accumulator through the record's row offsets (the replay
-// checks that no two additions of one step touch the same entry); then each
+// them: per batch, in steps (h, r), slot 4g+r adds its rows when its rank is
+// in 4h..4h+3, into the batch accumulator through the record's row offsets
+// (the replay checks that no two additions of one step touch the same
+// entry); then each
 // accumulator entry e is written to e + delta[c], c = number of column ends
 // before e in the record's end mask.  Also checks that each column's
 // contributors are ranked in ascending cell order and that every nonzero is
@@ -48,11 +49,16 @@ static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const g
       std::vector<double> acc(acc_total, 0.0);
       std::vector<int> stamp(acc_total, -1);
       std::vector<int> last_cell(16, -1);
-      int seen = 0;
-      for (int tt = 0; tt < tmax; tt++)
-        for (int n = 0; n < gtk::kBatchSlots; n++) {
+      int seen = 0, used = 0;
+      for (int n = 0; n < nslots; n++) used += (at[n] >> 16) != 0xFFFF;
+      // the kernel's steps: (h, r) adds slot 4g+r whose rank is in 4h..4h+3
+      for (int hh = 0; hh < (tmax + 3) / 4; hh++)
+        for (int rr = 0; rr < 4; rr++)
+        for (int gg = 0; gg < 4; gg++) {
+          const int n = 4 * gg + rr;
           const int tn = at[n] >> 16, abase = at[n] & 0xFFFF;
-          if (tn != tt) continue;
+          const int tt = hh * 4 + rr;  // stamp of the step
+          if (tn == 0xFFFF || tn / 4 != hh) continue;
           if (n >= nslots) return fail("empty slot with a rank");
           const int s = sb[n] & 0xFFFF, lb = sb[n] >> 16;
           if (s >= ncell || lb >= nd) return fail("slot word out of range");
@@ -80,7 +86,7 @@ static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const g
             acc[ent] += 1.0 + cells[s] * 7.0 + a * 131.0 + lb * 977.0;
           }
         }
-      if (seen != nslots) return fail("slots without a rank step");
+      if (seen != used) return fail("slots without a rank step");
       int ends = 0;
       for (int k = 0; k < acc_total; k++) ends += (endmask[k / 64] >> (k % 64)) & 1;
       if (ends != ncols || !((endmask[(acc_total - 1) / 64] >> ((acc_total - 1) % 64)) & 1))
