@@ -693,23 +693,25 @@ struct BatchTile {
     CT bfrag[MB][KS];
     bfrag_load(l.tab, lane & 15, lane >> 4, bfrag);
     const int bt0 = a.tile_batch_ptr[tile], bt1 = a.tile_batch_ptr[tile + 1];
-    auto claim = [&]() -> int {
+    // a claim is issued with the prefetch of a record and its result read
+    // one batch later, so the atomic's round trip overlaps that batch
+    auto claim_issue = [&]() -> int {
       int v = 0;
       if (lane == 0) v = atomicAdd(l.claim, 1);
-      return bt0 + (int)__builtin_amdgcn_readfirstlane(v);
+      return v;
     };
-    int bi = bt0 + wave, bn = bt1;
+    int bi = bt0 + wave, pend = bt1 - bt0;
     BatchRegs<OffT, MB> nxt;
     if (bi < bt1) {
       load_batch<CT, OffT, NS, MB>(a.rec, a, bi, lane, nxt);
-      bn = claim();
+      pend = claim_issue();
     }
     while (bi < bt1) {
       const BatchRegs<OffT, MB> cur = nxt;
-      bi = bn;
+      bi = bt0 + (int)__builtin_amdgcn_readfirstlane(pend);
       if (bi < bt1) {
         load_batch<CT, OffT, NS, MB>(a.rec, a, bi, lane, nxt);
-        bn = claim();
+        pend = claim_issue();
       }
       batch(a, l, acc, bfrag, cur, lane, sink);
       between();
