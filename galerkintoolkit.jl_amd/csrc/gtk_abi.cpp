@@ -110,9 +110,9 @@ struct gtk_ctx {
   std::vector<int32_t> item_face;
   // device copies
   DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef, d_face_jac, d_item_face;
-  // wavefronts per workgroup of the matrix-core kernels, per precision
-  // (FP64: 8, two per SIMD within 256 registers; mixed: 12, three per SIMD);
-  // 0 = the default, GTK_WAVES overrides both
+  // wavefronts per workgroup of the matrix-core kernels; 0 = per kernel:
+  // 12 (three per SIMD) when that instantiation fits its registers without
+  // scratch, else 8 (two per SIMD within 256 registers); GTK_WAVES overrides
   int waves = 0;
   size_t lds_budget = 156 * 1024;  // LDS a workgroup may use (less gives several workgroups per CU)
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
@@ -737,7 +737,18 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
     out = it->second.get();
     return GTK_OK;
   }
-  const int waves = ctx->waves ? ctx->waves : (precision == GTK_FP64 ? 8 : 12);
+  int waves = ctx->waves;
+  if (!waves) {
+    // 12 wavefronts (three per SIMD, at most 168 registers each) unless the
+    // 12-wavefront instantiation spills to scratch; then 8 (two per SIMD)
+    waves = 8;
+    gtk::KernelInfo k12;
+    hipFuncAttributes fa;
+    const int off_bytes = plan.max_seg < 0xFF ? 1 : 2;
+    if (gtk::find_matrix_kernel(elem_key(ctx), form, precision, off_bytes, true, 12, k12) && k12.mfma &&
+        hipFuncGetAttributes(&fa, k12.fn) == hipSuccess && fa.localSizeBytes == 0)
+      waves = 12;
+  }
   std::unique_ptr<ExecPlan> ep;
   int rc = build_exec(ctx, plan, form, precision, waves, 0, ep);
   if (rc) return rc;
