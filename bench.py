@@ -160,7 +160,7 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
     kdesc = "lognormal kappa per cell" if lognormal_kappa else "kappa=1"
     pdesc = f"interior vertices perturbed U(-{perturb}h,{perturb}h) (seed {args.seed}), " if perturb else ""
     return {
-        "metric": "cells/s + HBM GB/s, Q2 hex Laplace assembly; mixed vs FP64 at stated rel err",
+        "metric": f"cells/s + HBM GB/s, Q{order} hex Laplace assembly; mixed vs FP64 at stated rel err",
         "value": cells / (wallmx * 1e-3),
         "unit": "cells/s",
         "n_gpus": 1,

@@ -111,8 +111,9 @@ struct gtk_ctx {
   // device copies
   DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef, d_face_jac, d_item_face;
   // wavefronts per workgroup of the matrix-core kernels; 0 = per kernel:
-  // 12 (three per SIMD) when that instantiation fits its registers without
-  // scratch, else 8 (two per SIMD within 256 registers); GTK_WAVES overrides
+  // 16 or 12 (four or three per SIMD) when that instantiation fits its
+  // registers without scratch, else 8 (two per SIMD within 256 registers);
+  // GTK_WAVES overrides
   int waves = 0;
   size_t lds_budget = 156 * 1024;  // LDS a workgroup may use (less gives several workgroups per CU)
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
@@ -516,21 +517,31 @@ int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int
   const size_t nc1 = (size_t)std::max<int64_t>(1, nc);
   if ((rc = alloc(ctx, counts, nc1 * sizeof(int32_t))) || (rc = alloc(ctx, fill, nc1 * sizeof(int32_t))) ||
       (rc = alloc(ctx, len, nc1 * sizeof(int32_t))) || (rc = alloc(ctx, ptr, (nc1 + 1) * sizeof(int64_t))) ||
-      (rc = alloc(ctx, c64, (nc1 + 1) * sizeof(int64_t))) || (rc = alloc(ctx, maxlist, sizeof(int32_t))) ||
+      (rc = alloc(ctx, c64, (nc1 + 1) * sizeof(int64_t))) || (rc = alloc(ctx, maxlist, 2 * sizeof(int32_t))) ||
       (rc = alloc(ctx, tmp, gtk::symbolic_scan_tmp(nc) * sizeof(int64_t))))
     return rc;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, st));
   HIPCHK(ctx, hipMemsetAsync(counts.p, 0, nc1 * sizeof(int32_t), st));
   HIPCHK(ctx, hipMemsetAsync(fill.p, 0, nc1 * sizeof(int32_t), st));
-  HIPCHK(ctx, hipMemsetAsync(maxlist.p, 0, sizeof(int32_t), st));
+  // maxlist[0]: longest candidate list of a skipped column; [1]: most cells of a column
+  HIPCHK(ctx, hipMemsetAsync(maxlist.p, 0, 2 * sizeof(int32_t), st));
   gtk::launch_column_count(cd, ncell, nd, cols, counts.as<int32_t>(), st);
+  gtk::launch_max_count(counts.as<int32_t>(), nc, maxlist.as<int32_t>() + 1, st);
   gtk::launch_scan(counts.as<int32_t>(), nc, 0, ptr.as<int64_t>(), tmp.as<int64_t>(), st);
   int64_t ninc = 0;
+  int32_t max_cells = 0;
   HIPCHK(ctx, hipMemcpyAsync(&ninc, ptr.as<int64_t>() + nc, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(&max_cells, maxlist.as<int32_t>() + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(ctx, hipStreamSynchronize(st));
+  const int64_t max_list = (int64_t)max_cells * nd;
+  if (max_list > gtk::kSymbolicMaxList)
+    FAIL(ctx, GTK_EUNSUP, "a column gathers " + std::to_string(max_list) + " candidate rows, more than the " +
+                              std::to_string(gtk::kSymbolicMaxList) + " the device symbolic phase sorts: use gtk_build_pattern");
+  int cap = 64;  // LDS entries of the sort: the longest list's power of two
+  while (cap < max_list) cap <<= 1;
   if ((rc = alloc(ctx, inc, (size_t)std::max<int64_t>(1, ninc) * sizeof(int32_t)))) return rc;
   gtk::launch_column_fill(cd, ncell, nd, cols, ptr.as<int64_t>(), fill.as<int32_t>(), inc.as<int32_t>(), st);
-  gtk::launch_column_rows(cd, nd, rows, nc, ptr.as<int64_t>(), inc.as<int32_t>(), 0, len.as<int32_t>(),
+  gtk::launch_column_rows(cd, nd, rows, nc, ptr.as<int64_t>(), inc.as<int32_t>(), 0, cap, len.as<int32_t>(),
                           maxlist.as<int32_t>(), nullptr, nullptr, st);
   gtk::launch_scan(len.as<int32_t>(), nc, 1, c64.as<int64_t>(), tmp.as<int64_t>(), st);
   int32_t too_long = 0;
@@ -546,7 +557,7 @@ int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int
   if ((rc = alloc(ctx, plan->rowval, (size_t)std::max<int64_t>(1, pat.nnz) * sizeof(int32_t))) ||
       (rc = alloc(ctx, plan->colptr, (nc1 + 1) * sizeof(int32_t))))
     return rc;
-  gtk::launch_column_rows(cd, nd, rows, nc, ptr.as<int64_t>(), inc.as<int32_t>(), 1, nullptr, maxlist.as<int32_t>(),
+  gtk::launch_column_rows(cd, nd, rows, nc, ptr.as<int64_t>(), inc.as<int32_t>(), 1, cap, nullptr, maxlist.as<int32_t>(),
                           c64.as<int64_t>(), plan->rowval.as<int32_t>(), st);
   gtk::launch_colptr32(c64.as<int64_t>(), nc, plan->colptr.as<int32_t>(), st);
   HIPCHK(ctx, hipEventRecord(ctx->ev1, st));
@@ -739,15 +750,20 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
   }
   int waves = ctx->waves;
   if (!waves) {
-    // 12 wavefronts (three per SIMD, at most 168 registers each) unless the
-    // 12-wavefront instantiation spills to scratch; then 8 (two per SIMD)
+    // the most wavefronts per workgroup (16: four per SIMD within 128
+    // registers; 12: three within 168) whose instantiation needs no
+    // scratch; else 8 (two per SIMD within 256)
     waves = 8;
-    gtk::KernelInfo k12;
-    hipFuncAttributes fa;
     const int off_bytes = plan.max_seg < 0xFF ? 1 : 2;
-    if (gtk::find_matrix_kernel(elem_key(ctx), form, precision, off_bytes, true, 12, k12) && k12.mfma &&
-        hipFuncGetAttributes(&fa, k12.fn) == hipSuccess && fa.localSizeBytes == 0)
-      waves = 12;
+    for (const int w : {16, 12}) {
+      gtk::KernelInfo kw;
+      hipFuncAttributes fa;
+      if (gtk::find_matrix_kernel(elem_key(ctx), form, precision, off_bytes, true, w, kw) && kw.mfma &&
+          hipFuncGetAttributes(&fa, kw.fn) == hipSuccess && fa.localSizeBytes == 0) {
+        waves = w;
+        break;
+      }
+    }
   }
   std::unique_ptr<ExecPlan> ep;
   int rc = build_exec(ctx, plan, form, precision, waves, 0, ep);

@@ -86,7 +86,7 @@ struct KernelInfo {
 
 // allow_mfma: the plan's columns fit the matrix-core batches (at most
 // kBatchSlots contributors per column); otherwise the VALU kernel is used.
-// waves: wavefronts per workgroup of the matrix-core kernels (4 or 8).
+// waves: wavefronts per workgroup of the matrix-core kernels (8, 12 or 16).
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out);
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);

@@ -26,12 +26,16 @@ void launch_column_count(const int32_t *cell_dofs, int64_t n_cells, int nd, int 
 // must be zero on entry.
 void launch_column_fill(const int32_t *cell_dofs, int64_t n_cells, int nd, int cols, const int64_t *ptr,
                         int32_t *fill, int32_t *inc, hipStream_t st);
-// Pass 0: len[j] = number of distinct rows (of the set `rows`) of column j;
-// the longest list is reported in *maxlist.  Pass 1: the sorted rows
-// (1-based) at rowval[colptr[j] - 1 ...].
+// *out = max(*out, max of counts[0..n)).
+void launch_max_count(const int32_t *counts, int64_t n, int32_t *out, hipStream_t st);
+// Pass 0: len[j] = number of distinct rows (of the set `rows`) of column j.
+// Pass 1: the sorted rows (1-based) at rowval[colptr[j] - 1 ...].  `cap`
+// (a power of two, at most kSymbolicMaxList) bounds the candidate rows of a
+// column (cells x nd); a longer column is skipped and its length reported
+// in *maxlist.
 void launch_column_rows(const int32_t *cell_dofs, int nd, int rows, int64_t ncols, const int64_t *ptr,
-                        const int32_t *inc, int pass, int32_t *len, int32_t *maxlist, const int64_t *colptr64,
-                        int32_t *rowval, hipStream_t st);
+                        const int32_t *inc, int pass, int cap, int32_t *len, int32_t *maxlist,
+                        const int64_t *colptr64, int32_t *rowval, hipStream_t st);
 // colptr (Int32, 1-based) from the int64 scan.
 void launch_colptr32(const int64_t *colptr64, int64_t n, int32_t *colptr, hipStream_t st);
 // map[(cell*nd + b)*nd + a] = 1-based index of entry (row of a, column of b)

@@ -102,6 +102,15 @@ __global__ void k_column_count(const int32_t *cell_dofs, int64_t total, int cols
   }
 }
 
+// *out = max(*out, largest count)
+__global__ void k_max_count(const int32_t *counts, int64_t n, int32_t *out) {
+  int32_t m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, counts[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
 __global__ void k_column_fill(const int32_t *cell_dofs, int64_t total, int nd, int cols, const int64_t *ptr,
                               int32_t *fill, int32_t *inc) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -113,55 +122,90 @@ __global__ void k_column_fill(const int32_t *cell_dofs, int64_t total, int nd, i
   }
 }
 
-// one wavefront (the whole workgroup) per column
+// candidate row e of column j (0 when the DOF is not in the row set)
+__device__ inline int32_t column_candidate(const int32_t *cell_dofs, int nd, int rows, const int32_t *inc, int64_t k0,
+                                           int e) {
+  const int32_t cell = inc[k0 + e / nd];
+  return in_set(cell_dofs[(int64_t)cell * nd + e % nd], rows);
+}
+
+// one wavefront (the whole workgroup) per column: the column's candidate
+// rows (its cells' DOFs in the row set) are sorted by a bitonic network,
+// in registers (one per lane, partners by lane xor) when they fit one
+// wavefront, else in LDS (`cap` entries, a power of two, dynamic shared
+// memory); the first of each run of equal rows is kept
 __global__ __launch_bounds__(64) void k_column_rows(const int32_t *cell_dofs, int nd, int rows, int64_t ncols,
-                                                    const int64_t *ptr, const int32_t *inc, int pass, int32_t *len,
-                                                    int32_t *maxlist, const int64_t *colptr64, int32_t *rowval) {
-  __shared__ int32_t L[kSymbolicMaxList];
+                                                    const int64_t *ptr, const int32_t *inc, int pass, int cap,
+                                                    int32_t *len, int32_t *maxlist, const int64_t *colptr64,
+                                                    int32_t *rowval) {
+  extern __shared__ int32_t L[];
   const int tid = threadIdx.x;
   for (int64_t j = blockIdx.x; j < ncols; j += gridDim.x) {
     const int64_t k0 = ptr[j];
     const int64_t total = (ptr[j + 1] - k0) * nd;  // candidate rows, duplicates included
-    if (total > kSymbolicMaxList) {
+    if (total > (cap > 64 ? cap : 64)) {
       if (tid == 0) atomicMax(maxlist, (int32_t)(total < INT32_MAX ? total : INT32_MAX));
       continue;
     }
-    int P = 1;
-    while (P < total) P <<= 1;
-    for (int e = tid; e < P; e += 64) {
-      int32_t r = INT32_MAX;  // sorts to the end, never counted
-      if (e < total) {
-        const int32_t cell = inc[k0 + e / nd];
-        const int32_t m = in_set(cell_dofs[(int64_t)cell * nd + e % nd], rows);
-        if (m) r = m;
-      }
-      L[e] = r;
-    }
-    __syncthreads();
-    for (int k = 2; k <= P; k <<= 1)
-      for (int s = k >> 1; s > 0; s >>= 1) {
-        for (int i = tid; i < P / 2; i += 64) {
-          const int lo = (i / s) * 2 * s + i % s, hi = lo + s;
-          const bool up = (lo & k) == 0;
-          const int32_t x = L[lo], y = L[hi];
-          if ((x > y) == up) {
-            L[lo] = y;
-            L[hi] = x;
-          }
-        }
-        __syncthreads();
-      }
     int run = 0;
-    for (int e0 = 0; e0 < P; e0 += 64) {
-      const int e = e0 + tid;
-      const int32_t x = e < P ? L[e] : INT32_MAX;
-      const bool f = x != INT32_MAX && (e == 0 || L[e - 1] != x);
+    if (total <= 64) {
+      int32_t x = INT32_MAX;  // sorts to the end, never counted
+      if (tid < total) {
+        const int32_t m = column_candidate(cell_dofs, nd, rows, inc, k0, tid);
+        if (m) x = m;
+      }
+      for (int k = 2; k <= 64; k <<= 1)
+        for (int sd = k >> 1; sd > 0; sd >>= 1) {
+          const int32_t y = __shfl_xor(x, sd);
+          const bool up = (tid & k) == 0, lower = (tid & sd) == 0;
+          x = (lower == up) ? min(x, y) : max(x, y);
+        }
+      const int32_t prev = __shfl(x, tid > 0 ? tid - 1 : 0);
+      const bool f = x != INT32_MAX && (tid == 0 || prev != x);
       const uint64_t m = __ballot(f);
-      if (pass == 1 && f) rowval[colptr64[j] - 1 + run + __popcll(m & ((1ull << tid) - 1))] = x;
-      run += __popcll(m);
+      if (pass == 1 && f) rowval[colptr64[j] - 1 + __popcll(m & ((1ull << tid) - 1))] = x;
+      run = __popcll(m);
+    } else {
+      int P = 1, lp = 0;
+      while (P < total) {
+        P <<= 1;
+        lp++;
+      }
+      for (int e = tid; e < P; e += 64) {
+        int32_t r = INT32_MAX;
+        if (e < total) {
+          const int32_t m = column_candidate(cell_dofs, nd, rows, inc, k0, e);
+          if (m) r = m;
+        }
+        L[e] = r;
+      }
+      __syncthreads();
+      for (int lk = 1; lk <= lp; lk++)
+        for (int ls = lk - 1; ls >= 0; ls--) {
+          const int sd = 1 << ls;
+          for (int i = tid; i < P / 2; i += 64) {
+            // i-th pair of the step: lo has bit ls clear, hi = lo + sd
+            const int lo = ((i >> ls) << (ls + 1)) | (i & (sd - 1)), hi = lo + sd;
+            const bool up = ((lo >> lk) & 1) == 0;
+            const int32_t x = L[lo], y = L[hi];
+            if ((x > y) == up) {
+              L[lo] = y;
+              L[hi] = x;
+            }
+          }
+          __syncthreads();
+        }
+      for (int e0 = 0; e0 < P; e0 += 64) {
+        const int e = e0 + tid;
+        const int32_t x = e < P ? L[e] : INT32_MAX;
+        const bool f = x != INT32_MAX && (e == 0 || L[e - 1] != x);
+        const uint64_t m = __ballot(f);
+        if (pass == 1 && f) rowval[colptr64[j] - 1 + run + __popcll(m & ((1ull << tid) - 1))] = x;
+        run += __popcll(m);
+      }
+      __syncthreads();  // L is reused by the next column
     }
     if (pass == 0 && tid == 0) len[j] = run;
-    __syncthreads();  // L is reused by the next column
   }
 }
 
@@ -245,12 +289,19 @@ void launch_column_fill(const int32_t *cell_dofs, int64_t n_cells, int nd, int c
   k_column_fill<<<grid_for(total, 256), 256, 0, st>>>(cell_dofs, total, nd, cols, ptr, fill, inc);
 }
 
+void launch_max_count(const int32_t *counts, int64_t n, int32_t *out, hipStream_t st) {
+  if (n == 0) return;
+  k_max_count<<<grid_for(n, 256), 256, 0, st>>>(counts, n, out);
+}
+
 void launch_column_rows(const int32_t *cell_dofs, int nd, int rows, int64_t ncols, const int64_t *ptr,
-                        const int32_t *inc, int pass, int32_t *len, int32_t *maxlist, const int64_t *colptr64,
-                        int32_t *rowval, hipStream_t st) {
+                        const int32_t *inc, int pass, int cap, int32_t *len, int32_t *maxlist,
+                        const int64_t *colptr64, int32_t *rowval, hipStream_t st) {
   if (ncols == 0) return;
   const unsigned g = (unsigned)(ncols < (1 << 20) ? ncols : (1 << 20));
-  k_column_rows<<<g, 64, 0, st>>>(cell_dofs, nd, rows, ncols, ptr, inc, pass, len, maxlist, colptr64, rowval);
+  const size_t shmem = cap > 64 ? (size_t)cap * sizeof(int32_t) : 0;
+  k_column_rows<<<g, 64, shmem, st>>>(cell_dofs, nd, rows, ncols, ptr, inc, pass, cap, len, maxlist, colptr64,
+                                      rowval);
 }
 
 void launch_colptr32(const int64_t *colptr64, int64_t n, int32_t *colptr, hipStream_t st) {
