@@ -59,7 +59,9 @@ struct ExecPlan {
 
 // Device work space of the solve on a (FREE, FREE) block.
 struct SolveWork {
-  DevBuf rowval, perm, val64, val32, dinv, x, b, r, d, rr, z, p, q, partial, partial2, sc, bad;
+  DevBuf perm, val64, val32, dinv, x, b, r, d, rr, z, p, q, partial, partial2, sc, bad;
+  DevBuf bcol, bval;  // BSR(3) copy of the FP32 values (three-component fields)
+  int block3 = -1;    // pattern made of aligned 3x3 blocks: -1 not checked, 0 no, 1 yes
 };
 
 struct MatrixPlan {
@@ -1058,13 +1060,9 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   const bool f32 = precision == GTK_MIXED;
   const size_t vb = f32 ? sizeof(float) : sizeof(double);
   int rc;
-  if (!w.perm.p) {
-    std::vector<int32_t> perm;
-    const std::string e = gtk::build_transpose_perm(P.pat, perm);
-    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
-    if ((rc = upload(ctx, w.perm, perm.data(), perm.size()))) return rc;
-    if ((rc = upload(ctx, w.rowval, P.pat.rowval.data(), P.pat.rowval.size()))) return rc;
-  }
+  // row indices of the CSR view: the pattern's rowval on the device (the
+  // device symbolic phase leaves it there)
+  if (!P.rowval.p && (rc = upload(ctx, P.rowval, P.pat.rowval.data(), P.pat.rowval.size()))) return rc;
   if ((rc = alloc(ctx, w.val64, std::max<int64_t>(1, nnz) * sizeof(double)))) return rc;
   if (f32 && (rc = alloc(ctx, w.val32, std::max<int64_t>(1, nnz) * sizeof(float)))) return rc;
   const size_t nb = std::max<int64_t>(1, n);
@@ -1077,7 +1075,7 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   gtk::SolveMat M;
   M.n = n;
   M.ptr = P.colptr.as<int32_t>();
-  M.idx = w.rowval.as<int32_t>();
+  M.idx = P.rowval.as<int32_t>();
   M.lanes = gtk::solve_rows_per_lane_group(n ? (double)nnz / n : 1.0);
   if (const char *e = std::getenv("GTK_SPMV_LANES")) {  // measurement override
     const int v = std::atoi(e);
@@ -1085,16 +1083,48 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   }
   hipStream_t st = ctx->stream;
   double *sc = w.sc.as<double>();
+  if (!w.perm.p) {  // CSR position -> CSC position, on first solve
+    if ((rc = alloc(ctx, w.perm, std::max<int64_t>(1, nnz) * sizeof(int32_t)))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
+    gtk::launch_transpose_perm(M, w.perm.as<int32_t>(), w.bad.as<int>(), st);
+    int asym = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&asym, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    if (asym) {
+      w.perm.release();
+      FAIL(ctx, GTK_EINVAL, "the pattern is not structurally symmetric");
+    }
+  }
   HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
   gtk::launch_gather_values(nnz, w.perm.as<int32_t>(), P.nzval.as<double>(), w.val64.as<double>(),
                             f32 ? w.val32.as<float>() : nullptr, st);
   gtk::launch_diag_inv(M, w.val64.as<double>(), w.dinv.p, !f32, w.bad.as<int>(), st);
   HIPCHK(ctx, hipMemcpyAsync(w.b.p, b, n * 8, hipMemcpyHostToDevice, st));
   HIPCHK(ctx, hipMemcpyAsync(w.x.p, x, n * 8, hipMemcpyHostToDevice, st));
-  int bad = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&bad, w.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  // three-component fields numbered node by node: the FP32 iterations use a
+  // BSR(3) copy (one column index per 9 values) when the pattern is made of
+  // aligned 3x3 blocks (checked once per block; GTK_SOLVE_BSR=0 keeps CSR)
+  const char *bsr_env = std::getenv("GTK_SOLVE_BSR");
+  const bool try_bsr = f32 && ctx->ncomp == 3 && n % 3 == 0 && nnz > 0 && !(bsr_env && bsr_env[0] == '0');
+  if (try_bsr && w.block3 < 0) gtk::launch_block3_check(M, w.bad.as<int>() + 2, st);
+  int bad[4] = {0, 0, 0, 0};
+  HIPCHK(ctx, hipMemcpyAsync(bad, w.bad.p, sizeof(bad), hipMemcpyDeviceToHost, st));
   HIPCHK(ctx, hipStreamSynchronize(st));
-  if (bad) FAIL(ctx, GTK_EINVAL, "the matrix has a non-positive diagonal entry (not SPD)");
+  if (bad[0]) FAIL(ctx, GTK_EINVAL, "the matrix has a non-positive diagonal entry (not SPD)");
+  if (try_bsr && w.block3 < 0) w.block3 = bad[2] ? 0 : 1;
+  if (try_bsr && w.block3 == 1) {
+    M.nb = nnz / 9;
+    const bool first = !w.bcol.p;
+    if (first && ((rc = alloc(ctx, w.bcol, M.nb * sizeof(int32_t))) || (rc = alloc(ctx, w.bval, nnz * sizeof(float)))))
+      return rc;
+    M.bcol = w.bcol.as<int32_t>();
+    M.bval = w.bval.as<float>();
+    if (const char *e = std::getenv("GTK_BSR_LANES")) {  // measurement override
+      const int v = std::atoi(e);
+      if (v == 8 || v == 16 || v == 32) M.block_lanes = v;
+    }
+    gtk::launch_block3_build(M, w.val32.as<float>(), w.bcol.as<int32_t>(), w.bval.as<float>(), first, st);
+  }
   auto read_sc = [&](int slot, double &out) -> int {
     HIPCHK(ctx, hipMemcpyAsync(&out, sc + slot, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(ctx, hipStreamSynchronize(st));

@@ -15,6 +15,12 @@ struct SolveMat {
   const int32_t *ptr = nullptr;  // colptr, 1-based
   const int32_t *idx = nullptr;  // rowval, 1-based
   int lanes = 16;                // lanes per row of the SpMV
+  // BSR(3) copy for the FP32 iterations (null: none): block columns and the
+  // FP32 values by block entry, bval[(3i+j) * nb + block]
+  const int32_t *bcol = nullptr;
+  const float *bval = nullptr;
+  int64_t nb = 0;        // blocks
+  int block_lanes = 32;  // lanes per block row of the BSR(3) SpMV (8, 16 or 32)
 };
 
 // Vectors of one conjugate-gradient solve (FP64 or FP32).
@@ -41,6 +47,16 @@ void launch_cg_iteration(const SolveMat &m, const void *val, const CgVectors &v,
                          double *sc, int it, bool read_rr, hipStream_t st);
 void launch_refine(int64_t n, double s, const void *d, bool f64, double *x, hipStream_t st);
 void launch_diag_inv(const SolveMat &m, const double *val, void *dinv, bool f64, int *bad, hipStream_t st);
+// perm[p] = CSC position of CSR entry p of the structurally symmetric
+// pattern (m.ptr, m.idx); bad[0] = 1 when the pattern is not symmetric.
+void launch_transpose_perm(const SolveMat &m, int32_t *perm, int *bad, hipStream_t st);
+// bad[0] = 1 unless (m.ptr, m.idx) is made of 3x3 blocks (rows 3I..3I+2
+// share one column list of whole aligned triples); m.n % 3 == 0.
+void launch_block3_check(const SolveMat &m, int *bad, hipStream_t st);
+// BSR(3) copy of a pattern that passed launch_block3_check (m.nb = nnz / 9
+// blocks) from the row-ordered FP32 values; block columns when with_cols.
+void launch_block3_build(const SolveMat &m, const float *val_rows, int32_t *bcol, float *bval, bool with_cols,
+                         hipStream_t st);
 void launch_gather_values(int64_t n, const int32_t *perm, const double *a, double *v64, float *v32, hipStream_t st);
 
 }  // namespace gtk

@@ -192,6 +192,119 @@ __global__ __launch_bounds__(kRB) void refine_kernel(int64_t n, double s, const 
 }
 
 // inverse diagonal of the CSR rows (rows sorted by column index)
+// 3x3 block structure of a CSR pattern (vector fields of three components
+// numbered node by node): rows 3I, 3I+1, 3I+2 share one column list made of
+// whole aligned triples 3J, 3J+1, 3J+2.  *bad = 1 where a row breaks it.
+__global__ __launch_bounds__(kRB) void block3_check_kernel(int64_t n, const int32_t *__restrict__ ptr,
+                                                           const int32_t *__restrict__ idx, int *__restrict__ bad) {
+  for (int64_t r = (int64_t)blockIdx.x * kRB + threadIdx.x; r < n; r += (int64_t)gridDim.x * kRB) {
+    const int64_t r0 = r - r % 3;
+    const int32_t k0 = ptr[r] - 1, len = ptr[r + 1] - ptr[r];
+    const int32_t h0 = ptr[r0] - 1, hlen = ptr[r0 + 1] - ptr[r0];
+    bool ok = len % 3 == 0 && len == hlen;
+    for (int32_t t = 0; ok && t < len; t++) {
+      const int32_t c = idx[k0 + t] - 1;  // 0-based column
+      ok = c % 3 == t % 3 && (t % 3 == 0 || idx[k0 + t - 1] == c) && (r == r0 || idx[h0 + t] - 1 == c);
+    }
+    if (!ok) *bad = 1;
+  }
+}
+
+// BSR(3) copy of a CSR pattern that passed block3_check_kernel: the blocks
+// of block row I are b = (ptr[3I]-1)/9 + m, m < (ptr[3I+1]-ptr[3I])/3, the
+// m-th at block column (idx[ptr[3I]-1+3m]-1)/3; values by entry of the
+// block, bval[(3i+j) nb + b] = A[3I+i][3J+j] (FP32, from the row-ordered
+// values).  One thread per block row.
+__global__ __launch_bounds__(kRB) void block3_build_kernel(int64_t nbr, int64_t nb, const int32_t *__restrict__ ptr,
+                                                           const int32_t *__restrict__ idx,
+                                                           const float *__restrict__ val, int32_t *__restrict__ bcol,
+                                                           float *__restrict__ bval, int with_cols) {
+  for (int64_t I = (int64_t)blockIdx.x * kRB + threadIdx.x; I < nbr; I += (int64_t)gridDim.x * kRB) {
+    const int32_t k0 = ptr[3 * I] - 1, b0 = k0 / 9, len = (ptr[3 * I + 1] - ptr[3 * I]) / 3;
+    for (int32_t m = 0; m < len; m++) {
+      const int64_t b = b0 + m;
+      if (with_cols) bcol[b] = (idx[k0 + 3 * m] - 1) / 3;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) bval[(3 * i + j) * nb + b] = val[ptr[3 * I + i] - 1 + 3 * m + j];
+    }
+  }
+}
+
+// y = A x over the BSR(3) copy, FP32; S lanes per block row, two blocks per
+// lane in flight
+template <int S>
+__global__ __launch_bounds__(kRB) void bsr3_spmv_kernel(int64_t nbr, int64_t nb, const int32_t *__restrict__ ptr,
+                                                        const int32_t *__restrict__ bcol,
+                                                        const float *__restrict__ bval, const float *__restrict__ x,
+                                                        float *__restrict__ y) {
+  const int64_t I = ((int64_t)blockIdx.x * kRB + threadIdx.x) / S;
+  const int sl = threadIdx.x & (S - 1);
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  if (I < nbr) {
+    const int32_t b0 = (ptr[3 * I] - 1) / 9, b1 = b0 + (ptr[3 * I + 1] - ptr[3 * I]) / 3;
+    for (int32_t b = b0 + sl; b < b1; b += 2 * S) {
+      const int32_t bb[2] = {b, b + S};
+      int32_t c[2];
+      float v[2][9], xv[2][3];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const bool in = bb[u] < b1;
+        c[u] = in ? 3 * bcol[bb[u]] : 0;
+#pragma unroll
+        for (int e = 0; e < 9; e++) v[u][e] = in ? bval[e * nb + bb[u]] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) xv[u][j] = x[c[u] + j];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        a0 = fmaf(v[u][0], xv[u][0], fmaf(v[u][1], xv[u][1], fmaf(v[u][2], xv[u][2], a0)));
+        a1 = fmaf(v[u][3], xv[u][0], fmaf(v[u][4], xv[u][1], fmaf(v[u][5], xv[u][2], a1)));
+        a2 = fmaf(v[u][6], xv[u][0], fmaf(v[u][7], xv[u][1], fmaf(v[u][8], xv[u][2], a2)));
+      }
+    }
+  }
+#pragma unroll
+  for (int o = S / 2; o >= 1; o >>= 1) {
+    a0 += __shfl_xor(a0, o, S);
+    a1 += __shfl_xor(a1, o, S);
+    a2 += __shfl_xor(a2, o, S);
+  }
+  if (I < nbr && sl == 0) {
+    y[3 * I] = a0;
+    y[3 * I + 1] = a1;
+    y[3 * I + 2] = a2;
+  }
+}
+
+// CSR view of a structurally symmetric CSC pattern: the rows of row i are
+// the rows of column i, so CSR entry p (row i, column j) sits in column i at
+// the position of j; perm[p] = the CSC position of entry (j, i) in column j.
+// One thread per column j; every entry (i, j) finds (j, i) by a binary
+// search in column i, else *bad = 1 (the pattern is not symmetric).
+__global__ __launch_bounds__(kRB) void transpose_perm_kernel(int64_t n, const int32_t *__restrict__ ptr,
+                                                             const int32_t *__restrict__ idx,
+                                                             int32_t *__restrict__ perm, int *__restrict__ bad) {
+  for (int64_t j = (int64_t)blockIdx.x * kRB + threadIdx.x; j < n; j += (int64_t)gridDim.x * kRB) {
+    const int32_t want = (int32_t)(j + 1);
+    for (int32_t k = ptr[j] - 1; k < ptr[j + 1] - 1; k++) {
+      const int32_t i = idx[k] - 1;
+      int32_t lo = ptr[i] - 1, hi = ptr[i + 1] - 1;
+      const int32_t end = hi;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (idx[mid] < want) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < end && idx[lo] == want) perm[lo] = k;
+      else *bad = 1;
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kRB) void diag_inv_kernel(int64_t n, const int32_t *__restrict__ ptr,
                                                        const int32_t *__restrict__ idx, const double *__restrict__ val,
@@ -261,7 +374,18 @@ void launch_cg_iteration(const SolveMat &m, const void *val, const CgVectors &v,
   const int64_t n = m.n;
   const int cur = (it & 1) ? 3 : 0;  // slot of the current r.z
   double *part_rz = partial2, *part_rr = partial2 + kSolvePartials;
-  launch_spmv(m, val, v.f64, v.p, v.f64, nullptr, v.q, v.f64, st);
+  if (!v.f64 && m.bcol) {
+    const int64_t nbr = m.n / 3;
+    const unsigned g8 = (unsigned)((nbr * 8 + kRB - 1) / kRB), g16 = (unsigned)((nbr * 16 + kRB - 1) / kRB),
+                   g32 = (unsigned)((nbr * 32 + kRB - 1) / kRB);
+    const float *x = (const float *)v.p;
+    float *y = (float *)v.q;
+    if (m.block_lanes == 8) bsr3_spmv_kernel<8><<<g8, kRB, 0, st>>>(nbr, m.nb, m.ptr, m.bcol, m.bval, x, y);
+    else if (m.block_lanes == 16) bsr3_spmv_kernel<16><<<g16, kRB, 0, st>>>(nbr, m.nb, m.ptr, m.bcol, m.bval, x, y);
+    else bsr3_spmv_kernel<32><<<g32, kRB, 0, st>>>(nbr, m.nb, m.ptr, m.bcol, m.bval, x, y);
+  } else {
+    launch_spmv(m, val, v.f64, v.p, v.f64, nullptr, v.q, v.f64, st);
+  }
   if (v.f64) dot_partial_kernel<double><<<kSolvePartials, kRB, 0, st>>>(n, (const double *)v.p, (const double *)v.q, partial);
   else dot_partial_kernel<float><<<kSolvePartials, kRB, 0, st>>>(n, (const float *)v.p, (const float *)v.q, partial);
   if (v.f64)
@@ -286,6 +410,27 @@ void launch_diag_inv(const SolveMat &m, const double *val, void *dinv, bool f64,
   const dim3 grid((unsigned)((m.n + kRB - 1) / kRB));
   if (f64) diag_inv_kernel<double><<<grid, kRB, 0, st>>>(m.n, m.ptr, m.idx, val, (double *)dinv, bad);
   else diag_inv_kernel<float><<<grid, kRB, 0, st>>>(m.n, m.ptr, m.idx, val, (float *)dinv, bad);
+}
+
+void launch_block3_check(const SolveMat &m, int *bad, hipStream_t st) {
+  if (m.n == 0) return;
+  const int64_t blocks = (m.n + kRB - 1) / kRB;
+  block3_check_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), kRB, 0, st>>>(m.n, m.ptr, m.idx, bad);
+}
+
+void launch_block3_build(const SolveMat &m, const float *val_rows, int32_t *bcol, float *bval, bool with_cols,
+                         hipStream_t st) {
+  const int64_t nbr = m.n / 3;
+  if (nbr == 0) return;
+  const int64_t blocks = (nbr + kRB - 1) / kRB;
+  block3_build_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), kRB, 0, st>>>(nbr, m.nb, m.ptr, m.idx, val_rows,
+                                                                                   bcol, bval, with_cols ? 1 : 0);
+}
+
+void launch_transpose_perm(const SolveMat &m, int32_t *perm, int *bad, hipStream_t st) {
+  if (m.n == 0) return;
+  const int64_t blocks = (m.n + kRB - 1) / kRB;
+  transpose_perm_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), kRB, 0, st>>>(m.n, m.ptr, m.idx, perm, bad);
 }
 
 void launch_gather_values(int64_t n, const int32_t *perm, const double *a, double *v64, float *v32, hipStream_t st) {
