@@ -122,6 +122,13 @@ struct NGeo {
   static constexpr int value = FORM == kLaplace ? D * (D + 1) / 2 : (FORM == kMass ? 1 : D * D + 3);
 };
 
+// elasticity on affine cells (order-1 simplices, J the same at every
+// point): one staged factor set per cell instead of one per point
+template <int D, int NV, int FORM, int NL>
+struct CellGeo {
+  static constexpr bool value = FORM == kElasticity && NV == D + 1 && NL == 0;
+};
+
 template <int D>
 __device__ inline int sym_index(int i, int j) {
   // packed symmetric storage: (0,0) (0,1) (0,2) (1,1) (1,2) (2,2) / (0,0) (0,1) (1,1)
@@ -145,10 +152,15 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   constexpr int NWAVES = kBlock / 64;
   constexpr int NT = NL > 0 ? NL : 1;            // table sets
   constexpr int NGPHI = NL > 0 ? 0 : NQ * D * NS;  // face mode: no gradient table
+  // affine cells (simplices with order-1 geometry) have one J at every
+  // point: elasticity keeps one factor set per cell (J^-1, |det J|, lambda,
+  // mu) and takes w_q in the point loop
+  constexpr bool kCellGeo = CellGeo<D, NV, FORM, NL>::value;
+  constexpr int GQ = kCellGeo ? 1 : NQ;  // factor sets per cell
   extern __shared__ __align__(16) unsigned char lds_raw[];
   CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NQ][D][NS]
   CT *phi_s = gphi_s + NGPHI;                      // [NT][NQ][NS]
-  CT *G_s = phi_s + NT * NQ * NS;                  // [cells][NQ][NG]
+  CT *G_s = phi_s + NT * NQ * NS;                  // [cells][GQ][NG]
   double *acc_base = reinterpret_cast<double *>(lds_raw + args.acc_offset);
 
   const int tile = blockIdx.x;
@@ -156,16 +168,16 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
   const int col0 = args.tile_col_ptr[tile];
   const int ncol = args.tile_col_ptr[tile + 1] - col0;
-  int32_t *lf_s = reinterpret_cast<int32_t *>(G_s + ncell * NQ * NG);  // [cells] local face (NL > 0)
+  int32_t *lf_s = reinterpret_cast<int32_t *>(G_s + ncell * GQ * NG);  // [cells] local face (NL > 0)
 
   for (int i = threadIdx.x; i < NGPHI; i += kBlock) gphi_s[i] = (CT)args.gphi[i];
   for (int i = threadIdx.x; i < NT * NQ * NS; i += kBlock) phi_s[i] = (CT)args.phi[i];
   if constexpr (NL > 0)
     for (int i = threadIdx.x; i < ncell; i += kBlock) lf_s[i] = args.item_face[args.tile_cells[cell0 + i]];
 
-  // phase A: FP64 geometry per (cell, point)
-  for (int idx = threadIdx.x; idx < ncell * NQ; idx += kBlock) {
-    const int s = idx / NQ, q = idx - s * NQ;
+  // phase A: FP64 geometry per (cell, point), or per cell (kCellGeo)
+  for (int idx = threadIdx.x; idx < ncell * GQ; idx += kBlock) {
+    const int s = idx / GQ, q = idx - s * GQ;
     const int64_t cell = args.tile_cells[cell0 + s];
     double Ji[D][D];
     double dV;
@@ -175,9 +187,10 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
       cell_jacobian_only<D, NV>(args.xyz, args.cell_nodes, args.gN + (size_t)l * NQ * NV * D, cell, q, J);
       dV = surface_measure<D>(J, args.face_jac + l * D * (D - 1)) * args.w[q];
     } else {
-      dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji) * args.w[q];
+      dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji);
+      if constexpr (!kCellGeo) dV *= args.w[q];
     }
-    CT *G = G_s + (s * NQ + q) * NG;
+    CT *G = G_s + (s * GQ + q) * NG;
     if constexpr (FORM == kLaplace) {
       const double k = args.coef ? args.coef[cell] : 1.0;
       const double kdV = k * dV;
@@ -212,6 +225,11 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   const bool active = cs < CPW;
   double *acc = acc_base + (size_t)(wave * CPW + (active ? cs : 0)) * args.max_seg;
   const int an = a / NC, ac = a - an * NC;
+  CT wq[kCellGeo ? NQ : 1];
+  if constexpr (kCellGeo) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) wq[q] = (CT)args.w[q];
+  }
   for (int base = wave * CPW; base < ncol; base += NWAVES * CPW) {
     const int ci = base + cs;
     if (!active || ci >= ncol) continue;
@@ -224,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
     for (int k = args.col_ctr_ptr[pc]; k < k1; k++) {
       const int s = args.ctr_slot[k], b = args.ctr_b[k];
       const int bn = b / NC, bc = b - bn * NC;
-      const CT *Gs = G_s + s * NQ * NG;
+      const CT *Gs = G_s + s * GQ * NG;
       CT v = 0;
       if constexpr (FORM == kLaplace) {
 #pragma unroll
@@ -251,9 +269,14 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
       } else {
         // sigma(eps(phi_b e_bc)) : eps(phi_a e_ac)
         //   = lambda (g_b)_bc (g_a)_ac + mu (delta_{ac,bc} g_a.g_b + (g_a)_bc (g_b)_ac)
+        CT Gc[kCellGeo ? NG : 1];  // the cell's factors, read once (kCellGeo)
+        if constexpr (kCellGeo) {
+#pragma unroll
+          for (int i = 0; i < NG; i++) Gc[i] = Gs[i];
+        }
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-          const CT *G = Gs + q * NG;
+          const CT *G = kCellGeo ? Gc : Gs + q * NG;
           CT ga[D], gb[D];
 #pragma unroll
           for (int i = 0; i < D; i++) {
@@ -271,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
           for (int i = 0; i < D; i++) dot = fma(ga[i], gb[i], dot);
           const CT lam = G[D * D + 1], mu = G[D * D + 2];
           CT t = lam * gb[bc] * ga[ac] + mu * ((ac == bc ? dot : (CT)0) + ga[bc] * gb[ac]);
-          v = fma(t, G[D * D], v);
+          v = fma(t, kCellGeo ? G[D * D] * wq[kCellGeo ? q : 0] : G[D * D], v);
         }
       }
       const int off = args.off_bytes == 1 ? (int)args.ctr_off[(int64_t)k * args.off_stride + a]
@@ -1088,7 +1111,8 @@ static KernelInfo matrix_info() {
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl>);
   k.table_bytes = (size_t)((E::nl > 0 ? 0 : E::nq * E::d * E::ns) + NT * E::nq * E::ns) * sizeof(CT);
-  k.geo_bytes_per_cell = (size_t)E::nq * NGeo<E::d, FORM>::value * sizeof(CT) + (E::nl > 0 ? sizeof(int32_t) : 0);
+  constexpr int GQ = CellGeo<E::d, E::nv, FORM, E::nl>::value ? 1 : E::nq;
+  k.geo_bytes_per_cell = (size_t)GQ * NGeo<E::d, FORM>::value * sizeof(CT) + (E::nl > 0 ? sizeof(int32_t) : 0);
   k.cols_per_wave = 64 / (E::ns * E::nc);
   return k;
 }

@@ -95,12 +95,13 @@ def test_dirichlet_columns(GT, O):
     assert relfro(Ad.nzval, nz) <= TOL_FP64
 
 
+@pytest.mark.parametrize("cells", [(3, 2, 3), (8, 6, 7)])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("mixed", TOL_MIXED)])
-def test_elasticity_p2_tets(GT, O, precision, tol):
+def test_elasticity_p2_tets(GT, O, precision, tol, cells):
     """P2 vector tetrahedra, per-cell Lame parameters, clamped on x = 0
-    (group 2-face-5), body force (0, 0, -1)."""
-    print(f"case elasticity p2 tets {precision}", flush=True)
-    cells = (3, 2, 3)
+    (group 2-face-5), body force (0, 0, -1); the larger mesh spans several
+    tiles."""
+    print(f"case elasticity p2 tets {precision} {cells}", flush=True)
     m, Om, V, p = build(GT, O, cells, 2, simplex=True, diri=2, group=5, ncomp=3, perturb=0.1)
     dO = GT.measure(Om, 5)
     p.quadrature(5)
