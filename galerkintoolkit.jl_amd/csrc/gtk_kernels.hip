@@ -122,6 +122,13 @@ struct NGeo {
   static constexpr int value = FORM == kLaplace ? D * (D + 1) / 2 : (FORM == kMass ? 1 : D * D + 3);
 };
 
+// lanes per column of the VALU matrix kernel: one per local row; for
+// elasticity one per node (NC = D rows each)
+template <int NS, int NC, int FORM>
+struct ValuLanes {
+  static constexpr int value = FORM == kElasticity ? NS : NS * NC;
+};
+
 // elasticity on affine cells (order-1 simplices, J the same at every
 // point): one staged factor set per cell instead of one per point
 template <int D, int NV, int FORM, int NL>
@@ -148,7 +155,11 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   static_assert(NL == 0 || FORM == kMass, "boundary items: mass form only");
   constexpr int NG = NGeo<D, FORM>::value;
   constexpr int NDOF = NS * NC;
-  constexpr int CPW = 64 / NDOF;
+  // lanes per column: one per local row, or (elasticity) one per node, that
+  // lane forming the node's NC rows from one pair of physical gradients
+  constexpr int LPC = ValuLanes<NS, NC, FORM>::value;
+  constexpr int RPL = NDOF / LPC;  // rows per lane
+  constexpr int CPW = 64 / LPC;
   constexpr int NWAVES = kBlock / 64;
   constexpr int NT = NL > 0 ? NL : 1;            // table sets
   constexpr int NGPHI = NL > 0 ? 0 : NQ * D * NS;  // face mode: no gradient table
@@ -221,10 +232,11 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
 
   // phase B: owned columns
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int cs = lane / NDOF, a = lane - cs * NDOF;
+  const int cs = lane / LPC, a = lane - cs * LPC;  // a: local row, or node (RPL > 1)
   const bool active = cs < CPW;
   double *acc = acc_base + (size_t)(wave * CPW + (active ? cs : 0)) * args.max_seg;
-  const int an = a / NC, ac = a - an * NC;
+  const int an = RPL > 1 ? a : a / NC;
+  const int ac = RPL > 1 ? 0 : a - an * NC;  // component of row a (one row per lane)
   CT wq[kCellGeo ? NQ : 1];
   if constexpr (kCellGeo) {
 #pragma unroll
@@ -237,13 +249,15 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
     const int64_t j = args.col_id[pc];
     const int64_t seg0 = (int64_t)args.colptr[j] - 1;
     const int len = args.colptr[j + 1] - args.colptr[j];
-    for (int r = a; r < len; r += NDOF) acc[r] = 0.0;
+    for (int r = a; r < len; r += LPC) acc[r] = 0.0;
     const int k1 = args.col_ctr_ptr[pc + 1];
     for (int k = args.col_ctr_ptr[pc]; k < k1; k++) {
       const int s = args.ctr_slot[k], b = args.ctr_b[k];
       const int bn = b / NC, bc = b - bn * NC;
       const CT *Gs = G_s + s * GQ * NG;
-      CT v = 0;
+      CT v[RPL];  // rows an * NC + r of the lane (RPL > 1), or row a
+#pragma unroll
+      for (int r = 0; r < RPL; r++) v[r] = 0;
       if constexpr (FORM == kLaplace) {
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -259,13 +273,13 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
             CT wd = 0;
 #pragma unroll
             for (int e = 0; e < D; e++) wd = fma(G[sym_index<D>(d, e)], gb[e], wd);
-            v = fma(ga[d], wd, v);
+            v[0] = fma(ga[d], wd, v[0]);
           }
         }
       } else if constexpr (FORM == kMass) {
         const CT *ph = phi_s + (NL > 0 ? lf_s[s] * NQ * NS : 0);
 #pragma unroll
-        for (int q = 0; q < NQ; q++) v = fma(ph[q * NS + an], Gs[q * NG] * ph[q * NS + bn], v);
+        for (int q = 0; q < NQ; q++) v[0] = fma(ph[q * NS + an], Gs[q * NG] * ph[q * NS + bn], v[0]);
       } else {
         // sigma(eps(phi_b e_bc)) : eps(phi_a e_ac)
         //   = lambda (g_b)_bc (g_a)_ac + mu (delta_{ac,bc} g_a.g_b + (g_a)_bc (g_b)_ac)
@@ -293,15 +307,27 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
 #pragma unroll
           for (int i = 0; i < D; i++) dot = fma(ga[i], gb[i], dot);
           const CT lam = G[D * D + 1], mu = G[D * D + 2];
-          CT t = lam * gb[bc] * ga[ac] + mu * ((ac == bc ? dot : (CT)0) + ga[bc] * gb[ac]);
-          v = fma(t, kCellGeo ? G[D * D] * wq[kCellGeo ? q : 0] : G[D * D], v);
+          const CT dV = kCellGeo ? G[D * D] * wq[kCellGeo ? q : 0] : G[D * D];
+#pragma unroll
+          for (int ac = 0; ac < RPL; ac++) {
+            CT t = lam * gb[bc] * ga[ac] + mu * ((ac == bc ? dot : (CT)0) + ga[bc] * gb[ac]);
+            v[ac] = fma(t, dV, v[ac]);
+          }
         }
       }
-      const int off = args.off_bytes == 1 ? (int)args.ctr_off[(int64_t)k * args.off_stride + a]
-                                          : (int)reinterpret_cast<const uint16_t *>(args.ctr_off)[(int64_t)k * args.off_stride + a];
-      if (off != (args.off_bytes == 1 ? 0xFF : 0xFFFF)) acc[off] += (double)v;
+      // vector fields: the scalar forms couple equal components only
+      // (u . v, grad u : grad v); the pattern holds every component pair
+      if constexpr (FORM != kElasticity && NC > 1)
+        if (ac != bc) v[0] = 0;
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int64_t e = (int64_t)k * args.off_stride + (RPL > 1 ? an * NC + r : a);
+        const int off = args.off_bytes == 1 ? (int)args.ctr_off[e]
+                                            : (int)reinterpret_cast<const uint16_t *>(args.ctr_off)[e];
+        if (off != (args.off_bytes == 1 ? 0xFF : 0xFFFF)) acc[off] += (double)v[r];
+      }
     }
-    for (int r = a; r < len; r += NDOF) args.nzval[seg0 + r] = acc[r];
+    for (int r = a; r < len; r += LPC) args.nzval[seg0 + r] = acc[r];
   }
 }
 
@@ -1113,7 +1139,7 @@ static KernelInfo matrix_info() {
   k.table_bytes = (size_t)((E::nl > 0 ? 0 : E::nq * E::d * E::ns) + NT * E::nq * E::ns) * sizeof(CT);
   constexpr int GQ = CellGeo<E::d, E::nv, FORM, E::nl>::value ? 1 : E::nq;
   k.geo_bytes_per_cell = (size_t)GQ * NGeo<E::d, FORM>::value * sizeof(CT) + (E::nl > 0 ? sizeof(int32_t) : 0);
-  k.cols_per_wave = 64 / (E::ns * E::nc);
+  k.cols_per_wave = 64 / ValuLanes<E::ns, E::nc, FORM>::value;
   return k;
 }
 

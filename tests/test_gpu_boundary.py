@@ -90,6 +90,25 @@ def test_boundary_mass(GT, case, precision, tol):
         assert relfro(A.nzval, R.data) <= tol, (rows, cols)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("mixed", 1e-6)])
+def test_boundary_mass_vector(GT, precision, tol):
+    """Three-component P2 field on tetrahedron faces: the mass u . v couples
+    equal components only; the pattern holds every component pair of a
+    face (explicit zeros where components differ), so the values are
+    compared as matrices."""
+    case = CASES[5]
+    print(f"case boundary mass vector {case} {precision}", flush=True)
+    m, Om, V, G, p, B = setup(GT, *case)
+    degree = 2 * case[1]
+    dG = GT.measure(G, degree)
+    kappa = GT.inputs.lognormal(8, len(G.faces))
+    prec = GT.FP64 if precision == "fp64" else GT.MIXED
+    A = GT.assemble_matrix(GT.Mass(dG, GT.face_constant_field(kappa, G)), float, V, V, precision=prec)
+    R = B.mass(degree, coef=kappa, shape=(V.nfree, V.nfree)).toarray()
+    Ad = A.to_scipy().toarray()
+    assert np.linalg.norm(Ad - R) / np.linalg.norm(R) <= tol
+
+
 @pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[4]])
 def test_volume_plus_boundary(GT, case):
     """Laplace over the cells plus a Robin mass over boundary faces, added on
