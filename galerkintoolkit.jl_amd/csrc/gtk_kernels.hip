@@ -419,12 +419,32 @@ constexpr int kBatchSlotsK = 16;  // contributor slots per matrix-core batch (MF
 // NQP x NGP block rounded up to an odd number of the units the batch loop
 // reads (8 bytes for FP32 rows, 16 bytes for FP64 rows), so that the 16
 // slots of a batch, which read 16 different cells, spread over the LDS banks.
-template <int NQP, int NGP, typename CT>
+// EXTRA: entries kept after the block (the split product's per-cell scale).
+template <int NQP, int NGP, typename CT, int EXTRA = 0>
 struct GeoStride {
   static constexpr int unit = (NGP * (int)sizeof(CT)) % 16 == 0 ? 16 : ((NGP * (int)sizeof(CT)) % 8 == 0 ? 8 : 4);
-  static constexpr int units = (NQP * NGP * (int)sizeof(CT) + unit - 1) / unit;
+  static constexpr int units = ((NQP * NGP + EXTRA) * (int)sizeof(CT) + unit - 1) / unit;
   static constexpr int value = ((units | 1) * unit) / (int)sizeof(CT);
 };
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// D(16x16) += A(16x32) B(32x16), FP16 inputs, FP32 accumulation; lane l
+// supplies A[l%16][8(l/16)+i] and B[8(l/16)+i][l%16], i = 0..7, and holds D
+// as the FP32 16x16x4 product (pinned by tests/test_gpu_mfma_layout.py).
+__device__ inline v4f mfma16x16x32(half8 a, half8 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// x = hi + lo + e with hi, lo FP16 (round to nearest), |e| <= 2^-22 |x| for
+// x in FP16's normal range; two values per conversion
+__device__ inline void split_f16(float x0, float x1, f16x2 &hi, f16x2 &lo) {
+  const f32x2 x = {x0, x1};
+  hi = __builtin_convertvector(x, f16x2);
+  lo = __builtin_convertvector(x - __builtin_convertvector(hi, f32x2), f16x2);
+}
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -503,7 +523,17 @@ struct BatchTile {
   static constexpr int TP = FORM == kLaplace ? (D == 3 ? 4 : 2) : 1;  // table entries per (q, b)
   static constexpr int QG = (NQ + 3) / 4;
   static constexpr int NQP = 4 * QG;  // quadrature rows padded with zero rows to the four lane groups
-  static constexpr int GS = GeoStride<NQP, NGP, CT>::value;  // CT elements per cell of staged factors
+  static constexpr int KS_ = QG * (FORM == kLaplace ? D : 1);
+  static constexpr int MB_ = (NS + 15) / 16;
+  // mixed form with a long K: the product on FP16 matrix cores, W and Bt
+  // each split into two FP16 parts (hi*hi + hi*lo + lo*hi, FP32
+  // accumulation), 32 K per instruction; the staged factors of a cell are
+  // scaled by a power of two (kept after them, GS - 1) so that W's parts
+  // stay in FP16's normal range
+  static constexpr bool kSplit = sizeof(CT) == 4 && KS_ * MB_ >= 12;
+  static constexpr int KT = (KS_ + 7) / 8;  // 32-wide K steps of the split product
+  static constexpr int kScaleExp = 9;       // scaled |G| about 2^3 .. 2^7
+  static constexpr int GS = GeoStride<NQP, NGP, CT, kSplit ? 1 : 0>::value;  // CT elements per cell of staged factors
   static constexpr int KS = QG * KD;
   static constexpr int MB = (NS + 15) / 16;
   static constexpr int CPR = kB / NQ;  // cells per geometry round
@@ -599,7 +629,24 @@ struct BatchTile {
           for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
       double Ji[D][D];
       const double det = inverse<D>(J, Ji);
-      const double kdV = l.K[sl] * fabs(det) * wq;
+      double kdV = l.K[sl] * fabs(det) * wq;
+      if constexpr (kSplit) {
+        // the cell's scale 2^e from kappa and its first edge h, the same in
+        // every thread of the cell: |G| ~ kappa h^(D-2) w (Laplace),
+        // kappa h^D w (mass); 2^-e is kept for the products
+        double h2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          const double e = X[D + d] - X[d];
+          h2 += e * e;
+        }
+        const double h = sqrt(h2);
+        const double mag = l.K[sl] * (FORM == kLaplace ? (D == 3 ? h : 1.0) : (D == 3 ? h * h * h : h * h));
+        int e = 0;
+        if (mag > 0.0 && mag < 1e300) e = min(100, max(-100, kScaleExp - ilogb(mag)));
+        kdV = ldexp(kdV, e);
+        if (tq == 0) l.G[(c0 + sl) * GS + GS - 1] = (CT)ldexp(1.0, -e);
+      }
       CT *G = l.G + (c0 + sl) * GS + tq * NGP;
       if constexpr (NQP > NQ) {  // padding rows of the cell: zero
         if (tq < NQP - NQ) {
@@ -638,9 +685,35 @@ struct BatchTile {
       }
   }
 
+  // the B operand of the split product: for K step t, lane group g supplies
+  // its values i = 8t .. 8t+7 in the order (point j = i / KD, direction
+  // d = i % KD) of the FP32 product, zero past KS; hi and lo FP16 parts
+  __device__ static void bfrag_split_load(const CT *tab, int nl, int g, half8 (&bh)[MB][KT], half8 (&bl)[MB][KT]) {
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+      for (int t = 0; t < KT; t++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+          float v[2];
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const int i = 8 * t + 2 * p + u, a = mb * 16 + nl, q = g * QG + i / KD, d = i % KD;
+            v[u] = (i < KS && a < NS && q < NQ) ? (float)tab[(q * NS + a) * TP + d] : 0.0f;
+          }
+          f16x2 hi, lo;
+          split_f16(v[0], v[1], hi, lo);
+          bh[mb][t][2 * p] = hi[0];
+          bh[mb][t][2 * p + 1] = hi[1];
+          bl[mb][t][2 * p] = lo[0];
+          bl[mb][t][2 * p + 1] = lo[1];
+        }
+  }
+
   // one batch: W, the matrix-core product, the rank-step additions and the
   // flat write-out
-  __device__ static void batch(const MatrixArgs &a, const Lds &l, double *acc, const CT (&bfrag)[MB][KS],
+  __device__ static void batch(const MatrixArgs &a, const Lds &l, double *acc, const CT (&bfrag)[MB][kSplit ? 1 : KS],
+                               const half8 (&bh)[MB][kSplit ? KT : 1], const half8 (&bl)[MB][kSplit ? KT : 1],
                                const BatchRegs<OffT, MB> &cur, int lane, int sink) {
     const int g = lane >> 4;
     const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
@@ -653,28 +726,71 @@ struct BatchTile {
     V4 c[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
+    if constexpr (kSplit) {
+      float w[KT * 8];  // this lane's W values in the split product's K order
 #pragma unroll
-    for (int j = 0; j < QG; j++) {
-      CT G[NGP], T[TP];
-      lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
-      lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
-      CT W[KD];
-      if constexpr (FORM == kLaplace) {
-        if constexpr (D == 3) {
-          W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
-          W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
-          W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+      for (int j = 0; j < QG; j++) {
+        CT G[NGP], T[TP];
+        lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
+        lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
+        if constexpr (FORM == kLaplace) {
+          if constexpr (D == 3) {
+            w[j * 3 + 0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
+            w[j * 3 + 1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
+            w[j * 3 + 2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+          } else {
+            w[j * 2 + 0] = G[0] * T[0] + G[1] * T[1];
+            w[j * 2 + 1] = G[1] * T[0] + G[2] * T[1];
+          }
         } else {
-          W[0] = G[0] * T[0] + G[1] * T[1];
-          W[1] = G[1] * T[0] + G[2] * T[1];
+          w[j] = G[0] * T[0];
         }
-      } else {
-        W[0] = G[0] * T[0];
       }
 #pragma unroll
-      for (int d = 0; d < KD; d++) {
+      for (int i = KS; i < KT * 8; i++) w[i] = 0.0f;
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
+      for (int t = 0; t < KT; t++) {
+        half8 wh, wl;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+          f16x2 hi, lo;
+          split_f16(w[8 * t + 2 * p], w[8 * t + 2 * p + 1], hi, lo);
+          wh[2 * p] = hi[0];
+          wh[2 * p + 1] = hi[1];
+          wl[2 * p] = lo[0];
+          wl[2 * p + 1] = lo[1];
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) {
+          c[mb] = mfma16x16x32(wh, bh[mb][t], c[mb]);
+          c[mb] = mfma16x16x32(wh, bl[mb][t], c[mb]);
+          c[mb] = mfma16x16x32(wl, bh[mb][t], c[mb]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < QG; j++) {
+        CT G[NGP], T[TP];
+        lds_load<CT, NGP, (NGP * sizeof(CT)) % 16 == 0 ? 16 : 8>(Gc + j * NGP, G);
+        lds_load<CT, TP, TP * sizeof(CT) >= 16 ? 16 : (TP * sizeof(CT) >= 8 ? 8 : 4)>(Tb + j * NS * TP, T);
+        CT W[KD];
+        if constexpr (FORM == kLaplace) {
+          if constexpr (D == 3) {
+            W[0] = G[0] * T[0] + G[1] * T[1] + G[2] * T[2];
+            W[1] = G[1] * T[0] + G[3] * T[1] + G[4] * T[2];
+            W[2] = G[2] * T[0] + G[4] * T[1] + G[5] * T[2];
+          } else {
+            W[0] = G[0] * T[0] + G[1] * T[1];
+            W[1] = G[1] * T[0] + G[2] * T[1];
+          }
+        } else {
+          W[0] = G[0] * T[0];
+        }
+#pragma unroll
+        for (int d = 0; d < KD; d++) {
+#pragma unroll
+          for (int mb = 0; mb < MB; mb++) c[mb] = mfma16x16x4(W[d], bfrag[mb][j * KD + d], c[mb]);
+        }
       }
     }
     // destinations of the 4 slots x MB rows this lane holds; rows outside
@@ -693,6 +809,13 @@ struct BatchTile {
         dst[mb][r] = off != (int)O4::none ? ab + off : sink;
       }
     }
+    // split product: the rows of slot 4g+r come scaled by its cell's 2^e;
+    // lane 4g+r (group 0) holds that slot's cell
+    double us[4] = {1.0, 1.0, 1.0, 1.0};
+    if constexpr (kSplit) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) us[r] = (double)l.G[__shfl((int)(cur.sb & 0xFFFF), 4 * g + r) * GS + GS - 1];
+    }
     // step (h, r) adds slot 4g+r of rank 4h..4h+3: the slots of one column
     // in one step are 4 ranks apart, never in one window, so no two lanes of
     // an addition touch one entry, and the placement of the columns
@@ -702,7 +825,8 @@ struct BatchTile {
       for (int r = 0; r < 4; r++) {
         if (win[r] == h) {
 #pragma unroll
-          for (int mb = 0; mb < MB; mb++) atomicAdd(&acc[dst[mb][r]], (double)c[mb][r]);
+          for (int mb = 0; mb < MB; mb++)
+            atomicAdd(&acc[dst[mb][r]], kSplit ? (double)c[mb][r] * us[r] : (double)c[mb][r]);
         }
       }
     }
@@ -739,8 +863,10 @@ struct BatchTile {
     double *acc = l.acc + (size_t)wave * a.acc_per_wave;
     const int sink = a.acc_per_wave - 64 + lane;
     for (int i = lane; i < a.acc_per_wave; i += 64) acc[i] = 0.0;
-    CT bfrag[MB][KS];
-    bfrag_load(l.tab, lane & 15, lane >> 4, bfrag);
+    CT bfrag[MB][kSplit ? 1 : KS];
+    half8 bh[MB][kSplit ? KT : 1], bl[MB][kSplit ? KT : 1];
+    if constexpr (kSplit) bfrag_split_load(l.tab, lane & 15, lane >> 4, bh, bl);
+    else bfrag_load(l.tab, lane & 15, lane >> 4, bfrag);
     const int bt0 = a.tile_batch_ptr[tile], bt1 = a.tile_batch_ptr[tile + 1];
     // a claim is issued with the prefetch of a record and its result read
     // one batch later, so the atomic's round trip overlaps that batch
@@ -762,7 +888,7 @@ struct BatchTile {
         load_batch<CT, OffT, NS, MB>(a.rec, a, bi, lane, nxt);
         pend = claim_issue();
       }
-      batch(a, l, acc, bfrag, cur, lane, sink);
+      batch(a, l, acc, bfrag, bh, bl, cur, lane, sink);
       between();
     }
   }
@@ -1107,7 +1233,7 @@ static KernelInfo matrix_batch_info() {
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&matrix_batch_kernel<E::d, E::nv, E::ns, E::nq, FORM, CT, OffT, NW>);
   k.table_bytes = (size_t)NQP * E::ns * TP * sizeof(CT);
-  k.geo_bytes_per_cell = (size_t)GeoStride<NQP, NGP, CT>::value * sizeof(CT);
+  k.geo_bytes_per_cell = (size_t)BatchTile<E::d, E::nv, E::ns, E::nq, FORM, CT, OffT, NW>::GS * sizeof(CT);
   k.rounds = (kBatchSlotsK * E::ns + 63) / 64;
   k.cols_per_wave = 16;
   k.mfma = true;

@@ -30,6 +30,37 @@ __global__ __launch_bounds__(64) void probe_f32(float *out, int chain) {
   for (int r = 0; r < 4; r++) out[l * 4 + r] = c[r];
 }
 
+// v_mfma_f32_16x16x32_f16: lane l supplies A[l%16][8*(l/16) + i] and
+// B[8*(l/16) + i][l%16], i = 0..7 (the layout this probe checks); the same
+// A and B values as above, so D[i][j] = 1000 i + j again.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(64) void probe_f16x32(float *out, int chain) {
+  const int l = threadIdx.x;
+  const int m = l & 15, g = l >> 4;
+  half8 a, b;
+  for (int i = 0; i < 8; i++) {
+    const int k = 8 * g + i;
+    a[i] = (_Float16)(k == 0 ? (float)m : (k == 1 ? 1.0f : 0.0f));
+    b[i] = (_Float16)(k == 0 ? 1000.0f : (k == 1 ? (float)m : 0.0f));
+  }
+  v4f c = {0, 0, 0, 0};
+  for (int t = 0; t < chain; t++) c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  for (int r = 0; r < 4; r++) out[l * 4 + r] = c[r];
+}
+
+extern "C" int mfma_probe_f16x32(int chain, double *host_out) {
+  void *d = nullptr;
+  if (hipMalloc(&d, 256 * sizeof(float)) != hipSuccess) return -1;
+  float tmp[256];
+  probe_f16x32<<<1, 64>>>((float *)d, chain);
+  if (hipMemcpy(tmp, d, 256 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return -2;
+  for (int i = 0; i < 256; i++) host_out[i] = tmp[i];
+  if (hipDeviceSynchronize() != hipSuccess) return -3;
+  (void)hipFree(d);
+  return 0;
+}
+
 extern "C" int mfma_probe(int f64, int chain, double *host_out) {
   void *d = nullptr;
   if (hipMalloc(&d, 256 * sizeof(double)) != hipSuccess) return -1;

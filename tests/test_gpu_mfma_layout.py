@@ -31,3 +31,24 @@ def test_mfma_16x16x4_layout(f64, chain):
         for l in range(64):
             print(l, [(int(x // 1000), int(x % 1000)) for x in v[l]])
     assert np.array_equal(got, expect)
+
+
+@pytest.mark.parametrize("chain", [1, 3])
+def test_mfma_16x16x32_f16_layout(chain):
+    """FP16 inputs, FP32 accumulation: lane l supplies A[l%16][8(l//16)+i]
+    and B[8(l//16)+i][l%16]; it holds D rows 4(l//16)+r of column l%16, as
+    the FP32 16x16x4 product."""
+    print(f"case mfma layout f16x32 chain={chain}", flush=True)
+    L = C.CDLL(LIB)
+    L.mfma_probe_f16x32.argtypes = [C.c_int, np.ctypeslib.ndpointer(np.float64, flags="C")]
+    out = np.zeros(256)
+    assert L.mfma_probe_f16x32(chain, out) == 0
+    got = out.reshape(64, 4)
+    lane = np.arange(64)[:, None]
+    r = np.arange(4)[None, :]
+    expect = chain * (1000.0 * (4 * (lane // 16) + r) + lane % 16)
+    if not np.array_equal(got, expect):
+        v = got / chain
+        for l in range(64):
+            print(l, [(int(x // 1000), int(x % 1000)) for x in v[l]])
+    assert np.array_equal(got, expect)
