@@ -170,7 +170,8 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 contraction, fp64 geometry + accumulation",
+        "dtype": ("fp16x2-split contraction (hi+lo parts, fp32 accumulate) on fp16 MFMA"
+                  if order == 2 else "fp32 contraction") + ", fp64 geometry + accumulation",
         "data": "synthetic",
         "config": {"workload": f"3D Laplace, {n}^3 Q{order} hex cells on the unit cube, {pdesc}"
                                f"{nq}-point Gauss, {kdesc}",
