@@ -125,6 +125,9 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
 // CSR view of a square, structurally symmetric pattern: perm[k] is the CSC
 // position of the k-th entry in row-major order (row i's entries by
 // ascending column), so the rows reuse colptr/rowval as pointers/indices.
+// The solve builds the same permutation on the device
+// (gtk_solve.hip, transpose_perm_kernel); this host form is the one the
+// host tests pin against a sparse transpose.
 std::string build_transpose_perm(const Pattern &p, std::vector<int32_t> &perm);
 // Row offsets of every contributor of a tile plan over the columns of `p`.
 std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t, OffsetPlan &out, int nthreads);
