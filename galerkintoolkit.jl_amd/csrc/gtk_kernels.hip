@@ -641,7 +641,7 @@ struct BatchTile {
           h2 += e * e;
         }
         const double h = sqrt(h2);
-        const double mag = l.K[sl] * (FORM == kLaplace ? (D == 3 ? h : 1.0) : (D == 3 ? h * h * h : h * h));
+        const double mag = fabs(l.K[sl]) * (FORM == kLaplace ? (D == 3 ? h : 1.0) : (D == 3 ? h * h * h : h * h));
         int e = 0;
         if (mag > 0.0 && mag < 1e300) e = min(100, max(-100, kScaleExp - ilogb(mag)));
         kdV = ldexp(kdV, e);

@@ -69,6 +69,27 @@ def test_laplace_matrix_mixed(GT, O, cells, order, degree, diri, perturb):
     assert relfro(A.nzval, nz) <= TOL_MIXED
 
 
+@pytest.mark.parametrize("cells,order,degree,diri,perturb", [CASES[1], CASES[3], CASES[4], CASES[5]])
+@pytest.mark.parametrize("spread", [0, 6])
+def test_mixed_kappa_and_mass(GT, O, cells, order, degree, diri, perturb, spread):
+    """Mixed form with a per-cell coefficient, lognormal times 10^U(-s, s)
+    (s = 6 spans twelve decades, so the per-cell scaling of the FP16 split
+    product on Q2 hexahedra sees very different cells side by side), for the
+    Laplace and the mass matrix."""
+    print(f"case mixed kappa/mass cells={cells} order={order} spread={spread}", flush=True)
+    m, Om, V, p = build(GT, O, cells, order, diri=diri, perturb=perturb)
+    dO = GT.measure(Om, degree)
+    p.quadrature(degree)
+    ncell = m.cells.shape[0]
+    k = GT.inputs.lognormal(13, ncell) * 10.0 ** (spread * (2 * GT.inputs.uniform(14, ncell) - 1))
+    for code, form in ((O.LAPLACE, GT.Laplace(dO, GT.face_constant_field(k, Om))),
+                       (O.MASS, GT.Mass(dO, GT.face_constant_field(k, Om)))):
+        A = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+        cp, rv, nz = p.assemble_matrix(code, coef=k)
+        assert np.array_equal(A.colptr, cp) and np.array_equal(A.rowval, rv)
+        assert relfro(A.nzval, nz) <= TOL_MIXED, code
+
+
 @pytest.mark.parametrize("cells,order,degree,diri,perturb", CASES[:5])
 def test_mass_and_load(GT, O, cells, order, degree, diri, perturb):
     print(f"case mass/load cells={cells} order={order}", flush=True)
