@@ -3,7 +3,8 @@ that hold at any size (the oracle is too slow there; the same kernels are
 pinned against it on small meshes in test_gpu_parity.py):
 
 - pattern sizes in closed form, nnz = prod((k^2 + 2k) n + 1) on an n^3 hex
-  mesh of order k (test/assembly_tests.jl:72-73 counts the same way);
+  mesh of order k (test/assembly_tests.jl:72-73 counts the same way), and
+  230 n^3 + 138 n^2 + 24 n + 1 per component pair for P2 tets;
 - zero row sums of the Laplace matrix without Dirichlet rows (constants are
   in its kernel) and its symmetry;
 - total mass and total load (f = 1) equal to the volume of the unit cube
@@ -86,6 +87,9 @@ def test_p2_tet_elasticity_48(GT):
     mu = 0.5 + 1.5 * GT.inputs.uniform(SEED + 1, nc)
     form = GT.Elasticity(dO, GT.face_constant_field(lam, Om), GT.face_constant_field(mu, Om))
     K = GT.assemble_matrix(form, float, V, V, precision=GT.FP64)
+    # scalar P2 tets: nnz = 230 n^3 + 138 n^2 + 24 n + 1 (the oracle gives it
+    # for n = 2..5); three components coupled pairwise: 9 times that
+    assert K.nzval.size == 9 * (230 * n ** 3 + 138 * n ** 2 + 24 * n + 1)
     S = K.to_scipy()
     scale = np.abs(K.nzval).max()
     xf, _ = dof_coordinates(GT, m, V)
