@@ -128,6 +128,16 @@ def test_dof_counts_and_nnz_closed_forms(O, cells, order):
     assert p.q(O.Q_NFREE) == np.prod([order * n - 1 for n in cells])
 
 
+def test_p2_tet_nnz_closed_form(O):
+    """P2 on the 6-tet split of n^3 cubes: nnz = 230 n^3 + 138 n^2 + 24 n + 1
+    at five sizes (a cubic through four points, checked at a fifth)."""
+    for n in (2, 3, 4, 5, 6):
+        p = O.Problem([n, n, n], simplexify=True).space(2)
+        p.quadrature(2)
+        cp, rv, nz = p.assemble_matrix(O.MASS)
+        assert len(rv) == 230 * n ** 3 + 138 * n ** 2 + 24 * n + 1, n
+
+
 def test_q1_numbering_closed_form_2d(O):
     # free id of interior node (i, j), 2 <= i, j <= n: (j-2)(n-1) + (i-1)
     n = 64
