@@ -503,14 +503,16 @@ __device__ inline void load_batch(const unsigned char *__restrict__ rec, const M
 //   Bt[(q,d), a] = d_d phi_a(q)                    (reference table, registers)
 // with the k dimension ordered so that lane group g = lane/16 covers the
 // quadrature points g*QG .. g*QG+QG-1 with all their directions: each lane
-// reads G and grad phi_b once per point with vector LDS loads.  Lane
+// reads G and grad phi_b once per point with vector LDS loads.  The product
+// runs on 16x16x4 FP64 or FP32 matrix cores, or (mixed form, long K) on
+// 16x16x32 FP16 ones with W and Bt split into two FP16 parts each.  Lane
 // (nl, g) ends up holding local row a = nl (+16) of slots 4g .. 4g+3.  The
 // rows are added into FP64 column accumulators in LDS through the record's
-// row offsets in rank steps t: step t adds, for each r, the rows of the
-// slots 4g+r whose rank within their column is t.  Slots of one step belong
-// to distinct columns, so no two lanes of one addition touch one address,
-// and every nonzero is summed in contributor order (cells ascending); an
-// addition no lane needs is skipped.  The accumulators of a batch are then
+// row offsets in steps (h, r): step (h, r) adds the rows of the slots 4g+r
+// whose rank within their column lies in [4h, 4h+4).  Slots of one step
+// belong to distinct columns, so no two lanes of one addition touch one
+// address, and every nonzero is summed in contributor order (cells
+// ascending); a step no lane needs is skipped.  The accumulators of a batch are then
 // written flat and coalesced: entry e goes to e + delta[c], c = number of
 // column ends before e (an end mask per batch), and cleared.  The record of
 // the next batch is loaded while the current one is computed.
