@@ -223,6 +223,9 @@ def bench_elasticity(GT, args, n):
     # the FP32 assembly is the one solved
     A = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
     x, st = GT.solve(A, b, precision=GT.MIXED, rtol=1e-10)
+    nd = 30
+    bytes_alg = alg_bytes(3, mesh.coords.shape[0], cells, 4, nd, 2, A.nnz)
+    achieved = bytes_alg / (kernmx * 1e-3) / 1e9
     return {
         "metric": "cells/s, P2 tet elasticity FP32 assembly + mixed-precision refined solve",
         "value": cells / (wallmx * 1e-3),
@@ -240,8 +243,12 @@ def bench_elasticity(GT, args, n):
                                f"U[0.5,2] per cell (seed {args.seed}), clamped on x=0, body force (0,0,-1)",
                    "name": "p2tet", "cells": cells, "dofs": int(V.nfree), "nnz": int(A.nnz),
                    "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "algorithmic bytes of the mixed assembly over its kernel time; the kernel is VALU-bound"},
         "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3)},
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err, "speedup_vs_fp64": kern64 / kernmx},
+        "bytes_alg_per_launch": bytes_alg,
         "solve": {"rel_residual": st.rel_residual, "outer_iterations": st.outer_iterations,
                   "inner_iterations": st.inner_iterations, "ms": st.solve_ms},
         "setup_s": setup_s,

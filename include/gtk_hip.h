@@ -55,7 +55,11 @@ typedef struct gtk_ctx gtk_ctx;
 /* arithmetic of the per-cell contraction */
 #define GTK_FP64 0   /* everything in FP64: the plain drop-in */
 #define GTK_MIXED 1  /* FP64 geometry (J, J^-1, det J) and FP64 cross-cell
-                        accumulation; per-cell contraction in FP32 */
+                        accumulation; per-cell contraction in FP32, or on
+                        FP16 matrix cores with each operand split into two
+                        FP16 parts (22-bit operands, FP32 accumulation) where
+                        the contraction is long (Q2 hexahedra); values within
+                        1e-6 (relative Frobenius) of GTK_FP64 */
 
 /* load-vector source kinds */
 #define GTK_F_CONST 0  /* one value per component */
