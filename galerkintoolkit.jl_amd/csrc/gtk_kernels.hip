@@ -136,6 +136,14 @@ struct CellGeo {
   static constexpr bool value = FORM == kElasticity && NV == D + 1 && NL == 0;
 };
 
+// values per staged factor set of the VALU kernel (CellGeo: J^-1, |det J|,
+// lambda, mu and the symmetric M = J^-1 J^-T)
+template <int D, int FORM, int NL, int NV>
+struct CellGeoStride {
+  static constexpr int value =
+      NGeo<D, FORM>::value + (CellGeo<D, NV, FORM, NL>::value ? D * (D + 1) / 2 : 0);
+};
+
 template <int D>
 __device__ inline int sym_index(int i, int j) {
   // packed symmetric storage: (0,0) (0,1) (0,2) (1,1) (1,2) (2,2) / (0,0) (0,1) (1,1)
@@ -162,16 +170,21 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   constexpr int CPW = 64 / LPC;
   constexpr int NWAVES = kBlock / 64;
   constexpr int NT = NL > 0 ? NL : 1;            // table sets
-  constexpr int NGPHI = NL > 0 ? 0 : NQ * D * NS;  // face mode: no gradient table
   // affine cells (simplices with order-1 geometry) have one J at every
-  // point: elasticity keeps one factor set per cell (J^-1, |det J|, lambda,
-  // mu) and takes w_q in the point loop
+  // point, so elasticity needs no point loop: with P = J^-T R_ab J^-1 and
+  // R_ab = sum_q w_q grad phi_a(q) grad phi_b(q)^T (reference integrals per
+  // node pair, in LDS), the point sums are
+  //   sum_q w_q (g_a)_i (g_b)_j = P_ij,  sum_q w_q g_a . g_b = tr P
+  // and one factor set per cell is staged: J^-1, |det J|, lambda, mu and
+  // M = J^-1 J^-T (tr P = sum R_ab o M)
   constexpr bool kCellGeo = CellGeo<D, NV, FORM, NL>::value;
   constexpr int GQ = kCellGeo ? 1 : NQ;  // factor sets per cell
+  constexpr int NGS = CellGeoStride<D, FORM, NL, NV>::value;  // values per factor set
+  constexpr int NGPHI = NL > 0 ? 0 : (kCellGeo ? NS * NS * D * D : NQ * D * NS);  // gradient table, or R
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NQ][D][NS]
+  CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NQ][D][NS], or R [NS][NS][D][D]
   CT *phi_s = gphi_s + NGPHI;                      // [NT][NQ][NS]
-  CT *G_s = phi_s + NT * NQ * NS;                  // [cells][GQ][NG]
+  CT *G_s = phi_s + NT * NQ * NS;                  // [cells][GQ][NGS]
   double *acc_base = reinterpret_cast<double *>(lds_raw + args.acc_offset);
 
   const int tile = blockIdx.x;
@@ -179,9 +192,19 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   const int ncell = args.tile_cell_ptr[tile + 1] - cell0;
   const int col0 = args.tile_col_ptr[tile];
   const int ncol = args.tile_col_ptr[tile + 1] - col0;
-  int32_t *lf_s = reinterpret_cast<int32_t *>(G_s + ncell * GQ * NG);  // [cells] local face (NL > 0)
+  int32_t *lf_s = reinterpret_cast<int32_t *>(G_s + ncell * GQ * NGS);  // [cells] local face (NL > 0)
 
-  for (int i = threadIdx.x; i < NGPHI; i += kBlock) gphi_s[i] = (CT)args.gphi[i];
+  if constexpr (kCellGeo) {
+    // R[a][b][m][n] = sum_q w_q d_m phi_a(q) d_n phi_b(q), in FP64
+    for (int i = threadIdx.x; i < NGPHI; i += kBlock) {
+      const int ab = i / (D * D), mn = i - ab * D * D, ia = ab / NS, ib = ab - ia * NS, m = mn / D, n = mn - m * D;
+      double r = 0.0;
+      for (int q = 0; q < NQ; q++) r = fma(args.w[q] * args.gphi[(q * D + m) * NS + ia], args.gphi[(q * D + n) * NS + ib], r);
+      gphi_s[i] = (CT)r;
+    }
+  } else {
+    for (int i = threadIdx.x; i < NGPHI; i += kBlock) gphi_s[i] = (CT)args.gphi[i];
+  }
   for (int i = threadIdx.x; i < NT * NQ * NS; i += kBlock) phi_s[i] = (CT)args.phi[i];
   if constexpr (NL > 0)
     for (int i = threadIdx.x; i < ncell; i += kBlock) lf_s[i] = args.item_face[args.tile_cells[cell0 + i]];
@@ -201,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
       dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji);
       if constexpr (!kCellGeo) dV *= args.w[q];
     }
-    CT *G = G_s + (s * GQ + q) * NG;
+    CT *G = G_s + (s * GQ + q) * NGS;
     if constexpr (FORM == kLaplace) {
       const double k = args.coef ? args.coef[cell] : 1.0;
       const double kdV = k * dV;
@@ -226,6 +249,17 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
       G[D * D] = (CT)dV;
       G[D * D + 1] = (CT)args.coef[2 * cell];
       G[D * D + 2] = (CT)args.coef[2 * cell + 1];
+      if constexpr (kCellGeo) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+          for (int j = i; j < D; j++) {
+            double m = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) m = fma(Ji[i][k], Ji[j][k], m);
+            G[D * D + 3 + sym_index<D>(i, j)] = (CT)m;
+          }
+      }
     }
   }
   __syncthreads();
@@ -237,11 +271,6 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
   double *acc = acc_base + (size_t)(wave * CPW + (active ? cs : 0)) * args.max_seg;
   const int an = RPL > 1 ? a : a / NC;
   const int ac = RPL > 1 ? 0 : a - an * NC;  // component of row a (one row per lane)
-  CT wq[kCellGeo ? NQ : 1];
-  if constexpr (kCellGeo) {
-#pragma unroll
-    for (int q = 0; q < NQ; q++) wq[q] = (CT)args.w[q];
-  }
   for (int base = wave * CPW; base < ncol; base += NWAVES * CPW) {
     const int ci = base + cs;
     if (!active || ci >= ncol) continue;
@@ -254,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
     for (int k = args.col_ctr_ptr[pc]; k < k1; k++) {
       const int s = args.ctr_slot[k], b = args.ctr_b[k];
       const int bn = b / NC, bc = b - bn * NC;
-      const CT *Gs = G_s + s * GQ * NG;
+      const CT *Gs = G_s + s * GQ * NGS;
       CT v[RPL];  // rows an * NC + r of the lane (RPL > 1), or row a
 #pragma unroll
       for (int r = 0; r < RPL; r++) v[r] = 0;
@@ -283,35 +312,70 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
       } else {
         // sigma(eps(phi_b e_bc)) : eps(phi_a e_ac)
         //   = lambda (g_b)_bc (g_a)_ac + mu (delta_{ac,bc} g_a.g_b + (g_a)_bc (g_b)_ac)
-        CT Gc[kCellGeo ? NG : 1];  // the cell's factors, read once (kCellGeo)
         if constexpr (kCellGeo) {
+          CT Gc[NGS], R[D * D];  // the cell's factors; R_ab of this lane's node and the column's node
 #pragma unroll
-          for (int i = 0; i < NG; i++) Gc[i] = Gs[i];
-        }
+          for (int i = 0; i < NGS; i++) Gc[i] = Gs[i];
+          const CT *Rab = gphi_s + (an * NS + bn) * D * D;
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-          const CT *G = kCellGeo ? Gc : Gs + q * NG;
-          CT ga[D], gb[D];
+          for (int i = 0; i < D * D; i++) R[i] = Rab[i];
+          CT jb[D];  // column bc of J^-1
 #pragma unroll
-          for (int i = 0; i < D; i++) {
-            CT sa = 0, sb = 0;
+          for (int m = 0; m < D; m++) jb[m] = bc == 0 ? Gc[m * D] : (bc == 1 ? Gc[m * D + 1] : Gc[m * D + D - 1]);
+          CT u[D], w[D];  // R jb and R^T jb
+#pragma unroll
+          for (int m = 0; m < D; m++) {
+            CT su = 0, sw = 0;
+#pragma unroll
+            for (int n = 0; n < D; n++) {
+              su = fma(R[m * D + n], jb[n], su);
+              sw = fma(R[n * D + m], jb[n], sw);
+            }
+            u[m] = su;
+            w[m] = sw;
+          }
+          CT tr = 0;  // sum R o M, M symmetric
+#pragma unroll
+          for (int m = 0; m < D; m++)
+#pragma unroll
+            for (int n = 0; n < D; n++) tr = fma(R[m * D + n], Gc[D * D + 3 + sym_index<D>(m, n)], tr);
+          const CT det = Gc[D * D], lam = Gc[D * D + 1], mu = Gc[D * D + 2];
+#pragma unroll
+          for (int r = 0; r < RPL; r++) {
+            CT pc = 0, pr = 0;  // P[r][bc] and P[bc][r]
 #pragma unroll
             for (int m = 0; m < D; m++) {
-              sa = fma(G[m * D + i], gphi_s[(q * D + m) * NS + an], sa);
-              sb = fma(G[m * D + i], gphi_s[(q * D + m) * NS + bn], sb);
+              pc = fma(Gc[m * D + r], u[m], pc);
+              pr = fma(Gc[m * D + r], w[m], pr);
             }
-            ga[i] = sa;
-            gb[i] = sb;
+            v[r] = det * (lam * pc + mu * ((r == bc ? tr : (CT)0) + pr));
           }
-          CT dot = 0;
+        } else {
 #pragma unroll
-          for (int i = 0; i < D; i++) dot = fma(ga[i], gb[i], dot);
-          const CT lam = G[D * D + 1], mu = G[D * D + 2];
-          const CT dV = kCellGeo ? G[D * D] * wq[kCellGeo ? q : 0] : G[D * D];
+          for (int q = 0; q < NQ; q++) {
+            const CT *G = Gs + q * NGS;
+            CT ga[D], gb[D];
 #pragma unroll
-          for (int ac = 0; ac < RPL; ac++) {
-            CT t = lam * gb[bc] * ga[ac] + mu * ((ac == bc ? dot : (CT)0) + ga[bc] * gb[ac]);
-            v[ac] = fma(t, dV, v[ac]);
+            for (int i = 0; i < D; i++) {
+              CT sa = 0, sb = 0;
+#pragma unroll
+              for (int m = 0; m < D; m++) {
+                sa = fma(G[m * D + i], gphi_s[(q * D + m) * NS + an], sa);
+                sb = fma(G[m * D + i], gphi_s[(q * D + m) * NS + bn], sb);
+              }
+              ga[i] = sa;
+              gb[i] = sb;
+            }
+            CT dot = 0;
+#pragma unroll
+            for (int i = 0; i < D; i++) dot = fma(ga[i], gb[i], dot);
+            const CT lam = G[D * D + 1], mu = G[D * D + 2];
+            const CT dV = G[D * D];
+#pragma unroll
+            for (int ac = 0; ac < RPL; ac++) {
+              CT t = lam * gb[bc] * ga[ac] + mu * ((ac == bc ? dot : (CT)0) + ga[bc] * gb[ac]);
+              v[ac] = fma(t, dV, v[ac]);
+            }
           }
         }
       }
@@ -1264,9 +1328,12 @@ static KernelInfo matrix_info() {
   constexpr int NT = E::nl > 0 ? E::nl : 1;
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl>);
-  k.table_bytes = (size_t)((E::nl > 0 ? 0 : E::nq * E::d * E::ns) + NT * E::nq * E::ns) * sizeof(CT);
-  constexpr int GQ = CellGeo<E::d, E::nv, FORM, E::nl>::value ? 1 : E::nq;
-  k.geo_bytes_per_cell = (size_t)GQ * NGeo<E::d, FORM>::value * sizeof(CT) + (E::nl > 0 ? sizeof(int32_t) : 0);
+  constexpr bool cell_geo = CellGeo<E::d, E::nv, FORM, E::nl>::value;
+  constexpr int ngphi = E::nl > 0 ? 0 : (cell_geo ? E::ns * E::ns * E::d * E::d : E::nq * E::d * E::ns);
+  k.table_bytes = (size_t)(ngphi + NT * E::nq * E::ns) * sizeof(CT);
+  constexpr int GQ = cell_geo ? 1 : E::nq;
+  k.geo_bytes_per_cell = (size_t)GQ * CellGeoStride<E::d, FORM, E::nl, E::nv>::value * sizeof(CT) +
+                         (E::nl > 0 ? sizeof(int32_t) : 0);
   k.cols_per_wave = 64 / ValuLanes<E::ns, E::nc, FORM>::value;
   return k;
 }
