@@ -280,8 +280,26 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
     const int len = args.colptr[j + 1] - args.colptr[j];
     for (int r = a; r < len; r += LPC) acc[r] = 0.0;
     const int k1 = args.col_ctr_ptr[pc + 1];
+    // contributor k+1's slot, local column and row offsets are loaded while
+    // contributor k is computed
+    const int none = args.off_bytes == 1 ? 0xFF : 0xFFFF;
+    auto fetch = [&](int k, int &s_, int &b_, int (&o_)[RPL]) {
+      s_ = args.ctr_slot[k];
+      b_ = args.ctr_b[k];
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int64_t e = (int64_t)k * args.off_stride + (RPL > 1 ? an * NC + r : a);
+        o_[r] = args.off_bytes == 1 ? (int)args.ctr_off[e] : (int)reinterpret_cast<const uint16_t *>(args.ctr_off)[e];
+      }
+    };
+    int s_n = 0, b_n = 0, o_n[RPL];
+    if (args.col_ctr_ptr[pc] < k1) fetch(args.col_ctr_ptr[pc], s_n, b_n, o_n);
     for (int k = args.col_ctr_ptr[pc]; k < k1; k++) {
-      const int s = args.ctr_slot[k], b = args.ctr_b[k];
+      const int s = s_n, b = b_n;
+      int off[RPL];
+#pragma unroll
+      for (int r = 0; r < RPL; r++) off[r] = o_n[r];
+      if (k + 1 < k1) fetch(k + 1, s_n, b_n, o_n);
       const int bn = b / NC, bc = b - bn * NC;
       const CT *Gs = G_s + s * GQ * NGS;
       CT v[RPL];  // rows an * NC + r of the lane (RPL > 1), or row a
@@ -342,13 +360,13 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
           const CT det = Gc[D * D], lam = Gc[D * D + 1], mu = Gc[D * D + 2];
 #pragma unroll
           for (int r = 0; r < RPL; r++) {
-            CT pc = 0, pr = 0;  // P[r][bc] and P[bc][r]
+            CT pcol = 0, prow = 0;  // P[r][bc] and P[bc][r]
 #pragma unroll
             for (int m = 0; m < D; m++) {
-              pc = fma(Gc[m * D + r], u[m], pc);
-              pr = fma(Gc[m * D + r], w[m], pr);
+              pcol = fma(Gc[m * D + r], u[m], pcol);
+              prow = fma(Gc[m * D + r], w[m], prow);
             }
-            v[r] = det * (lam * pc + mu * ((r == bc ? tr : (CT)0) + pr));
+            v[r] = det * (lam * pcol + mu * ((r == bc ? tr : (CT)0) + prow));
           }
         } else {
 #pragma unroll
@@ -384,12 +402,8 @@ __global__ __launch_bounds__(kBlock) void matrix_tile_kernel(MatrixArgs args) {
       if constexpr (FORM != kElasticity && NC > 1)
         if (ac != bc) v[0] = 0;
 #pragma unroll
-      for (int r = 0; r < RPL; r++) {
-        const int64_t e = (int64_t)k * args.off_stride + (RPL > 1 ? an * NC + r : a);
-        const int off = args.off_bytes == 1 ? (int)args.ctr_off[e]
-                                            : (int)reinterpret_cast<const uint16_t *>(args.ctr_off)[e];
-        if (off != (args.off_bytes == 1 ? 0xFF : 0xFFFF)) acc[off] += (double)v[r];
-      }
+      for (int r = 0; r < RPL; r++)
+        if (off[r] != none) acc[off[r]] += (double)v[r];
     }
     for (int r = a; r < len; r += LPC) args.nzval[seg0 + r] = acc[r];
   }
