@@ -245,7 +245,8 @@ def bench_elasticity(GT, args, n):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "note": "algorithmic bytes of the mixed assembly over its kernel time; the kernel is VALU-bound"},
+                     "note": "algorithmic bytes of the mixed assembly over its kernel time; the kernel is bound by the "
+                             "latency of its per-contributor loads, not by HBM bandwidth"},
         "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3)},
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err, "speedup_vs_fp64": kern64 / kernmx},
         "bytes_alg_per_launch": bytes_alg,
