@@ -9,6 +9,7 @@
 namespace gtk {
 
 constexpr int kBlock = 256;            // threads per workgroup (4 waves of 64)
+constexpr int kValuBlock = 512;        // threads per workgroup of the VALU matrix kernel (one per CU: 2 waves per SIMD)
 
 enum { kLaplace = 0, kMass = 1, kElasticity = 2 };
 enum { kFConst = 0, kFCell = 1, kFNodal = 2 };
