@@ -159,7 +159,8 @@ __device__ inline int sym_index(int i, int j) {
 // NL > 0: boundary items (face mode, mass only): the tables are per local
 // face, [NL][NQ][...], and the measure is the face's.
 template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT, int NL = 0>
-__global__ __launch_bounds__(kValuBlock) void matrix_tile_kernel(MatrixArgs args) {
+__global__ __launch_bounds__(ValuBlock<CT>::value) void matrix_tile_kernel(MatrixArgs args) {
+  constexpr int kVB = ValuBlock<CT>::value;
   static_assert(NL == 0 || FORM == kMass, "boundary items: mass form only");
   constexpr int NG = NGeo<D, FORM>::value;
   constexpr int NDOF = NS * NC;
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(kValuBlock) void matrix_tile_kernel(MatrixArgs args
   constexpr int LPC = ValuLanes<NS, NC, FORM>::value;
   constexpr int RPL = NDOF / LPC;  // rows per lane
   constexpr int CPW = 64 / LPC;
-  constexpr int NWAVES = kValuBlock / 64;
+  constexpr int NWAVES = kVB / 64;
   constexpr int NT = NL > 0 ? NL : 1;            // table sets
   // affine cells (simplices with order-1 geometry) have one J at every
   // point, so elasticity needs no point loop: with P = J^-T R_ab J^-1 and
@@ -196,21 +197,21 @@ __global__ __launch_bounds__(kValuBlock) void matrix_tile_kernel(MatrixArgs args
 
   if constexpr (kCellGeo) {
     // R[a][b][m][n] = sum_q w_q d_m phi_a(q) d_n phi_b(q), in FP64
-    for (int i = threadIdx.x; i < NGPHI; i += kValuBlock) {
+    for (int i = threadIdx.x; i < NGPHI; i += kVB) {
       const int ab = i / (D * D), mn = i - ab * D * D, ia = ab / NS, ib = ab - ia * NS, m = mn / D, n = mn - m * D;
       double r = 0.0;
       for (int q = 0; q < NQ; q++) r = fma(args.w[q] * args.gphi[(q * D + m) * NS + ia], args.gphi[(q * D + n) * NS + ib], r);
       gphi_s[i] = (CT)r;
     }
   } else {
-    for (int i = threadIdx.x; i < NGPHI; i += kValuBlock) gphi_s[i] = (CT)args.gphi[i];
+    for (int i = threadIdx.x; i < NGPHI; i += kVB) gphi_s[i] = (CT)args.gphi[i];
   }
-  for (int i = threadIdx.x; i < NT * NQ * NS; i += kValuBlock) phi_s[i] = (CT)args.phi[i];
+  for (int i = threadIdx.x; i < NT * NQ * NS; i += kVB) phi_s[i] = (CT)args.phi[i];
   if constexpr (NL > 0)
-    for (int i = threadIdx.x; i < ncell; i += kValuBlock) lf_s[i] = args.item_face[args.tile_cells[cell0 + i]];
+    for (int i = threadIdx.x; i < ncell; i += kVB) lf_s[i] = args.item_face[args.tile_cells[cell0 + i]];
 
   // phase A: FP64 geometry per (cell, point), or per cell (kCellGeo)
-  for (int idx = threadIdx.x; idx < ncell * GQ; idx += kValuBlock) {
+  for (int idx = threadIdx.x; idx < ncell * GQ; idx += kVB) {
     const int s = idx / GQ, q = idx - s * GQ;
     const int64_t cell = args.tile_cells[cell0 + s];
     double Ji[D][D];
@@ -1349,7 +1350,7 @@ static KernelInfo matrix_info() {
   k.geo_bytes_per_cell = (size_t)GQ * CellGeoStride<E::d, FORM, E::nl, E::nv>::value * sizeof(CT) +
                          (E::nl > 0 ? sizeof(int32_t) : 0);
   k.cols_per_wave = 64 / ValuLanes<E::ns, E::nc, FORM>::value;
-  k.block = kValuBlock;
+  k.block = ValuBlock<CT>::value;
   return k;
 }
 
