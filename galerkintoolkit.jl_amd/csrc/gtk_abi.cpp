@@ -699,10 +699,21 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
     ep->lds_claim = (int)(ep->acc_offset + align16(stage_bytes));
     ep->lds = ep->lds_claim + 16;
   } else {
-    if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, waves, ep->k))
+    if (!gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, 0, ep->k))
       FAIL(ctx, GTK_EUNSUP, "no kernel for this (element, quadrature, form, precision)");
-    const size_t waves = (size_t)ep->k.block / 64;
-    const size_t acc_bytes = waves * ep->k.cols_per_wave * std::max(1, plan.max_seg) * sizeof(double);
+    auto acc_of = [&](const gtk::KernelInfo &k) {
+      return (size_t)k.block / 64 * k.cols_per_wave * std::max(1, plan.max_seg) * sizeof(double);
+    };
+    // long columns: the preferred workgroup's accumulators would leave no
+    // room for a tile of 8 cells; then the 4-wave workgroup
+    // (GTK_VALU_WAVES=4 forces it: tests)
+    const char *vw = std::getenv("GTK_VALU_WAVES");
+    if ((vw && std::atoi(vw) == 4) ||
+        align16(ep->k.table_bytes) + acc_of(ep->k) + 1024 + 8 * ep->k.geo_bytes_per_cell > kLdsBudget) {
+      gtk::KernelInfo k4;
+      if (gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, 4, k4)) ep->k = k4;
+    }
+    const size_t acc_bytes = acc_of(ep->k);
     const int max_cells = tile_cells_for(ep->k, acc_bytes, kLdsBudget);
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);

@@ -158,9 +158,11 @@ __device__ inline int sym_index(int i, int j) {
 
 // NL > 0: boundary items (face mode, mass only): the tables are per local
 // face, [NL][NQ][...], and the measure is the face's.
-template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT, int NL = 0>
-__global__ __launch_bounds__(ValuBlock<CT>::value) void matrix_tile_kernel(MatrixArgs args) {
-  constexpr int kVB = ValuBlock<CT>::value;
+// BS: threads per workgroup (ValuBlock<CT>, or 256 where the accumulators of
+// that many waves would leave no LDS for a tile)
+template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT, int NL = 0, int BS = ValuBlock<CT>::value>
+__global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
+  constexpr int kVB = BS;
   static_assert(NL == 0 || FORM == kMass, "boundary items: mass form only");
   constexpr int NG = NGeo<D, FORM>::value;
   constexpr int NDOF = NS * NC;
@@ -1338,11 +1340,11 @@ static KernelInfo matrix_batch_info(int off_bytes, int waves) {
 
 // LDS sizes below follow the kernels' layouts (the host's tile limits use
 // the same formulas)
-template <typename E, int FORM, typename CT>
-static KernelInfo matrix_info() {
+template <typename E, int FORM, typename CT, int BS>
+static KernelInfo matrix_info_bs() {
   constexpr int NT = E::nl > 0 ? E::nl : 1;
   KernelInfo k;
-  k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl>);
+  k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl, BS>);
   constexpr bool cell_geo = CellGeo<E::d, E::nv, FORM, E::nl>::value;
   constexpr int ngphi = E::nl > 0 ? 0 : (cell_geo ? E::ns * E::ns * E::d * E::d : E::nq * E::d * E::ns);
   k.table_bytes = (size_t)(ngphi + NT * E::nq * E::ns) * sizeof(CT);
@@ -1350,8 +1352,14 @@ static KernelInfo matrix_info() {
   k.geo_bytes_per_cell = (size_t)GQ * CellGeoStride<E::d, FORM, E::nl, E::nv>::value * sizeof(CT) +
                          (E::nl > 0 ? sizeof(int32_t) : 0);
   k.cols_per_wave = 64 / ValuLanes<E::ns, E::nc, FORM>::value;
-  k.block = ValuBlock<CT>::value;
+  k.block = BS;
   return k;
+}
+
+// waves == 4: the 256-thread workgroup (fallback for long columns)
+template <typename E, int FORM, typename CT>
+static KernelInfo matrix_info(int waves = 0) {
+  return waves == 4 ? matrix_info_bs<E, FORM, CT, 256>() : matrix_info_bs<E, FORM, CT, ValuBlock<CT>::value>();
 }
 
 template <typename E, int FK>
@@ -1380,7 +1388,7 @@ static bool try_matrix(const ElemKey &key, int form, int precision, int off_byte
   if constexpr (E::nl > 0) {
     // boundary items: the mass form on the VALU kernel
     if (form != kMass) return false;
-    out = precision == 0 ? matrix_info<E, kMass, double>() : matrix_info<E, kMass, float>();
+    out = precision == 0 ? matrix_info<E, kMass, double>(waves) : matrix_info<E, kMass, float>(waves);
   } else if constexpr (E::nc == 1) {
     if (form != kLaplace && form != kMass) return false;
     if (allow_mfma) {
@@ -1392,13 +1400,13 @@ static bool try_matrix(const ElemKey &key, int form, int precision, int off_byte
                              : matrix_batch_info<E, kMass, float>(off_bytes, waves);
     } else {
       if (form == kLaplace)
-        out = precision == 0 ? matrix_info<E, kLaplace, double>() : matrix_info<E, kLaplace, float>();
+        out = precision == 0 ? matrix_info<E, kLaplace, double>(waves) : matrix_info<E, kLaplace, float>(waves);
       else
-        out = precision == 0 ? matrix_info<E, kMass, double>() : matrix_info<E, kMass, float>();
+        out = precision == 0 ? matrix_info<E, kMass, double>(waves) : matrix_info<E, kMass, float>(waves);
     }
   } else {
     if (form != kElasticity || E::nc != E::d) return false;
-    out = precision == 0 ? matrix_info<E, kElasticity, double>() : matrix_info<E, kElasticity, float>();
+    out = precision == 0 ? matrix_info<E, kElasticity, double>(waves) : matrix_info<E, kElasticity, float>(waves);
   }
   return true;
 }

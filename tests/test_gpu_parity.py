@@ -116,13 +116,16 @@ def test_dirichlet_columns(GT, O):
     assert relfro(Ad.nzval, nz) <= TOL_FP64
 
 
-@pytest.mark.parametrize("cells", [(3, 2, 3), (8, 6, 7)])
+@pytest.mark.parametrize("cells,valu_waves", [((3, 2, 3), None), ((8, 6, 7), None), ((8, 6, 7), "4")])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("mixed", TOL_MIXED)])
-def test_elasticity_p2_tets(GT, O, precision, tol, cells):
+def test_elasticity_p2_tets(GT, O, precision, tol, cells, valu_waves, monkeypatch):
     """P2 vector tetrahedra, per-cell Lame parameters, clamped on x = 0
     (group 2-face-5), body force (0, 0, -1); the larger mesh spans several
-    tiles."""
-    print(f"case elasticity p2 tets {precision} {cells}", flush=True)
+    tiles; GTK_VALU_WAVES=4 runs the 4-wave workgroup the library falls back
+    to when long columns leave no LDS for the preferred one."""
+    if valu_waves:
+        monkeypatch.setenv("GTK_VALU_WAVES", valu_waves)
+    print(f"case elasticity p2 tets {precision} {cells} valu_waves={valu_waves}", flush=True)
     m, Om, V, p = build(GT, O, cells, 2, simplex=True, diri=2, group=5, ncomp=3, perturb=0.1)
     dO = GT.measure(Om, 5)
     p.quadrature(5)
