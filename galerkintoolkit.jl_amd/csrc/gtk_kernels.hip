@@ -1309,8 +1309,6 @@ struct Elem {
 
 template <typename E, int FORM, typename CT, typename OffT, int NW>
 static KernelInfo matrix_batch_info() {
-  constexpr int NG = NGeo<E::d, FORM>::value;
-  constexpr int NGP = NG == 3 ? 4 : NG;
   constexpr int TP = FORM == kLaplace ? (E::d == 3 ? 4 : 2) : 1;
   constexpr int NQP = 4 * ((E::nq + 3) / 4);
   KernelInfo k;
