@@ -404,9 +404,11 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
       // (u . v, grad u : grad v); the pattern holds every component pair
       if constexpr (FORM != kElasticity && NC > 1)
         if (ac != bc) v[0] = 0;
+      // the lanes of a column add to distinct rows, so an LDS add with no
+      // return (in program order per lane) replaces the read-modify-write
 #pragma unroll
       for (int r = 0; r < RPL; r++)
-        if (off[r] != none) acc[off[r]] += (double)v[r];
+        if (off[r] != none) atomicAdd(&acc[off[r]], (double)v[r]);
     }
     for (int r = a; r < len; r += LPC) args.nzval[seg0 + r] = acc[r];
   }
