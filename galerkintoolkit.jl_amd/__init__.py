@@ -6,8 +6,12 @@ GT.assemble_matrix/assemble_vector, GT.update_matrix!) whose numeric work runs
 in libgtk_hip.so (HIP kernels for gfx950 behind the C ABI of include/gtk_hip.h).
 """
 from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, FormSum, Laplace, Load, Mass, Measure,
-                       SparseMatrixCSC, assemble_matrix, assemble_matrix_with_free_and_dirichlet_columns,
-                       assemble_vector, face_constant_field, linear_problem, measure, solve, update_matrix)
+                       MatrixCache, SparseMatrixCSC, VectorCache, assemble_matrix, assemble_matrix_and_vector,
+                       assemble_matrix_and_vector_with_free_and_dirichlet_columns,
+                       assemble_matrix_with_free_and_dirichlet_columns, assemble_vector, face_constant_field,
+                       linear_problem, measure, solve, update_matrix, update_matrix_and_vector,
+                       update_matrix_and_vector_with_free_and_dirichlet_columns,
+                       update_matrix_with_free_and_dirichlet_columns, update_vector)
 from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
 from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space
 from .batch import MatrixBatch, assemble_matrices, autotune
@@ -18,6 +22,10 @@ __all__ = [
     "Load",
     "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
     "assemble_vector", "face_constant_field", "linear_problem", "measure", "solve", "update_matrix", "Mesh",
+    "MatrixCache", "VectorCache", "assemble_matrix_and_vector",
+    "assemble_matrix_and_vector_with_free_and_dirichlet_columns", "update_matrix_and_vector",
+    "update_matrix_and_vector_with_free_and_dirichlet_columns", "update_matrix_with_free_and_dirichlet_columns",
+    "update_vector",
     "cartesian_mesh", "perturb_interior_nodes", "MatrixBatch", "assemble_matrices", "autotune", "Domain",
     "LagrangeSpace", "boundary", "interior", "lagrange_space", "inputs", "refcell",
 ]

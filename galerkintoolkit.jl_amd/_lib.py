@@ -22,7 +22,8 @@ EXPORTS = [
     "gtk_set_space", "gtk_set_reference", "gtk_set_reference_faces", "gtk_build_pattern", "gtk_build_pattern_device",
     "gtk_local_nnz_map", "gtk_add_block",
     "gtk_get_pattern", "gtk_set_cell_coefficients",
-    "gtk_assemble_matrix", "gtk_get_values", "gtk_values_device_ptr", "gtk_assemble_vector",
+    "gtk_assemble_matrix", "gtk_get_values", "gtk_set_values", "gtk_values_device_ptr", "gtk_assemble_vector",
+    "gtk_plan_info", "gtk_lift_dirichlet",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
     "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_waves", "gtk_batch_set_load", "gtk_batch_get_vector",
     "gtk_batch_build", "gtk_batch_run",
@@ -37,6 +38,17 @@ class GtkStats(C.Structure):
 class GtkSolveStats(C.Structure):
     _fields_ = [("outer_iterations", C.c_int), ("inner_iterations", C.c_int), ("converged", C.c_int),
                 ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
+
+
+class GtkPlanInfo(C.Structure):
+    _fields_ = [("matrix_cores", C.c_int), ("threads", C.c_int), ("grid", C.c_int), ("tiles", C.c_int),
+                ("max_tile_cells", C.c_int), ("stage_cells", C.c_int), ("lds_bytes", C.c_int), ("pad", C.c_int),
+                ("batches", C.c_int64), ("tile_cells", C.c_int64), ("cells", C.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        d["halo_redundancy"] = self.tile_cells / max(1, self.cells)
+        return d
 
 
 class GtkError(RuntimeError):
@@ -77,6 +89,9 @@ def load():
     L.gtk_set_cell_coefficients.argtypes = [vp, i, C.c_void_p]
     L.gtk_assemble_matrix.argtypes = [vp, i, i, i, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_get_values.argtypes = [vp, i, i, f64]
+    L.gtk_set_values.argtypes = [vp, i, i, f64]
+    L.gtk_lift_dirichlet.argtypes = [vp, i, f64, f64]
+    L.gtk_plan_info.argtypes = [vp, i, i, i, i, C.POINTER(GtkPlanInfo)]
     L.gtk_values_device_ptr.argtypes = [vp, i, i, C.POINTER(C.c_void_p)]
     L.gtk_assemble_vector.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_assemble_matrix_async.argtypes = [vp, i, i, i, i]
@@ -221,6 +236,21 @@ class Context:
         out = np.zeros(max(nnz, 1), np.float64)
         self.check(self.L.gtk_get_values(self.h, rows, cols, out), "gtk_get_values")
         return out[:nnz]
+
+    def set_values(self, rows, cols, nzval):
+        self.check(self.L.gtk_set_values(self.h, rows, cols, np.ascontiguousarray(nzval, np.float64)),
+                   "gtk_set_values")
+
+    def lift_dirichlet(self, xd, b, rows=GTK_FREE):
+        out = np.array(b, np.float64, copy=True)
+        self.check(self.L.gtk_lift_dirichlet(self.h, rows, np.ascontiguousarray(xd, np.float64), out),
+                   "gtk_lift_dirichlet")
+        return out
+
+    def plan_info(self, form, precision, rows, cols):
+        info = GtkPlanInfo()
+        self.check(self.L.gtk_plan_info(self.h, form, precision, rows, cols, C.byref(info)), "gtk_plan_info")
+        return info.as_dict()
 
     def values_device_ptr(self, rows, cols):
         p = C.c_void_p()

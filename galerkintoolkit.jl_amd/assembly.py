@@ -2,14 +2,17 @@
 
 Mirrors GT.measure (GalerkinToolkit.jl src/problems.jl:4-25),
 GT.assemble_matrix / GT.assemble_vector (src/problems.jl:244-274, :319-350),
-GT.update_matrix! (src/problems.jl:352-361) and
-assemble_matrix_with_free_and_dirichlet_columns (src/problems.jl:363-380).
+GT.update_vector! / GT.update_matrix! (src/problems.jl:276-285, :352-361),
+the *_with_free_and_dirichlet_columns and assemble_matrix_and_vector*
+families with their update_*! twins (src/problems.jl:363-435) and
+PartitionedSolvers_linear_problem (src/problems.jl:439-452).
 The reference takes a Julia lambda and compiles it; here the weak form is
 one of the integrands the library implements (Laplace with a per-cell
 coefficient, mass, isotropic elasticity, and the load f . v), passed as a
 form object.  The result is the reference's SparseMatrixCSC{Float64,Int32}
 layout (1-based colptr/rowval) and a Float64 vector.
 """
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -148,6 +151,7 @@ class _Assembler:
         _, gN = R.Lagrange(mesh.cellkind, 1).tabulate(dOmega.x)
         self.ctx.set_reference(dOmega.w, phi, gphi, gN)
         self.patterns = {}
+        self.values_owner = {}  # (rows, cols) -> token of the matrix whose values the device holds
 
     def pattern(self, rows, cols, symbolic="host"):
         """CSC pattern of the (rows, cols) block, built once: symbolic="host"
@@ -211,6 +215,7 @@ class _BoundaryAssembler(_Assembler):
         self.ctx.set_reference_faces(dGamma.w, np.stack(phi), np.stack(gN), np.stack(A), lface)
         self.items = len(faces)
         self.patterns = {}
+        self.values_owner = {}
 
 
 def _assembler(space, dOmega, device):
@@ -252,45 +257,120 @@ def _coefficients(form, ncells):
     return np.full(ncells, float(k)), 1
 
 
+def _with_parameters(form, parameters):
+    """The form with some of its coefficient fields replaced: the values the
+    reference passes as `parameters` to the generated loop
+    (src/problems.jl:352-361)."""
+    if not parameters:
+        return form
+    if isinstance(form, FormSum):
+        return FormSum([_with_parameters(t, parameters) for t in form.terms])
+    names = {f.name for f in dataclasses.fields(form)}
+    upd = {k: v for k, v in parameters.items() if k in names and k != "dOmega"}
+    return dataclasses.replace(form, **upd) if upd else form
+
+
+def _lock(A, a, rows, cols):
+    """Marks A as the owner of the device values of block (rows, cols) of
+    assembler a; A.nzval becomes read-only so that a change of the values
+    means a new array (seen by solve)."""
+    A.nzval.setflags(write=False)
+    A._token = object()
+    A._nzval_ref = A.nzval
+    A.device = (a, rows, cols)
+    a.values_owner[(rows, cols)] = A._token
+
+
+def _refill(A, nzval):
+    """Values of an existing A replaced in place (update_matrix!)."""
+    A.nzval.setflags(write=True)
+    A.nzval[:] = nzval
+    A.nzval.setflags(write=False)
+
+
+class MatrixCache:
+    """What update_matrix needs to refill a matrix: the form, its block and
+    precision (the reference keeps the generated loops, the allocation and
+    the compress cache, src/problems.jl:319-350)."""
+
+    def __init__(self, form, space, rows, cols, precision, device):
+        self.form, self.space, self.rows, self.cols = form, space, rows, cols
+        self.precision, self.device = precision, device
+
+
+class VectorCache:
+    """What update_vector needs to refill a vector (src/problems.jl:244-274)."""
+
+    def __init__(self, form, space, rows, device):
+        self.form, self.space, self.rows, self.device = form, space, rows, device
+
+
+def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic="host"):
+    """(assembler of the block's device values, nzval, stats, pattern)."""
+    if isinstance(form, FormSum):
+        return _assemble_matrix_sum(form, space, rows, cols, precision, device, symbolic)
+    if form.dOmega.on_boundary and not isinstance(form, Mass):
+        raise NotImplementedError("boundary integrals of matrices: the mass form (Robin terms)")
+    a = _assembler(space, form.dOmega, device)
+    pat = a.pattern(rows, cols, symbolic)
+    coef, ncoef = _coefficients(form, _ncoef_items(form, space))
+    a.ctx.set_cell_coefficients(coef, ncoef)
+    prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+    nzval, stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, pat[0])
+    return a, nzval, stats, pat
+
+
 def assemble_matrix(form, T, trial_space, test_space, free_or_dirichlet=(FREE, FREE), precision=FP64, reuse=False,
                     device=0, symbolic="host"):
-    """GT.assemble_matrix(a, Float64, U, V; free_or_dirichlet) on the GPU.
-    symbolic: where the pattern is built the first time ("host" or
-    "device")."""
+    """GT.assemble_matrix(a, Float64, U, V; free_or_dirichlet, reuse) on the
+    GPU (src/problems.jl:319-350).  symbolic: where the pattern is built the
+    first time ("host" or "device").  reuse=True returns (A, cache) for
+    update_matrix."""
     if T not in (float, np.float64):
         raise TypeError("values are Float64")
     if trial_space is not test_space:
         raise NotImplementedError("trial and test spaces must be the same space")
     space = test_space
-    if isinstance(form, FormSum):
-        if reuse:
-            raise NotImplementedError("reuse of a sum of integrals")
-        return _assemble_matrix_sum(form, T, space, free_or_dirichlet, precision, device, symbolic)
-    if form.dOmega.on_boundary and not isinstance(form, Mass):
-        raise NotImplementedError("boundary integrals of matrices: the mass form (Robin terms)")
-    a = _assembler(space, form.dOmega, device)
     rows, cols = free_or_dirichlet
-    nnz, ncols, nrows, colptr, rowval = a.pattern(rows, cols, symbolic)
-    coef, ncoef = _coefficients(form, _ncoef_items(form, space))
-    a.ctx.set_cell_coefficients(coef, ncoef)
-    prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
-    nzval, stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, nnz)
+    a, nzval, stats, pat = _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic)
+    nnz, ncols, nrows, colptr, rowval = pat
     A = SparseMatrixCSC(nrows, ncols, colptr, rowval, nzval)
     A.stats = stats
-    A.device = (a, rows, cols)
+    _lock(A, a, rows, cols)
     if reuse:
-        return A, (a, form, rows, cols, prec)
+        return A, MatrixCache(form, space, rows, cols, precision, device)
     return A
 
 
-def update_matrix(A, cache, form=None):
-    """GT.update_matrix!(A, cache): refill values with the same pattern
-    (new coefficients or coordinates); the pattern is not rebuilt."""
-    a, form0, rows, cols, prec = cache
-    form = form0 if form is None else form
-    coef, ncoef = _coefficients(form, _ncoef_items(form, a.space))
-    a.ctx.set_cell_coefficients(coef, ncoef)
-    A.nzval, A.stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, A.nnz)
+def _set_coordinates(form, space, device, coords):
+    """New node coordinates (same topology) for every context the form's
+    terms run on."""
+    terms = form.terms if isinstance(form, FormSum) else [form]
+    xyz = np.ascontiguousarray(coords, np.float64)
+    if xyz.shape != space.mesh.coords.shape:
+        raise ValueError("coordinates must keep the mesh's shape")
+    for t in terms:
+        _assembler(space, t.dOmega, device).ctx.update_coordinates(xyz)
+
+
+def update_matrix(A, cache, parameters=None, coords=None, form=None):
+    """GT.update_matrix!(A, cache; parameters) (src/problems.jl:352-361): the
+    values of A are recomputed in place with the same pattern.  parameters:
+    coefficient fields of the form to replace (e.g. {"kappa": field});
+    coords: new node coordinates of the mesh (same topology); form: a whole
+    replacement form of the same kind over the same measure."""
+    if form is not None:
+        cache.form = form
+    cache.form = _with_parameters(cache.form, parameters)
+    if coords is not None:
+        _set_coordinates(cache.form, cache.space, cache.device, coords)
+    a, nzval, stats, pat = _assemble_matrix_values(cache.form, cache.space, cache.rows, cache.cols, cache.precision,
+                                                   cache.device)
+    if pat[0] != A.nnz:
+        raise ValueError("the cache does not belong to this matrix")
+    _refill(A, nzval)
+    A.stats = stats
+    _lock(A, a, cache.rows, cache.cols)
     return A
 
 
@@ -306,7 +386,7 @@ def _load_data(form, space):
     return L.GTK_F_CONST, np.broadcast_to(np.asarray(f, np.float64), (space.ncomp,)).copy()
 
 
-def _assemble_matrix_sum(fs, T, space, free_or_dirichlet, precision, device, symbolic):
+def _assemble_matrix_sum(fs, space, rows, cols, precision, device, symbolic):
     """One volume integral plus boundary mass integrals: the volume block is
     assembled, then each boundary block is added into its values on the
     device (the boundary pattern lies inside the volume pattern)."""
@@ -314,49 +394,132 @@ def _assemble_matrix_sum(fs, T, space, free_or_dirichlet, precision, device, sym
     bnd = [t for t in fs.terms if t.dOmega.on_boundary]
     if len(vol) != 1:
         raise NotImplementedError("a sum of matrix integrals: one over the cells plus boundary integrals")
-    rows, cols = free_or_dirichlet
-    A = assemble_matrix(vol[0], T, space, space, free_or_dirichlet, precision=precision, device=device,
-                        symbolic=symbolic)
-    a_vol = A.device[0]
+    a_vol, _, stats, pat = _assemble_matrix_values(vol[0], space, rows, cols, precision, device, symbolic)
     for t in bnd:
-        Ab = assemble_matrix(t, T, space, space, free_or_dirichlet, precision=precision, device=device)
-        a_vol.ctx.add_block(rows, cols, Ab.device[0].ctx)
-    A.nzval = a_vol.ctx.get_values(rows, cols, A.nnz)
-    return A
+        a_b, _, _, _ = _assemble_matrix_values(t, space, rows, cols, precision, device)
+        a_vol.ctx.add_block(rows, cols, a_b.ctx)
+    nzval = a_vol.ctx.get_values(rows, cols, pat[0])
+    return a_vol, nzval, stats, pat
 
 
-def assemble_vector(form, T, space, free_or_dirichlet=FREE, device=0):
-    """GT.assemble_vector(l, Float64, V; free_or_dirichlet) on the GPU; l may
-    be a sum of loads over cells and boundary faces."""
+def _assemble_vector_values(form, space, rows, device):
     if isinstance(form, FormSum):
-        return sum(assemble_vector(t, T, space, free_or_dirichlet, device) for t in form.terms)
+        return sum(_assemble_vector_values(t, space, rows, device) for t in form.terms)
     kind, data = _load_data(form, space)
     a = _assembler(space, form.dOmega, device)
-    n = space.nfree if free_or_dirichlet == FREE else space.ndiri
-    b, stats = a.ctx.assemble_vector(kind, data, free_or_dirichlet, n)
+    n = space.nfree if rows == FREE else space.ndiri
+    b, stats = a.ctx.assemble_vector(kind, data, rows, n)
     return b
 
 
-def assemble_matrix_with_free_and_dirichlet_columns(form, T, U, V, precision=FP64, device=0):
-    A = assemble_matrix(form, T, U, V, (FREE, FREE), precision=precision, device=device)
-    Ad = assemble_matrix(form, T, U, V, (FREE, DIRICHLET), precision=precision, device=device)
+def assemble_vector(form, T, space, free_or_dirichlet=FREE, device=0, reuse=False):
+    """GT.assemble_vector(l, Float64, V; free_or_dirichlet, reuse) on the GPU
+    (src/problems.jl:244-274); l may be a sum of loads over cells and
+    boundary faces.  reuse=True returns (b, cache) for update_vector."""
+    if T not in (float, np.float64):
+        raise TypeError("values are Float64")
+    b = _assemble_vector_values(form, space, free_or_dirichlet, device)
+    if reuse:
+        return b, VectorCache(form, space, free_or_dirichlet, device)
+    return b
+
+
+def update_vector(b, cache, parameters=None, coords=None):
+    """GT.update_vector!(b, cache; parameters) (src/problems.jl:276-285): b
+    is recomputed in place; parameters replace fields of the load (e.g.
+    {"f": field}), coords the node coordinates (same topology)."""
+    cache.form = _with_parameters(cache.form, parameters)
+    if coords is not None:
+        _set_coordinates(cache.form, cache.space, cache.device, coords)
+    nb = _assemble_vector_values(cache.form, cache.space, cache.rows, cache.device)
+    if nb.shape != b.shape:
+        raise ValueError("the cache does not belong to this vector")
+    b[:] = nb
+    return b
+
+
+def assemble_matrix_with_free_and_dirichlet_columns(form, T, U, V, precision=FP64, device=0, reuse=False,
+                                                    free_or_dirichlet=FREE):
+    """(A, Ad): the (rows, FREE) and (rows, DIRICHLET) blocks
+    (src/problems.jl:363-380); with reuse=True also their cache."""
+    A, ca = assemble_matrix(form, T, U, V, (free_or_dirichlet, FREE), precision=precision, device=device, reuse=True)
+    Ad, cd = assemble_matrix(form, T, U, V, (free_or_dirichlet, DIRICHLET), precision=precision, device=device,
+                             reuse=True)
+    if reuse:
+        return A, Ad, (ca, cd)
     return A, Ad
+
+
+def update_matrix_with_free_and_dirichlet_columns(A, Ad, cache, parameters=None, coords=None):
+    """src/problems.jl:382-389."""
+    ca, cd = cache
+    update_matrix(A, ca, parameters, coords)
+    update_matrix(Ad, cd, parameters)
+    return A, Ad
+
+
+def assemble_matrix_and_vector(a, l, T, U, V, precision=FP64, device=0, reuse=False, free_or_dirichlet=(FREE, FREE)):
+    """(A, b) (src/problems.jl:391-404); with reuse=True also the cache."""
+    A, mc = assemble_matrix(a, T, U, V, free_or_dirichlet, precision=precision, device=device, reuse=True)
+    b, vc = assemble_vector(l, T, V, free_or_dirichlet[0], device=device, reuse=True)
+    if reuse:
+        return A, b, {"matrix_cache": mc, "vector_cache": vc}
+    return A, b
+
+
+def update_matrix_and_vector(A, b, cache, parameters=None, coords=None):
+    """src/problems.jl:406-411."""
+    update_matrix(A, cache["matrix_cache"], parameters, coords)
+    update_vector(b, cache["vector_cache"], parameters)
+    return A, b
+
+
+def assemble_matrix_and_vector_with_free_and_dirichlet_columns(a, l, T, U, V, precision=FP64, device=0, reuse=False,
+                                                               free_or_dirichlet=FREE):
+    """(A, Ad, b) (src/problems.jl:413-428); with reuse=True also the cache."""
+    A, Ad, mc = assemble_matrix_with_free_and_dirichlet_columns(a, T, U, V, precision=precision, device=device,
+                                                                reuse=True, free_or_dirichlet=free_or_dirichlet)
+    b, vc = assemble_vector(l, T, V, free_or_dirichlet, device=device, reuse=True)
+    if reuse:
+        return A, Ad, b, {"matrix_cache": mc, "vector_cache": vc}
+    return A, Ad, b
+
+
+def update_matrix_and_vector_with_free_and_dirichlet_columns(A, Ad, b, cache, parameters=None, coords=None):
+    """src/problems.jl:430-435."""
+    update_matrix_with_free_and_dirichlet_columns(A, Ad, cache["matrix_cache"], parameters, coords)
+    update_vector(b, cache["vector_cache"], parameters)
+    return A, Ad, b
+
+
+def _ensure_device_values(A):
+    """Makes the device hold A's own values on its block: uploads them when
+    another matrix was assembled into that block since, or A.nzval was
+    replaced."""
+    a, rows, cols = A.device
+    if a.values_owner.get((rows, cols)) is not A._token or A.nzval is not A._nzval_ref:
+        a.ctx.set_values(rows, cols, A.nzval)
+        A._nzval_ref = A.nzval
+        a.values_owner[(rows, cols)] = A._token
 
 
 def solve(A, b, precision=FP64, rtol=1e-10, x0=None, max_outer=30, max_inner=20000, inner_rtol=1e-4):
     """Solve A x = b on the GPU for a (FREE, FREE) matrix of assemble_matrix
-    (symmetric positive definite forms), with the values of its last device
-    assembly.  precision=MIXED: FP64 iterative refinement around an FP32
-    Jacobi-preconditioned CG (inner_rtol per step); FP64: one FP64
-    Jacobi-preconditioned CG.  Stands where the reference solves the
-    assembled linear problem (src/problems.jl:439-462, PS.solve).
-    Returns (x, stats); raises if rtol is not reached."""
+    (symmetric positive definite forms).  The device holds the values of the
+    last assembly on A's block; if that was another matrix (another form on
+    the same space and measure) or A's values were replaced, A's own values
+    are uploaded first, so the solve always uses A.  precision=MIXED: FP64
+    iterative refinement around an FP32 Jacobi-preconditioned CG (inner_rtol
+    per step); FP64: one FP64 Jacobi-preconditioned CG.  Stands where the
+    reference solves the assembled linear problem (src/problems.jl:439-462,
+    PS.solve).  Returns (x, stats); raises if rtol is not reached."""
     dev = getattr(A, "device", None)
     if dev is None:
-        raise ValueError("A must come from assemble_matrix (its values live on the device)")
+        raise ValueError("A must come from assemble_matrix (its pattern lives on the device)")
     a, rows, cols = dev
     if (rows, cols) != (FREE, FREE):
         raise ValueError("solve needs the (FREE, FREE) block")
+    _ensure_device_values(A)
     prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
     x, st = a.ctx.solve(prec, np.asarray(b, np.float64), x0, rtol, max_outer, max_inner, inner_rtol)
     if not st.converged:
@@ -368,11 +531,13 @@ def solve(A, b, precision=FP64, rtol=1e-10, x0=None, max_outer=30, max_inner=200
 def linear_problem(uhd, a_form, l_form, precision=FP64, device=0):
     """GT.PartitionedSolvers_linear_problem(uhd, a, l): the (FREE, FREE)
     matrix, and the load vector lifted by the Dirichlet values of uhd,
-    b = l - A_{FREE,DIRICHLET} x_D (src/problems.jl:439-452)."""
+    b = l - A_{FREE,DIRICHLET} x_D (src/problems.jl:439-452); the lifting
+    runs on the device (gtk_lift_dirichlet)."""
     V = uhd.space
-    A, Ad = assemble_matrix_with_free_and_dirichlet_columns(a_form, float, V, V, precision=precision, device=device)
-    b = assemble_vector(l_form, float, V, device=device)
+    A, Ad, b = assemble_matrix_and_vector_with_free_and_dirichlet_columns(a_form, l_form, float, V, V,
+                                                                          precision=precision, device=device)
     xd = np.asarray(uhd.values[V.nfree:], np.float64)
     if xd.size and Ad.nnz:
-        b = b - Ad.to_scipy() @ xd
+        _ensure_device_values(Ad)
+        b = Ad.device[0].ctx.lift_dirichlet(xd, b)
     return A, b

@@ -62,6 +62,8 @@ struct SolveWork {
   DevBuf perm, val64, val32, dinv, x, b, r, d, rr, z, p, q, partial, partial2, sc, bad;
   DevBuf bcol, bval;  // BSR(3) copy of the FP32 values (three-component fields)
   int block3 = -1;    // pattern made of aligned 3x3 blocks: -1 not checked, 0 no, 1 yes
+  bool bsr_cols = false;                // block columns of the BSR(3) copy written
+  bool prepared[2] = {false, false};    // solve plan built for FP64 / mixed
 };
 
 struct MatrixPlan {
@@ -70,6 +72,7 @@ struct MatrixPlan {
   DevBuf colptr, nzval;
   DevBuf rowval;  // device copy (device symbolic phase, local map), on demand
   SolveWork sw;
+  DevBuf lift_rptr, lift_perm, lift_col, lift_x, lift_b;  // row view of the block for gtk_lift_dirichlet
   std::map<int, std::unique_ptr<ExecPlan>> exec;  // key: form * 2 + precision
   bool valid = false;
 };
@@ -121,9 +124,13 @@ struct gtk_ctx {
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
   std::map<int, std::unique_ptr<VectorPlan>> vplans;
   bool has_mesh = false, has_space = false, has_ref = false;
+  // bumped by every call that replaces the mesh, the space, the reference
+  // tables or a pattern: a batch built on older plans refuses to run
+  uint64_t generation = 1;
 };
 
 static std::string g_create_err;
+
 
 #define FAIL(ctx, code, msg)  \
   do {                        \
@@ -334,6 +341,7 @@ int gtk_set_mesh(gtk_ctx *ctx, int dim, int64_t n_nodes, const double *xyz, int6
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_mesh = true;
   ctx->has_space = false;
+  ctx->generation++;
   ctx->mplans.clear();
   ctx->vplans.clear();
   return GTK_OK;
@@ -374,6 +382,7 @@ int gtk_set_space(gtk_ctx *ctx, int ncomp, int dofs_per_cell, int64_t n_free, in
   if (rc) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_space = true;
+  ctx->generation++;
   ctx->mplans.clear();
   ctx->vplans.clear();
   return GTK_OK;
@@ -406,6 +415,7 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
   if ((rc = upload(ctx, ctx->d_gN, ctx->gN.data(), ctx->gN.size()))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_ref = true;
+  ctx->generation++;
   return GTK_OK;
 }
 
@@ -439,6 +449,7 @@ int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, i
   if ((rc = upload(ctx, ctx->d_item_face, ctx->item_face.data(), ctx->item_face.size()))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_ref = true;
+  ctx->generation++;
   return GTK_OK;
 }
 
@@ -495,6 +506,7 @@ int gtk_build_pattern(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int64_t *n
   if (nnz) *nnz = plan->pat.nnz;
   if (ncols) *ncols = plan->pat.ncols;
   ctx->mplans[key_of(rows, cols)] = std::move(plan);
+  ctx->generation++;
   return GTK_OK;
 }
 
@@ -581,6 +593,7 @@ int gtk_build_pattern_device(gtk_ctx *ctx, int rows, int cols, int64_t *nnz, int
   if (nnz) *nnz = pat.nnz;
   if (ncols) *ncols = pat.ncols;
   ctx->mplans[key_of(rows, cols)] = std::move(plan);
+  ctx->generation++;
   return GTK_OK;
 }
 
@@ -933,6 +946,30 @@ static void fill_stats(const gtk_ctx *ctx, const MatrixPlan &p, int form, int pr
   st->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * D * nd * nd + 2 * D * D * nd + 2 * ctx->nv * D * D);
 }
 
+int gtk_plan_info(gtk_ctx *ctx, int form, int precision, int rows, int cols, gtk_plan_info_t *out) {
+  if (!ctx || !out) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
+  if (form < GTK_LAPLACE || form > GTK_ELASTICITY) FAIL(ctx, GTK_EINVAL, "unknown form");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ExecPlan *ep = nullptr;
+  int rc = get_exec(ctx, *p, form, precision, ep);
+  if (rc) return rc;
+  std::memset(out, 0, sizeof(*out));
+  out->matrix_cores = ep->k.mfma ? 1 : 0;
+  out->threads = ep->k.block;
+  out->grid = ep->k.mfma ? std::min(ep->tiles.ntiles, ep->resident) : ep->tiles.ntiles;
+  out->tiles = ep->tiles.ntiles;
+  out->max_tile_cells = ep->tiles.max_tile_cells;
+  out->stage_cells = ep->stage_cells;
+  out->lds_bytes = (int)ep->lds;
+  out->batches = ep->k.mfma ? ep->bat.nbatches : 0;
+  out->tile_cells = ep->tiles.tile_cells.size();
+  out->cells = ctx->n_cells;
+  return GTK_OK;
+}
+
 int gtk_assemble_matrix_async(gtk_ctx *ctx, int form, int precision, int rows, int cols) {
   if (!ctx) return GTK_EINVAL;
   MatrixPlan *p = find_plan(ctx, rows, cols);
@@ -1002,6 +1039,55 @@ int gtk_get_values(gtk_ctx *ctx, int rows, int cols, double *nzval_out) {
   return GTK_OK;
 }
 
+int gtk_set_values(gtk_ctx *ctx, int rows, int cols, const double *nzval) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  if (!nzval && p->pat.nnz) FAIL(ctx, GTK_EINVAL, "nzval is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (p->pat.nnz)
+    HIPCHK(ctx, hipMemcpyAsync(p->nzval.p, nzval, p->pat.nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return GTK_OK;
+}
+
+int gtk_lift_dirichlet(gtk_ctx *ctx, int rows, const double *xd, double *b) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, GTK_DIRICHLET);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern(rows, GTK_DIRICHLET) and gtk_assemble_matrix first");
+  const int64_t nr = p->pat.nrows, nc = p->pat.ncols, nnz = p->pat.nnz;
+  if ((!xd && nc) || (!b && nr)) FAIL(ctx, GTK_EINVAL, "xd and b are required");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if (!p->lift_rptr.p) {
+    // row view, built once per block: columns ascending, so each row lists
+    // its entries in ascending column order
+    std::vector<int64_t> rptr(nr + 1, 0);
+    for (int64_t k = 0; k < nnz; k++) rptr[p->pat.rowval[k]]++;
+    for (int64_t i = 0; i < nr; i++) rptr[i + 1] += rptr[i];
+    std::vector<int64_t> next(rptr.begin(), rptr.end() - 1);
+    std::vector<int32_t> perm(std::max<int64_t>(1, nnz)), col(std::max<int64_t>(1, nnz));
+    for (int64_t j = 0; j < nc; j++)
+      for (int64_t k = p->pat.colptr[j] - 1; k < p->pat.colptr[j + 1] - 1; k++) {
+        const int64_t slot = next[p->pat.rowval[k] - 1]++;
+        perm[slot] = (int32_t)k;
+        col[slot] = (int32_t)j;
+      }
+    if ((rc = upload(ctx, p->lift_rptr, rptr.data(), rptr.size())) || (rc = upload(ctx, p->lift_perm, perm.data(), perm.size())) ||
+        (rc = upload(ctx, p->lift_col, col.data(), col.size())) ||
+        (rc = alloc(ctx, p->lift_x, std::max<int64_t>(1, nc) * sizeof(double))) ||
+        (rc = alloc(ctx, p->lift_b, std::max<int64_t>(1, nr) * sizeof(double))))
+      return rc;
+  }
+  if (nc) HIPCHK(ctx, hipMemcpyAsync(p->lift_x.p, xd, nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if (nr) HIPCHK(ctx, hipMemcpyAsync(p->lift_b.p, b, nr * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  gtk::launch_lift(nr, p->lift_rptr.as<int64_t>(), p->lift_perm.as<int32_t>(), p->lift_col.as<int32_t>(),
+                   p->nzval.as<double>(), p->lift_x.as<double>(), p->lift_b.as<double>(), ctx->stream);
+  if (nr) HIPCHK(ctx, hipMemcpyAsync(b, p->lift_b.p, nr * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return GTK_OK;
+}
+
 int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr) {
   if (!ctx) return GTK_EINVAL;
   MatrixPlan *p = find_plan(ctx, rows, cols);
@@ -1053,48 +1139,51 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
   return GTK_OK;
 }
 
-int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rtol, int max_outer, int max_inner,
-              double inner_rtol, gtk_solve_stats *stats) {
-  if (!ctx) return GTK_EINVAL;
-  auto t0 = std::chrono::steady_clock::now();
-  MatrixPlan *pp = find_plan(ctx, GTK_FREE, GTK_FREE);
+}  // extern "C"
+
+// Device vectors of one solve call: the FP64 iterate, right-hand side and
+// residual, the conjugate-gradient vectors (FP32 or FP64), the partial sums
+// of the dot products, the scalars and the check flags.
+struct SolveBufs {
+  double *x = nullptr, *b = nullptr, *r = nullptr;
+  void *dinv = nullptr, *d = nullptr, *rr = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *partial = nullptr, *partial2 = nullptr, *sc = nullptr;
+  int *bad = nullptr;
+};
+
+static bool solve_try_bsr(const gtk_ctx *ctx, const MatrixPlan &P, bool f32) {
+  const char *bsr_env = std::getenv("GTK_SOLVE_BSR");
+  return f32 && ctx->ncomp == 3 && P.pat.ncols % 3 == 0 && P.pat.nnz > 0 && !(bsr_env && bsr_env[0] == '0');
+}
+
+static int solve_check_args(gtk_ctx *ctx, MatrixPlan *pp, int precision, double rtol, int max_outer, int max_inner,
+                            double inner_rtol) {
   if (!pp) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern(GTK_FREE, GTK_FREE) and gtk_assemble_matrix first");
-  if (!b || !x) FAIL(ctx, GTK_EINVAL, "b and x are required");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
   if (!(rtol > 0) || max_outer < 1 || max_inner < 1 || !(inner_rtol > 0 && inner_rtol < 1))
     FAIL(ctx, GTK_EINVAL, "rtol > 0, max_outer >= 1, max_inner >= 1, 0 < inner_rtol < 1");
-  MatrixPlan &P = *pp;
-  const int64_t n = P.pat.ncols, nnz = P.pat.nnz;
-  if (n != P.pat.nrows) FAIL(ctx, GTK_EINVAL, "the block is not square");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (pp->pat.ncols != pp->pat.nrows) FAIL(ctx, GTK_EINVAL, "the block is not square");
+  return GTK_OK;
+}
+
+// The solve's plan on a (FREE, FREE) block, built once per block and
+// precision: the pattern's rows on the device, the CSR-position -> CSC
+// position permutation, the row-ordered value arrays and, for
+// three-component fields made of aligned 3x3 blocks, the BSR(3) copy.
+static int solve_prepare(gtk_ctx *ctx, MatrixPlan &P, bool f32) {
   SolveWork &w = P.sw;
-  const bool f32 = precision == GTK_MIXED;
-  const size_t vb = f32 ? sizeof(float) : sizeof(double);
+  const int64_t n = P.pat.ncols, nnz = P.pat.nnz;
+  hipStream_t st = ctx->stream;
   int rc;
-  // row indices of the CSR view: the pattern's rowval on the device (the
-  // device symbolic phase leaves it there)
   if (!P.rowval.p && (rc = upload(ctx, P.rowval, P.pat.rowval.data(), P.pat.rowval.size()))) return rc;
   if ((rc = alloc(ctx, w.val64, std::max<int64_t>(1, nnz) * sizeof(double)))) return rc;
   if (f32 && (rc = alloc(ctx, w.val32, std::max<int64_t>(1, nnz) * sizeof(float)))) return rc;
-  const size_t nb = std::max<int64_t>(1, n);
-  if ((rc = alloc(ctx, w.dinv, nb * vb)) || (rc = alloc(ctx, w.x, nb * 8)) || (rc = alloc(ctx, w.b, nb * 8)) ||
-      (rc = alloc(ctx, w.r, nb * 8)) || (rc = alloc(ctx, w.d, nb * vb)) || (rc = alloc(ctx, w.rr, nb * vb)) ||
-      (rc = alloc(ctx, w.z, nb * vb)) || (rc = alloc(ctx, w.p, nb * vb)) || (rc = alloc(ctx, w.q, nb * vb)) ||
-      (rc = alloc(ctx, w.partial, gtk::kSolvePartials * 8)) || (rc = alloc(ctx, w.partial2, 2 * gtk::kSolvePartials * 8)) ||
-      (rc = alloc(ctx, w.sc, gtk::kSolveScalars * 8)) || (rc = alloc(ctx, w.bad, 16)))
-    return rc;
+  if ((rc = alloc(ctx, w.bad, 16))) return rc;
   gtk::SolveMat M;
   M.n = n;
   M.ptr = P.colptr.as<int32_t>();
   M.idx = P.rowval.as<int32_t>();
-  M.lanes = gtk::solve_rows_per_lane_group(n ? (double)nnz / n : 1.0);
-  if (const char *e = std::getenv("GTK_SPMV_LANES")) {  // measurement override
-    const int v = std::atoi(e);
-    if (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) M.lanes = v;
-  }
-  hipStream_t st = ctx->stream;
-  double *sc = w.sc.as<double>();
-  if (!w.perm.p) {  // CSR position -> CSC position, on first solve
+  if (!w.perm.p) {  // CSR position -> CSC position
     if ((rc = alloc(ctx, w.perm, std::max<int64_t>(1, nnz) * sizeof(int32_t)))) return rc;
     HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
     gtk::launch_transpose_perm(M, w.perm.as<int32_t>(), w.bad.as<int>(), st);
@@ -1106,35 +1195,63 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
       FAIL(ctx, GTK_EINVAL, "the pattern is not structurally symmetric");
     }
   }
-  HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
+  if (solve_try_bsr(ctx, P, f32)) {
+    if (w.block3 < 0) {  // checked once per block
+      HIPCHK(ctx, hipMemsetAsync(w.bad.p, 0, 16, st));
+      gtk::launch_block3_check(M, w.bad.as<int>() + 2, st);
+      int bad[4] = {0, 0, 0, 0};
+      HIPCHK(ctx, hipMemcpyAsync(bad, w.bad.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+      w.block3 = bad[2] ? 0 : 1;
+    }
+    if (w.block3 == 1 && !w.bcol.p &&
+        ((rc = alloc(ctx, w.bcol, (nnz / 9) * sizeof(int32_t))) || (rc = alloc(ctx, w.bval, nnz * sizeof(float)))))
+      return rc;
+  }
+  w.prepared[f32 ? 1 : 0] = true;
+  return GTK_OK;
+}
+
+// The solve itself on a prepared block, on device vectors only (no
+// allocation): values gathered into row order, the inverse diagonal, then
+// FP64 refinement around the (FP32 or FP64) Jacobi-preconditioned CG.
+static int solve_core(gtk_ctx *ctx, MatrixPlan &P, bool f32, const SolveBufs &u, double rtol, int max_outer,
+                      int max_inner, double inner_rtol, gtk_solve_stats *stats, std::chrono::steady_clock::time_point t0,
+                      double *host_x) {
+  SolveWork &w = P.sw;
+  const int64_t n = P.pat.ncols, nnz = P.pat.nnz;
+  gtk::SolveMat M;
+  M.n = n;
+  M.ptr = P.colptr.as<int32_t>();
+  M.idx = P.rowval.as<int32_t>();
+  M.lanes = gtk::solve_rows_per_lane_group(n ? (double)nnz / n : 1.0);
+  if (const char *e = std::getenv("GTK_SPMV_LANES")) {  // measurement override
+    const int v = std::atoi(e);
+    if (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) M.lanes = v;
+  }
+  hipStream_t st = ctx->stream;
+  double *sc = u.sc;
+  int rc;
+  HIPCHK(ctx, hipMemsetAsync(u.bad, 0, 16, st));
   gtk::launch_gather_values(nnz, w.perm.as<int32_t>(), P.nzval.as<double>(), w.val64.as<double>(),
                             f32 ? w.val32.as<float>() : nullptr, st);
-  gtk::launch_diag_inv(M, w.val64.as<double>(), w.dinv.p, !f32, w.bad.as<int>(), st);
-  HIPCHK(ctx, hipMemcpyAsync(w.b.p, b, n * 8, hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(w.x.p, x, n * 8, hipMemcpyHostToDevice, st));
-  // three-component fields numbered node by node: the FP32 iterations use a
-  // BSR(3) copy (one column index per 9 values) when the pattern is made of
-  // aligned 3x3 blocks (checked once per block; GTK_SOLVE_BSR=0 keeps CSR)
-  const char *bsr_env = std::getenv("GTK_SOLVE_BSR");
-  const bool try_bsr = f32 && ctx->ncomp == 3 && n % 3 == 0 && nnz > 0 && !(bsr_env && bsr_env[0] == '0');
-  if (try_bsr && w.block3 < 0) gtk::launch_block3_check(M, w.bad.as<int>() + 2, st);
-  int bad[4] = {0, 0, 0, 0};
-  HIPCHK(ctx, hipMemcpyAsync(bad, w.bad.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+  gtk::launch_diag_inv(M, w.val64.as<double>(), u.dinv, !f32, u.bad, st);
+  int bad = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&bad, u.bad, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(ctx, hipStreamSynchronize(st));
-  if (bad[0]) FAIL(ctx, GTK_EINVAL, "the matrix has a non-positive diagonal entry (not SPD)");
-  if (try_bsr && w.block3 < 0) w.block3 = bad[2] ? 0 : 1;
-  if (try_bsr && w.block3 == 1) {
+  if (bad) FAIL(ctx, GTK_EINVAL, "the matrix has a non-positive diagonal entry (not SPD)");
+  // three-component fields numbered node by node: the FP32 iterations use
+  // the BSR(3) copy (one column index per 9 values)
+  if (solve_try_bsr(ctx, P, f32) && w.block3 == 1) {
     M.nb = nnz / 9;
-    const bool first = !w.bcol.p;
-    if (first && ((rc = alloc(ctx, w.bcol, M.nb * sizeof(int32_t))) || (rc = alloc(ctx, w.bval, nnz * sizeof(float)))))
-      return rc;
     M.bcol = w.bcol.as<int32_t>();
     M.bval = w.bval.as<float>();
     if (const char *e = std::getenv("GTK_BSR_LANES")) {  // measurement override
       const int v = std::atoi(e);
       if (v == 8 || v == 16 || v == 32) M.block_lanes = v;
     }
-    gtk::launch_block3_build(M, w.val32.as<float>(), w.bcol.as<int32_t>(), w.bval.as<float>(), first, st);
+    gtk::launch_block3_build(M, w.val32.as<float>(), w.bcol.as<int32_t>(), w.bval.as<float>(), !w.bsr_cols, st);
+    w.bsr_cols = true;
   }
   auto read_sc = [&](int slot, double &out) -> int {
     HIPCHK(ctx, hipMemcpyAsync(&out, sc + slot, 8, hipMemcpyDeviceToHost, st));
@@ -1143,25 +1260,25 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   };
   // FP64 residual r = b - A x and its norm
   auto residual = [&](double &rn) -> int {
-    gtk::launch_spmv(M, w.val64.p, true, w.x.p, true, w.b.p, w.r.p, true, st);
-    gtk::launch_dot(n, w.r.p, w.r.p, true, w.partial.as<double>(), sc, 6, st);
+    gtk::launch_spmv(M, w.val64.p, true, u.x, true, u.b, u.r, true, st);
+    gtk::launch_dot(n, u.r, u.r, true, u.partial, sc, 6, st);
     double v = 0;
     int e = read_sc(6, v);
     rn = std::sqrt(std::max(0.0, v));
     return e;
   };
   double bn2 = 0;
-  gtk::launch_dot(n, w.b.p, w.b.p, true, w.partial.as<double>(), sc, 6, st);
+  gtk::launch_dot(n, u.b, u.b, true, u.partial, sc, 6, st);
   if ((rc = read_sc(6, bn2))) return rc;
   const double bn = std::sqrt(bn2);
   gtk::CgVectors v;
   v.f64 = !f32;
-  v.d = w.d.p;
-  v.r = w.rr.p;
-  v.z = w.z.p;
-  v.p = w.p.p;
-  v.q = w.q.p;
-  v.dinv = w.dinv.p;
+  v.d = u.d;
+  v.r = u.rr;
+  v.z = u.z;
+  v.p = u.p;
+  v.q = u.q;
+  v.dinv = u.dinv;
   const void *val = f32 ? w.val32.p : w.val64.p;
   constexpr int kCheck = 8;  // iterations between convergence reads
   double rn = 0;
@@ -1171,24 +1288,24 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
   while (rn > target && outer < max_outer) {
     // inner solve of A d = r / |r| (FP32 for the mixed form, FP64 otherwise)
     const double s = rn > 0 ? 1.0 / rn : 1.0;
-    gtk::launch_cg_start(n, s, w.r.as<double>(), v, w.partial.as<double>(), sc, st);
+    gtk::launch_cg_start(n, s, u.r, v, u.partial, sc, st);
     // the FP64 form runs one CG to the outer tolerance; the mixed form to inner_rtol
     const double itol = f32 ? inner_rtol : std::max(target / rn, 1e-300);
     int it = 0;
     while (it < max_inner) {
       for (int k = 0; k < kCheck && it < max_inner; k++, it++)
-        gtk::launch_cg_iteration(M, val, v, w.partial.as<double>(), w.partial2.as<double>(), sc, it,
-                                 k == kCheck - 1 || it == max_inner - 1, st);
+        gtk::launch_cg_iteration(M, val, v, u.partial, u.partial2, sc, it, k == kCheck - 1 || it == max_inner - 1,
+                                 st);
       double rr = 0;
       if ((rc = read_sc(5, rr))) return rc;
       if (!(std::sqrt(std::max(0.0, rr)) > itol)) break;
     }
     inner_total += it;
-    gtk::launch_refine(n, rn, v.d, !f32, w.x.as<double>(), st);
+    gtk::launch_refine(n, rn, v.d, !f32, u.x, st);
     outer++;
     if ((rc = residual(rn))) return rc;
   }
-  HIPCHK(ctx, hipMemcpyAsync(x, w.x.p, n * 8, hipMemcpyDeviceToHost, st));
+  if (host_x && n) HIPCHK(ctx, hipMemcpyAsync(host_x, u.x, n * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(ctx, hipStreamSynchronize(st));
   if (stats) {
     stats->outer_iterations = outer;
@@ -1198,6 +1315,51 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
     stats->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return GTK_OK;
+}
+
+extern "C" {
+
+int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rtol, int max_outer, int max_inner,
+              double inner_rtol, gtk_solve_stats *stats) {
+  if (!ctx) return GTK_EINVAL;
+  auto t0 = std::chrono::steady_clock::now();
+  MatrixPlan *pp = find_plan(ctx, GTK_FREE, GTK_FREE);
+  int rc = solve_check_args(ctx, pp, precision, rtol, max_outer, max_inner, inner_rtol);
+  if (rc) return rc;
+  if (!b || !x) FAIL(ctx, GTK_EINVAL, "b and x are required");
+  MatrixPlan &P = *pp;
+  const int64_t n = P.pat.ncols;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  SolveWork &w = P.sw;
+  const bool f32 = precision == GTK_MIXED;
+  const size_t vb = f32 ? sizeof(float) : sizeof(double);
+  if ((rc = solve_prepare(ctx, P, f32))) return rc;
+  const size_t nb = std::max<int64_t>(1, n);
+  if ((rc = alloc(ctx, w.dinv, nb * vb)) || (rc = alloc(ctx, w.x, nb * 8)) || (rc = alloc(ctx, w.b, nb * 8)) ||
+      (rc = alloc(ctx, w.r, nb * 8)) || (rc = alloc(ctx, w.d, nb * vb)) || (rc = alloc(ctx, w.rr, nb * vb)) ||
+      (rc = alloc(ctx, w.z, nb * vb)) || (rc = alloc(ctx, w.p, nb * vb)) || (rc = alloc(ctx, w.q, nb * vb)) ||
+      (rc = alloc(ctx, w.partial, gtk::kSolvePartials * 8)) || (rc = alloc(ctx, w.partial2, 2 * gtk::kSolvePartials * 8)) ||
+      (rc = alloc(ctx, w.sc, gtk::kSolveScalars * 8)))
+    return rc;
+  SolveBufs u;
+  u.x = w.x.as<double>();
+  u.b = w.b.as<double>();
+  u.r = w.r.as<double>();
+  u.dinv = w.dinv.p;
+  u.d = w.d.p;
+  u.rr = w.rr.p;
+  u.z = w.z.p;
+  u.p = w.p.p;
+  u.q = w.q.p;
+  u.partial = w.partial.as<double>();
+  u.partial2 = w.partial2.as<double>();
+  u.sc = w.sc.as<double>();
+  u.bad = w.bad.as<int>();
+  if (n) {
+    HIPCHK(ctx, hipMemcpyAsync(u.b, b, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(u.x, x, n * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
+  return solve_core(ctx, P, f32, u, rtol, max_outer, max_inner, inner_rtol, stats, t0, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1213,6 +1375,7 @@ struct gtk_batch {
   std::string err;
   std::vector<gtk_ctx *> ctxs;
   std::vector<std::unique_ptr<ExecPlan>> plans;  // one per problem
+  std::vector<uint64_t> gens;                    // each context's generation when the plans were built
   int tile_cells[3] = {0, 0, 0};                 // cap on the cells of a tile per element type (0: LDS bound)
   int waves = 8;                                 // waves per workgroup
   std::vector<gtk::MatrixArgs> args;  // host copy of d_problems
@@ -1379,6 +1542,7 @@ int gtk_batch_build(gtk_batch *b) {
   if (!gtk::find_batched_kernel(b->form, b->precision, b->waves, b->k))
     BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
   b->plans.clear();
+  b->gens.clear();
   std::vector<gtk::MatrixArgs> args;
   std::vector<gtk::VectorArgs> vargs(b->ctxs.size());
   std::vector<int32_t> types, fkinds(b->ctxs.size(), -1);
@@ -1401,6 +1565,7 @@ int gtk_batch_build(gtk_batch *b) {
     b->cells += ctx->n_cells;
     b->nnz += P->pat.nnz;
     b->plans.push_back(std::move(ep));
+    b->gens.push_back(ctx->generation);
   }
   // load-vector tiles after every matrix tile: short items fill the tail
   for (size_t p = 0; p < b->ctxs.size(); p++) {
@@ -1455,6 +1620,12 @@ int gtk_batch_run(gtk_batch *b, gtk_stats *stats) {
   if (!b) return GTK_EINVAL;
   if (!b->gexec) BFAIL(b, GTK_ESTATE, "gtk_batch_build first");
   auto t0 = std::chrono::steady_clock::now();
+  // the batch records hold absolute offsets into each problem's values:
+  // a new mesh, space, reference tables or pattern invalidates them
+  for (size_t p = 0; p < b->ctxs.size(); p++)
+    if (b->ctxs[p]->generation != b->gens[p])
+      BFAIL(b, GTK_ESTATE, "problem " + std::to_string(p) +
+                               " changed its mesh, space, reference tables or pattern: gtk_batch_build again");
   BHIPCHK(b, hipSetDevice(b->device));
   // the graph reads the per-problem records through d_problems: refresh
   // them when a context moved a buffer (new coefficients, kappa reset)

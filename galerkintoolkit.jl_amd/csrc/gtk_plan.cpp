@@ -337,7 +337,11 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
     for (int32_t b = t.tile_batch_ptr[tl]; b < t.tile_batch_ptr[tl + 1]; b++) batch_tile[b] = tl;
   for (int64_t b = 0; b < out.nbatches; b++)
     if (t.batch_col[b + 1] - t.batch_col[b] > kBatchCols) return "batch with too many columns";
-  parallel_for(out.nbatches, nthreads, [&](int, int64_t b0, int64_t b1) {
+  // the rows of one slot are added by the lanes of one instruction: they
+  // must be distinct (no cell lists one DOF of the row set twice)
+  std::vector<int64_t> dup(std::max(1, nthreads), -1);
+  parallel_for(out.nbatches, nthreads, [&](int th, int64_t b0, int64_t b1) {
+    std::vector<int64_t> stamp(std::max(1, max_seg), -1);
     for (int64_t b = b0; b < b1; b++) {
       uint8_t *r = out.rec.data() + (size_t)b * out.rec_bytes;
       uint32_t *sb = reinterpret_cast<uint32_t *>(r + BatchPlan::kSlotSb);
@@ -369,6 +373,8 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
             int32_t rr = map_dof(dofs[a], p.rows);
             if (!rr) continue;
             const int o = (int)(std::lower_bound(seg, seg + len, rr) - seg);
+            if (stamp[o] == k) dup[th] = cells[t.ctr_slot[k]];
+            stamp[o] = k;
             const size_t pos = (size_t)a * kBatchSlots + ns;
             if (out.off_bytes == 1) off[pos] = (uint8_t)o;
             else reinterpret_cast<uint16_t *>(off)[pos] = (uint16_t)o;
@@ -384,6 +390,8 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
       reinterpret_cast<uint32_t *>(r)[1] = (uint32_t)abase;
     }
   });
+  for (int64_t c : dup)
+    if (c >= 0) return "cell " + std::to_string(c) + " lists one DOF of the row set twice";
   return "";
 }
 
@@ -442,8 +450,13 @@ std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t
   std::vector<int32_t> col_tile(ncol_plan);
   for (int tl = 0; tl < t.ntiles; tl++)
     for (int32_t i = t.tile_col_ptr[tl]; i < t.tile_col_ptr[tl + 1]; i++) col_tile[i] = tl;
-  std::string err;
-  parallel_for(ncol_plan, nthreads, [&](int, int64_t b, int64_t e) {
+  // the VALU kernel's lanes of one contributor add their rows into the
+  // column accumulator in one instruction, in program order: that is
+  // deterministic only if those rows are distinct, i.e. no cell lists one
+  // DOF of the row set twice
+  std::vector<int64_t> dup(std::max(1, nthreads), -1);
+  parallel_for(ncol_plan, nthreads, [&](int th, int64_t b, int64_t e) {
+    std::vector<int64_t> stamp(std::max(1, max_seg), -1);
     for (int64_t i = b; i < e; i++) {
       int32_t j = t.col_id[i];
       const int32_t *seg = p.rowval.data() + (p.colptr[j] - 1);
@@ -456,14 +469,19 @@ std::string build_offsets(const MeshView &m, const Pattern &p, const TilePlan &t
           int32_t r = map_dof(dofs[a], p.rows);
           if (!r) continue;
           const int32_t *it = std::lower_bound(seg, seg + len, r);
+          const int32_t o = (int32_t)(it - seg);
+          if (stamp[o] == k) dup[th] = c;
+          stamp[o] = k;
           const size_t pos = (size_t)k * out.stride + a;
-          if (out.bytes == 1) out.data[pos] = (uint8_t)(it - seg);
-          else reinterpret_cast<uint16_t *>(out.data.data())[pos] = (uint16_t)(it - seg);
+          if (out.bytes == 1) out.data[pos] = (uint8_t)o;
+          else reinterpret_cast<uint16_t *>(out.data.data())[pos] = (uint16_t)o;
         }
       }
     }
   });
-  return err;
+  for (int64_t c : dup)
+    if (c >= 0) return "cell " + std::to_string(c) + " lists one DOF of the row set twice";
+  return std::string();
 }
 
 }  // namespace gtk

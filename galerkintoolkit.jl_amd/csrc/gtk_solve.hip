@@ -334,7 +334,28 @@ __global__ __launch_bounds__(kRB) void gather_values_kernel(int64_t n, const int
   }
 }
 
+// b[i] += sum_k a[perm[k]] * (-xd[col[k]]) over row i's entries in
+// ascending column order, multiplications and additions unfused: the order
+// and rounding of mul!(b, Ad, xd, -1, 1) on a CSC matrix
+__global__ __launch_bounds__(kRB) void lift_kernel(int64_t n, const int64_t *__restrict__ rptr,
+                                                   const int32_t *__restrict__ perm, const int32_t *__restrict__ col,
+                                                   const double *__restrict__ a, const double *__restrict__ xd,
+                                                   double *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
+    double acc = b[i];
+    for (int64_t k = rptr[i]; k < rptr[i + 1]; k++) acc = __dadd_rn(acc, __dmul_rn(a[perm[k]], -xd[col[k]]));
+    b[i] = acc;
+  }
+}
+
 }  // namespace
+
+void launch_lift(int64_t nrows, const int64_t *rptr, const int32_t *perm, const int32_t *col, const double *a,
+                 const double *xd, double *b, hipStream_t st) {
+  if (nrows == 0) return;
+  const int64_t blocks = (nrows + kRB - 1) / kRB;
+  lift_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), kRB, 0, st>>>(nrows, rptr, perm, col, a, xd, b);
+}
 
 int solve_rows_per_lane_group(double mean_row) {
   return mean_row > 48 ? 16 : (mean_row > 12 ? 8 : 4);

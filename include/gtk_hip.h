@@ -169,6 +169,20 @@ int gtk_assemble_matrix(gtk_ctx *ctx, int form, int precision, int rows, int col
                         gtk_stats *stats);
 /* Copy the device values of a (rows, cols) block to the host. */
 int gtk_get_values(gtk_ctx *ctx, int rows, int cols, double *nzval_out);
+/* Replace the device values of a (rows, cols) block by nzval (host, nnz
+ * values in the block's CSC order): a matrix whose values the caller
+ * changed, or one assembled earlier and overwritten on the device by the
+ * assembly of another form on the same block, before gtk_solve.  The
+ * reference's solve reads the values of the SparseMatrixCSC it is given
+ * (src/problems.jl:439-462); this is how a caller hands them back. */
+int gtk_set_values(gtk_ctx *ctx, int rows, int cols, const double *nzval);
+/* Lifting of the Dirichlet values: b -= A_(rows, DIRICHLET) xd on the
+ * device, with the values last assembled into the (rows, GTK_DIRICHLET)
+ * block, each row summed in ascending column order with unfused multiply
+ * and add.  b: host, in/out, one value per row of the block; xd: host,
+ * n_diri values.  Replaces mul!(b, Ad, xd, -1, 1) in
+ * PartitionedSolvers_linear_problem (src/problems.jl:439-452). */
+int gtk_lift_dirichlet(gtk_ctx *ctx, int rows, const double *xd, double *b);
 /* Device address of the value array of a (rows, cols) block (for callers
  * that keep the matrix on the GPU). */
 int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr);
@@ -178,6 +192,25 @@ int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr);
  * f: GTK_F_CONST ncomp values, GTK_F_CELL n_cells*ncomp, GTK_F_NODAL
  * (n_free+n_diri) values.  b_out (host) may be NULL. */
 int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, double *b_out, gtk_stats *stats);
+
+/* Work plan of (form, precision) over a built (rows, cols) block, built
+ * here if no assembly built it yet: the kernel chosen and its launch shape,
+ * as the autotuner and the benchmarks report them.  No counterpart in the
+ * reference (its assembly loop has no launch shape). */
+typedef struct gtk_plan_info_t {
+  int matrix_cores;    /* 1: matrix-core batch kernel, 0: VALU kernel */
+  int threads;         /* threads per workgroup */
+  int grid;            /* workgroups launched */
+  int tiles;           /* tiles of the column-owner plan */
+  int max_tile_cells;  /* most cells (owned and halo) of one tile */
+  int stage_cells;     /* cells staged per geometry pass (matrix-core kernel) */
+  int lds_bytes;       /* LDS per workgroup */
+  int pad;
+  int64_t batches;     /* batch records (matrix-core kernel) */
+  int64_t tile_cells;  /* cells summed over the tiles: tile_cells / cells is the halo redundancy */
+  int64_t cells;       /* cells of the mesh */
+} gtk_plan_info_t;
+int gtk_plan_info(gtk_ctx *ctx, int form, int precision, int rows, int cols, gtk_plan_info_t *out);
 
 /* Enqueue the matrix assembly of a (rows, cols) block on the context's
  * stream without synchronising (for timing loops); gtk_sync waits. */

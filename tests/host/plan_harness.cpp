@@ -127,7 +127,7 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
   gtk::OffsetPlan o;
   std::string e = gtk::build_pattern(m, rows, cols, p, 4);
   if (e.empty()) e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, chunk, t, 4);
-  if (e.empty()) e = gtk::build_offsets(m, p, t, o, 4);
+  if (e.empty() && mode != 2) e = gtk::build_offsets(m, p, t, o, 4);  // mode 2: batch records only
   if (!e.empty()) { std::strncpy(err, e.c_str(), errlen - 1); return -1; }
   *nnz_out = p.nnz;
   *ntiles = t.ntiles;
@@ -152,7 +152,7 @@ extern "C" int harness_run(int dim, int64_t n_nodes, const double *xyz, int64_t 
       }
     }
   }
-  if (mode == 1) return replay_batches(m, p, t, nzval, err, errlen);
+  if (mode >= 1) return replay_batches(m, p, t, nzval, err, errlen);
   for (int64_t i = 0; i < p.nnz; i++) nzval[i] = -1.0;  // every entry must be written
   for (int tl = 0; tl < t.ntiles; tl++) {
     const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[tl];

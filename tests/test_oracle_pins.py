@@ -323,9 +323,10 @@ def test_mass_and_load_consistency(GT, O):
 
 
 @pytest.mark.parametrize("cells,simplex", [((2, 2, 2), True), ((2, 3, 2), False)])
-def test_elasticity_rigid_modes_and_patch(GT, O, cells, simplex):
-    """Rigid motions are in the kernel; a linear displacement solves the
-    homogeneous problem exactly (P2 tets, Q2 hexes)."""
+def test_elasticity_rigid_modes(GT, O, cells, simplex):
+    """Rigid motions are in the kernel and K is symmetric (P2 tets, Q2
+    hexes); the lambda/mu law itself is pinned by the uniform-strain energy
+    and the linear patch solve below."""
     D = 3
     m = GT.cartesian_mesh((0.0, 1.0) * 3, cells, simplexify=simplex)
     x = GT.perturb_interior_nodes(m, 0.15, 6)
@@ -350,3 +351,74 @@ def test_elasticity_rigid_modes_and_patch(GT, O, cells, simplex):
         u[comp == j] = xf[comp == j, i]
         assert np.max(np.abs(K @ u)) < 1e-11
     assert abs(K - K.T).max() < 1e-13
+
+
+def _vector_dofs(GT, m, V):
+    """Coordinates and component of each free DOF of a three-component
+    space (component-major inside each node)."""
+    xf, xd = dof_coordinates(GT, m, V)
+    return xf, np.tile(np.arange(3), V.nfree // 3), xd, np.tile(np.arange(3), V.ndiri // 3)
+
+
+def _tet_volumes(m):
+    X = m.coords[m.cells]
+    return np.abs(np.einsum("cij,cij->c", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])[:, None, :],
+                            (X[:, 3] - X[:, 0])[:, None, :])) / 6.0
+
+
+@pytest.mark.parametrize("cells,simplex", [((2, 2, 2), True), ((2, 3, 2), False)])
+def test_elasticity_uniform_strain_energy(GT, O, cells, simplex):
+    """u = E x (E not symmetric) has the uniform strain eps = (E + E^T)/2 and
+    the energy u^T K u = sum_c |Omega_c| (lambda_c (tr eps)^2 + 2 mu_c eps:eps)
+    (sigma = lambda tr(eps) I + 2 mu eps, docs example_linear_elasticity.jl:
+    86-95).  lambda != mu per cell on tets (cell volumes exact), constant
+    lambda = 0.6, mu = 1.7 on hexes (total volume 1): a swap of lambda and
+    mu, a factor on mu or a missing skew cancellation fails."""
+    m = GT.cartesian_mesh((0.0, 1.0) * 3, cells, simplexify=simplex)
+    m.coords = GT.perturb_interior_nodes(m, 0.15, 8)
+    V = GT.lagrange_space(GT.interior(m), 2, tensor_size=(3,))
+    p = O.Problem(list(cells), simplexify=simplex).space(2, ncomp=3)
+    p.set_coords(m.coords)
+    p.quadrature(5 if simplex else 4)
+    nc = p.q(O.Q_NCELLS)
+    rng = np.random.default_rng(9)
+    if simplex:
+        lam, mu, vol = rng.uniform(0.5, 2.0, nc), rng.uniform(0.5, 2.0, nc), _tet_volumes(m)
+    else:
+        lam, mu, vol = np.full(nc, 0.6), np.full(nc, 1.7), None
+    K = csc(*p.assemble_matrix(O.ELASTICITY, coef=np.stack([lam, mu], axis=1).ravel()), p.q(O.Q_NFREE))
+    xf, comp, _, _ = _vector_dofs(GT, m, V)
+    E = np.array([[0.3, -1.1, 0.4], [0.7, -0.2, 0.9], [-0.5, 0.25, 1.3]])
+    u = np.einsum("kj,kj->k", E[comp], xf)
+    eps = 0.5 * (E + E.T)
+    per_volume = lambda l, k: l * np.trace(eps) ** 2 + 2.0 * k * np.sum(eps * eps)
+    want = np.sum(vol * per_volume(lam, mu)) if simplex else per_volume(0.6, 1.7) * 1.0
+    assert abs(u @ (K @ u) - want) <= 1e-11 * want
+    # the same with lambda and mu exchanged differs (the check can tell them apart)
+    swapped = np.sum(vol * per_volume(mu, lam)) if simplex else per_volume(1.7, 0.6)
+    assert abs(swapped - want) > 1e-3 * want
+
+
+@pytest.mark.parametrize("cells,simplex", [((2, 2, 2), True), ((2, 2, 3), False)])
+def test_elasticity_linear_patch_solve(GT, O, cells, simplex):
+    """Constant lambda != mu, no body force, u = E x + c prescribed on the
+    whole boundary: the discrete solution at the free DOFs is E x + c
+    exactly (P2 and Q2 reproduce linear fields; tolerance of the
+    reference's patch tests, test/assembly_tests.jl:112-118)."""
+    m = GT.cartesian_mesh((0.0, 1.0) * 3, cells, simplexify=simplex)
+    m.coords = GT.perturb_interior_nodes(m, 0.15, 12)
+    V = GT.lagrange_space(GT.interior(m), 2, tensor_size=(3,), dirichlet_boundary=GT.boundary(m))
+    p = O.Problem(list(cells), simplexify=simplex).space(2, ncomp=3, dirichlet=1)
+    p.set_coords(m.coords)
+    p.quadrature(5 if simplex else 4)
+    nc = p.q(O.Q_NCELLS)
+    lm = np.stack([np.full(nc, 0.8), np.full(nc, 1.9)], axis=1).ravel()
+    nf = p.q(O.Q_NFREE)
+    K = csc(*p.assemble_matrix(O.ELASTICITY, coef=lm), nf)
+    Kd = csc(*p.assemble_matrix(O.ELASTICITY, coef=lm, rows=O.FREE, cols=O.DIRICHLET), nf)
+    xf, cf, xd, cd = _vector_dofs(GT, m, V)
+    E = np.array([[0.4, 0.1, -0.3], [-0.6, 0.2, 0.5], [0.9, -0.7, 0.1]])
+    c = np.array([0.2, -0.1, 0.05])
+    g = lambda x, k: np.einsum("kj,kj->k", E[k], x) + c[k]
+    u = spla.spsolve(K.tocsc(), -(Kd @ g(xd, cd)))
+    assert np.max(np.abs(u - g(xf, cf))) < 1e-10

@@ -135,3 +135,22 @@ def test_transpose_perm(harness, GT):
     cp = (B.indptr + 1).astype(np.int32)
     rv = (B.indices + 1).astype(np.int32)
     assert H.harness_transpose(V.nfree, cp, rv, np.zeros(B.nnz + 1, np.int32), err, 128) != 0
+
+
+@pytest.mark.parametrize("mode", [0, 2], ids=["offsets", "batches"])
+def test_duplicate_cell_dof_rejected(harness, GT, mode):
+    """A cell listing one DOF twice would make two lanes of one accumulation
+    instruction add into the same nonzero (order no longer fixed): both the
+    VALU offsets and the batch records refuse such a space."""
+    m = GT.cartesian_mesh((0.0, 1.0) * 2, (3, 3))
+    V = GT.lagrange_space(GT.interior(m), 1)
+    cd = np.ascontiguousarray(V.cell_dofs, np.int32).copy()
+    cd[4, 1] = cd[4, 0]
+    cn = np.ascontiguousarray(m.cells, np.int32)
+    cap = 100_000
+    colptr = np.zeros(V.nfree + 1, np.int32)
+    err = C.create_string_buffer(256)
+    rc = harness.harness_run(2, m.coords.shape[0], np.ascontiguousarray(m.coords), cn.shape[0], cn.shape[1], cn,
+                             cd.shape[1], V.nfree, V.ndiri, cd, 1, 1, 64, 8, C.byref(C.c_int64()), colptr,
+                             np.zeros(cap, np.int32), np.zeros(cap), cap, C.byref(C.c_int()), mode, err, 256)
+    assert rc == -1 and b"twice" in err.value, (rc, err.value)
