@@ -15,6 +15,7 @@ GTK_FREE, GTK_DIRICHLET = 1, 2
 GTK_LAPLACE, GTK_MASS, GTK_ELASTICITY = 0, 1, 2
 GTK_FP64, GTK_MIXED = 0, 1
 GTK_F_CONST, GTK_F_CELL, GTK_F_NODAL = 0, 1, 2
+GTK_NOALLOC_MAX_DOUBLES = 8 * 1024 * 1024  # scratch of gtk_assemble_vector_noalloc
 
 # every symbol the header declares
 EXPORTS = [
@@ -23,6 +24,7 @@ EXPORTS = [
     "gtk_local_nnz_map", "gtk_add_block",
     "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_set_values", "gtk_values_device_ptr", "gtk_assemble_vector",
+    "gtk_assemble_vector_noalloc",
     "gtk_plan_info", "gtk_lift_dirichlet",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
     "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_waves", "gtk_batch_set_load", "gtk_batch_get_vector",
@@ -94,6 +96,7 @@ def load():
     L.gtk_plan_info.argtypes = [vp, i, i, i, i, C.POINTER(GtkPlanInfo)]
     L.gtk_values_device_ptr.argtypes = [vp, i, i, C.POINTER(C.c_void_p)]
     L.gtk_assemble_vector.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
+    L.gtk_assemble_vector_noalloc.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_assemble_matrix_async.argtypes = [vp, i, i, i, i]
     L.gtk_sync.argtypes = [vp]
     L.gtk_time_matrix.argtypes = [vp, i, i, i, i, i, C.POINTER(C.c_double)]
@@ -211,6 +214,16 @@ class Context:
         st = GtkStats()
         self.check(self.L.gtk_assemble_vector(self.h, fkind, np.ascontiguousarray(f, np.float64), rows,
                                               out.ctypes.data_as(C.c_void_p), C.byref(st)), "gtk_assemble_vector")
+        return out[:n], st
+
+    def assemble_vector_noalloc(self, fkind, f, rows, n):
+        """gtk_assemble_vector_noalloc: the same load vector with no device
+        allocation (needs one gtk_assemble_vector on the row set first)."""
+        out = np.zeros(max(n, 1), np.float64)
+        st = GtkStats()
+        self.check(self.L.gtk_assemble_vector_noalloc(self.h, fkind, np.ascontiguousarray(f, np.float64), rows,
+                                                      out.ctypes.data_as(C.c_void_p), C.byref(st)),
+                   "gtk_assemble_vector_noalloc")
         return out[:n], st
 
     def assemble_matrix_async(self, form, precision, rows, cols):

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -130,6 +131,39 @@ struct gtk_ctx {
 };
 
 static std::string g_create_err;
+
+// Scratch of gtk_assemble_vector_noalloc: the staged load values and the
+// load vector before its copy to the host live in this file-scope device
+// array, so that call allocates nothing.  Every device the library runs on
+// holds its own copy until the process ends; its address on each device is
+// resolved once, in gtk_create with that device current, and host code only
+// ever uses that resolved address (the array's name is a host-side stand-in).
+static constexpr size_t kNoAllocDoubles = GTK_NOALLOC_MAX_DOUBLES;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+static constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return GTK_OK;  // the no-alloc call reports it
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return GTK_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return GTK_EDEVICE;
+  }
+  g_scratch[device] = static_cast<double *>(p);
+  return GTK_OK;
+}
+
+// Zero fill for the no-alloc path (a kernel, not a memset node, so callers
+// may capture the call in a graph).
+__global__ void zero_fill_kernel(double *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
 
 
 #define FAIL(ctx, code, msg)  \
@@ -289,6 +323,10 @@ int gtk_create(int device, gtk_ctx **out) {
   if (const char *l = std::getenv("GTK_LDS_KB")) {
     const long kb = std::strtol(l, nullptr, 10);
     if (kb >= 16 && kb <= 160) ctx->lds_budget = (size_t)kb * 1024;
+  }
+  if (resolve_scratch(device) != GTK_OK) {
+    delete ctx;
+    return GTK_EDEVICE;
   }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
@@ -1133,6 +1171,63 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
     stats->cells = ctx->n_cells;
     stats->nnz = vp.nrows;
     stats->bytes_alg = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 + vp.nrows * 8;
+    stats->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * ctx->dofs_per_cell + 2 * ctx->nv * ctx->dim * ctx->dim);
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return GTK_OK;
+}
+
+int gtk_assemble_vector_noalloc(gtk_ctx *ctx, int fkind, const double *f, int rows, double *b_out, gtk_stats *stats) {
+  if (!ctx) return GTK_EINVAL;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!ctx->has_space) FAIL(ctx, GTK_ESTATE, "gtk_set_space first");
+  if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
+  if (rows != GTK_FREE && rows != GTK_DIRICHLET) FAIL(ctx, GTK_EINVAL, "rows must be GTK_FREE or GTK_DIRICHLET");
+  if (fkind < GTK_F_CONST || fkind > GTK_F_NODAL || !f) FAIL(ctx, GTK_EINVAL, "invalid load");
+  // scratch layout: the vector (rounded up to 256 bytes), then the load values
+  const int64_t nrows = rows == GTK_FREE ? ctx->n_free : ctx->n_diri;
+  const size_t nout = ((size_t)std::max<int64_t>(1, nrows) + 31) & ~(size_t)31;
+  const size_t nf = load_count(ctx, fkind);
+  if (nout + nf > kNoAllocDoubles)
+    FAIL(ctx, GTK_EINVAL,
+         "gtk_assemble_vector_noalloc: " + std::to_string(nrows) + " rows and " + std::to_string(nf) +
+             " load values exceed the scratch of " + std::to_string(kNoAllocDoubles) +
+             " doubles (GTK_NOALLOC_MAX_DOUBLES); use gtk_assemble_vector");
+  double *s = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch[ctx->device] : nullptr;
+  if (!s) FAIL(ctx, GTK_EUNSUP, "no scratch on this device ordinal");
+  auto it = ctx->vplans.find(rows);
+  if (it == ctx->vplans.end() || !it->second)
+    FAIL(ctx, GTK_ESTATE, "gtk_assemble_vector on this row set first (it builds the tile plan)");
+  const VectorPlan &vp = *it->second;
+  gtk::KernelInfo k;
+  if (!gtk::find_vector_kernel(elem_key(ctx), fkind, k)) FAIL(ctx, GTK_EUNSUP, "no vector kernel for this element");
+  size_t lds = k.table_bytes + (size_t)vp.tiles.max_tile_cells * k.geo_bytes_per_cell;
+  if (lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (lds > 64 * 1024) HIPCHK(ctx, hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  double *sf = s + nout;
+  gtk::VectorArgs a = vector_args(ctx, vp, sf);
+  a.out = s;
+  void *args[] = {&a};
+  if (nf) HIPCHK(ctx, hipMemcpyAsync(sf, f, nf * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (vp.tiles.uncovered && nrows) {  // rows no cell touches stay zero; the array keeps the last call's values
+    const int64_t blocks = std::min<int64_t>(1024, (nrows + 255) / 256);
+    hipLaunchKernelGGL(zero_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, s, nrows);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  if (vp.tiles.ntiles)
+    HIPCHK(ctx, hipLaunchKernel(k.fn, dim3(vp.tiles.ntiles), dim3(gtk::kBlock), args, lds, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  if (b_out && nrows) HIPCHK(ctx, hipMemcpyAsync(b_out, s, nrows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->kernel_ms = ms;
+    stats->cells = ctx->n_cells;
+    stats->nnz = nrows;
+    stats->bytes_alg = (int64_t)ctx->n_nodes * ctx->dim * 8 + (int64_t)ctx->n_cells * ctx->nv * 4 + nrows * 8;
     stats->flops_alg = (int64_t)ctx->n_cells * ctx->nq * (2 * ctx->dofs_per_cell + 2 * ctx->nv * ctx->dim * ctx->dim);
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }

@@ -193,6 +193,28 @@ int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr);
  * (n_free+n_diri) values.  b_out (host) may be NULL. */
 int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, double *b_out, gtk_stats *stats);
 
+/* gtk_assemble_vector without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernel and result as gtk_assemble_vector,
+ * bit for bit; the load values and the vector are staged in a per-device
+ * scratch array inside the library (GTK_NOALLOC_MAX_DOUBLES doubles, 64 MiB,
+ * held by every device the library runs on until the process ends) instead
+ * of context buffers, and rows that no cell touches are cleared by a kernel,
+ * not a memset.
+ *  - GTK_EINVAL if the rows of the set (rounded up to 32) plus the load
+ *    values (ncomp, n_cells*ncomp or n_free+n_diri) exceed
+ *    GTK_NOALLOC_MAX_DOUBLES.
+ *  - GTK_ESTATE unless gtk_assemble_vector ran once on this context and row
+ *    set since its mesh, space or reference tables were last set (that call
+ *    builds the tile plan, which allocates).
+ * Every call on one GPU shares that GPU's scratch, whatever the context:
+ * calls that overlap on one GPU (two host threads, two contexts) race.  The
+ * scratch keeps the last call's contents. */
+#define GTK_NOALLOC_MAX_DOUBLES (8u * 1024u * 1024u)
+int gtk_assemble_vector_noalloc(gtk_ctx *ctx, int fkind, const double *f, int rows, double *b_out,
+                                gtk_stats *stats);
+
 /* Work plan of (form, precision) over a built (rows, cols) block, built
  * here if no assembly built it yet: the kernel chosen and its launch shape,
  * as the autotuner and the benchmarks report them.  No counterpart in the
