@@ -8,10 +8,16 @@ assembly of the stiffness values (CSC nzval, FP64 cross-cell accumulation)
 over the whole mesh, with mesh, plan and pattern already resident in HBM.
 
 Both arithmetic forms are timed: the FP64 drop-in and the mixed form
-(FP64 geometry and accumulation, FP32 per-cell contraction).  `value` is
-the mixed form; the FP64 form and the mixed-vs-FP64 error are reported
+(FP64 geometry and accumulation, split-FP16 per-cell contraction).  `value`
+is the mixed form; the FP64 form and the mixed-vs-FP64 error are reported
 beside it.  The CPU baseline is the oracle (a C restatement of the
-reference CPU loop) on a bounded sample of the same workload.
+reference CPU loop), median of five runs, single thread on a bounded sample
+and on all host threads over the whole workload.
+
+The default run also measures the other workloads and reports them as
+sub-results of the same line: `q1hex` (Q1 hex 128^3, lognormal kappa),
+`p2tet` (P2 tet elasticity 48^3 x 6 with the mixed-precision refined solve)
+and `batch` (256 uneven 2D Poisson problems in one graph launch).
 """
 import argparse
 import importlib.util
@@ -70,21 +76,50 @@ def copy_calibration(nbytes):
     print(f"copy calibration: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr)
 
 
-def cpu_baseline(n_sample, seed):
+def host_threads():
+    """Host threads for the threaded CPU baseline: OMP_NUM_THREADS when set,
+    else the CPU count, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_baseline(n_sample, n_full, seed, runs=5):
+    """The oracle on the Q2 hex workload: single thread on an n_sample^3
+    sample (the reference's path is single-threaded; the whole 64^3 mesh
+    takes about 10 s per run there), and on all host threads over the whole
+    n_full^3 workload; median of `runs` runs each, with the phase split."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker, timed here only as the CPU baseline
     GT = sys.modules["gtk_amd"]
-    m = GT.cartesian_mesh((0.0, 1.0) * 3, (n_sample,) * 3)
-    x = GT.perturb_interior_nodes(m, 0.2, seed)
-    p = O.Problem([n_sample] * 3).space(2)
-    p.set_coords(x)
-    p.quadrature(4)
-    t0 = time.perf_counter()
-    p.assemble_matrix(O.LAPLACE)
-    dt = time.perf_counter() - t0
-    return {"value": n_sample ** 3 / dt, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": f"{n_sample}^3 Q2 hex perturbed mesh ({n_sample ** 3} cells), FP64 numeric assembly + "
-                      f"COO->CSC in the reference loop order, single thread, {dt:.2f} s"}
+
+    def timed(n, threads):
+        m = GT.cartesian_mesh((0.0, 1.0) * 3, (n,) * 3)
+        x = GT.perturb_interior_nodes(m, 0.2, seed)
+        p = O.Problem([n] * 3).space(2)
+        p.set_coords(x)
+        p.quadrature(4)
+        O.set_threads(threads)
+        rec = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            p.assemble_matrix(O.LAPLACE)
+            rec.append((time.perf_counter() - t0, O.phase_ms()))
+        O.set_threads(1)
+        dt, ph = sorted(rec, key=lambda r: r[0])[len(rec) // 2]
+        return {"cells_per_s": n ** 3 / dt, "s_median": dt, "s_all": [round(r[0], 4) for r in rec],
+                "phase_ms": {k: round(v, 2) for k, v in ph.items()}, "cells": n ** 3, "threads": threads}
+
+    one = timed(n_sample, 1)
+    nt = host_threads()
+    full = timed(n_full, nt)
+    return {"value": one["cells_per_s"], "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"{n_sample}^3 Q2 hex perturbed mesh ({n_sample ** 3} cells), FP64 numeric assembly + COO->CSC "
+                      f"in the reference loop order, single thread, median of {runs} runs "
+                      f"({one['s_median']:.2f} s each; a 64^3 run takes about 10 s single-threaded)",
+            "single_thread": one,
+            "threaded": {**full, "note": f"the whole {n_full}^3 workload on {nt} host threads (cell loop and "
+                                         f"per-column compress threaded; same COO order, same result)"}}
 
 
 def alg_bytes(D, n_geo, cells, nv, nd, ncoef, nnz):
@@ -123,10 +158,12 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
     form = GT.Laplace(dO, GT.face_constant_field(kappa, Om) if lognormal_kappa else None)
     A64, cache = GT.assemble_matrix(form, float, V, V, precision=GT.FP64, reuse=True)
     ref = A64.nzval.copy()
-    Amx = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+    Amx, cache_mx = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED, reuse=True)
     rel_err = float(np.linalg.norm(Amx.nzval - ref) / np.linalg.norm(ref))
     setup_s = time.perf_counter() - t_setup
-    ctx = cache[0].ctx
+    ctx = A64.device[0].ctx
+    launch = {name: ctx.plan_info(L.GTK_LAPLACE, prec, L.GTK_FREE, L.GTK_FREE)
+              for name, prec in (("fp64", L.GTK_FP64), ("mixed", L.GTK_MIXED))}
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib << 20)
     wall64 = kern64 = wallmx = kernmx = float("nan")
@@ -148,8 +185,7 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         a_sym.ctx.close()
     # the same mixed assembly through the host boundary: launch + the values'
     # copy to host memory (PCIe), wall time of gtk_assemble_matrix
-    pcie_ms = min(GT.update_matrix(Amx, (cache[0], form, GT.FREE, GT.FREE, L.GTK_MIXED)).stats.total_ms
-                  for _ in range(3))
+    pcie_ms = min(GT.update_matrix(Amx, cache_mx).stats.total_ms for _ in range(3))
     nd = V.fe.nnodes
     nq = len(dO.w)
     headline = (order, degree, perturb, lognormal_kappa) == (2, 4, 0.2, False)
@@ -176,7 +212,7 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
         "config": {"workload": f"3D Laplace, {n}^3 Q{order} hex cells on the unit cube, {pdesc}"
                                f"{nq}-point Gauss, {kdesc}",
                    "name": label, "cells": cells, "dofs": int(V.nfree), "nnz": int(A64.nnz),
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "launch": launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (traffic or {}).get("mixed")},
@@ -220,9 +256,21 @@ def bench_elasticity(GT, args, n):
     ctx = A.device[0].ctx
     wall64, kern64 = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_FP64, args.steps, args.warmup)
     wallmx, kernmx = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_MIXED, args.steps, args.warmup)
-    # the FP32 assembly is the one solved
+    launch = {name: ctx.plan_info(L.GTK_ELASTICITY, prec, L.GTK_FREE, L.GTK_FREE)
+              for name, prec in (("fp64", L.GTK_FP64), ("mixed", L.GTK_MIXED))}
+    # the FP32 assembly is the one solved (mixed refinement to 1e-10); the
+    # FP64 solve of the FP64-assembled system to 1e-11 is the reference
     A = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
     x, st = GT.solve(A, b, precision=GT.MIXED, rtol=1e-10)
+    x64, st64 = GT.solve(A64, b, precision=GT.FP64, rtol=1e-11)
+    r64 = b - A64.to_scipy() @ x
+    sol = {"rel_err_vs_fp64_solve": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+           "rel_residual_fp64_system": float(np.linalg.norm(r64) / np.linalg.norm(b)),
+           "fp64_reference": {"rel_residual": st64.rel_residual, "inner_iterations": st64.inner_iterations,
+                              "ms": st64.solve_ms},
+           "note": "x: mixed assembly, FP32 Jacobi-PCG inside FP64 refinement to |b - A x| <= 1e-10 |b|; "
+                   "reference: FP64 assembly, FP64 Jacobi-PCG to 1e-11; the difference includes the assembly's "
+                   "FP32 rounding (rel_fro_err_vs_fp64 of the matrix) amplified by the condition number"}
     nd = 30
     bytes_alg = alg_bytes(3, mesh.coords.shape[0], cells, 4, nd, 2, A.nnz)
     achieved = bytes_alg / (kernmx * 1e-3) / 1e9
@@ -242,7 +290,7 @@ def bench_elasticity(GT, args, n):
         "config": {"workload": f"3D linear elasticity, {n}^3 cubes x 6 P2 tets, 14-point Strang, lambda and mu "
                                f"U[0.5,2] per cell (seed {args.seed}), clamped on x=0, body force (0,0,-1)",
                    "name": "p2tet", "cells": cells, "dofs": int(V.nfree), "nnz": int(A.nnz),
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "launch": launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "note": "algorithmic bytes of the mixed assembly over its kernel time; the kernel is bound by the "
@@ -251,7 +299,7 @@ def bench_elasticity(GT, args, n):
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err, "speedup_vs_fp64": kern64 / kernmx},
         "bytes_alg_per_launch": bytes_alg,
         "solve": {"rel_residual": st.rel_residual, "outer_iterations": st.outer_iterations,
-                  "inner_iterations": st.inner_iterations, "ms": st.solve_ms},
+                  "inner_iterations": st.inner_iterations, "ms": st.solve_ms, **sol},
         "setup_s": setup_s,
     }
 
@@ -376,20 +424,30 @@ def bench_batch(GT, args, count):
     return out
 
 
+SUB_DROP = ("n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "data")
+
+
+def sub_result(r):
+    """A workload's result as a sub-result of the headline line (the fields
+    the headline already states dropped)."""
+    return {k: v for k, v in r.items() if k not in SUB_DROP}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["q2hex", "q1hex", "p2tet", "batch"], default="q2hex",
-                    help="q2hex: Q2 hex 64^3 perturbed Laplace (the headline); q1hex: Q1 hex 128^3 with lognormal "
-                         "kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve; batch: 256 uneven 2D "
-                         "Poisson problems in one graph launch")
+    ap.add_argument("--workload", choices=["all", "q2hex", "q1hex", "p2tet", "batch"], default="all",
+                    help="all (default): the Q2 hex headline with the other three as sub-results of the same line; "
+                         "q2hex: Q2 hex 64^3 perturbed Laplace (the headline) alone; q1hex: Q1 hex 128^3 with "
+                         "lognormal kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve; batch: 256 "
+                         "uneven 2D Poisson problems in one graph launch")
     ap.add_argument("--problems", type=int, default=256, help="problems of the batch workload")
     ap.add_argument("--batch-cpu-stride", type=int, default=2, help="CPU baseline: every k-th problem of the batch")
     ap.add_argument("--tune-cache", default="", help="autotuner cache file (default: the user cache)")
     ap.add_argument("--cells", type=int, default=0, help="cells per direction (default per workload)")
-    ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the single-thread CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -403,13 +461,27 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     GT = load_package()
-    if args.workload == "q2hex":
+
+    def step(name):
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+
+    wl = args.workload
+    if wl in ("all", "q2hex"):
+        step("q2hex")
         out = bench_laplace(GT, args, args.cells or 64, 2, 4, 0.2, False, "q2hex")
         if not args.no_cpu_baseline:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.seed)
-    elif args.workload == "q1hex":
+            step("q2hex cpu baseline")
+            out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cells or 64, args.seed)
+        if wl == "all":
+            step("q1hex")
+            out["q1hex"] = sub_result(bench_laplace(GT, args, 128, 1, 2, 0.0, True, "q1hex"))
+            step("p2tet")
+            out["p2tet"] = sub_result(bench_elasticity(GT, args, 48))
+            step("batch")
+            out["batch"] = sub_result(bench_batch(GT, args, args.problems))
+    elif wl == "q1hex":
         out = bench_laplace(GT, args, args.cells or 128, 1, 2, 0.0, True, "q1hex")
-    elif args.workload == "p2tet":
+    elif wl == "p2tet":
         out = bench_elasticity(GT, args, args.cells or 48)
     else:
         out = bench_batch(GT, args, args.problems)

@@ -24,11 +24,23 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define INVALID 0
 #define MAXLF 12 /* max local faces of one dimension (hex edges) */
 
 static char g_err[512];
+/* threads of orc_assemble_matrix (CPU baseline at several core counts) and
+ * the wall time of its three phases: symbolic count, numeric loop, compress */
+static int g_threads = 1;
+static double g_phase_ms[3];
+int orc_set_threads(int n) { g_threads = n > 0 ? n : 1; return 0; }
+int orc_phase_ms(double *out) { memcpy(out, g_phase_ms, sizeof g_phase_ms); return 0; }
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
 const char *orc_last_error(void) { return g_err; }
 
@@ -1221,18 +1233,25 @@ int orc_assemble_matrix(orc_problem *p, int form, const double *coef, int rows, 
   tables t;
   if (make_tables(p, &t)) return -1;
   int D = p->D, ns = t.ns, nc = p->ncomp, nd = p->L.ndofs;
-  /* symbolic pass: count (allocate_matrix_barrier_impl!, assembly.jl:119-153) */
-  int64_t ncoo = 0;
+  double t0 = now_ms();
+  /* symbolic pass: count (allocate_matrix_barrier_impl!, assembly.jl:119-153);
+   * cell c's COO entries start at cell_off[c], so the COO order is cell-major
+   * whatever the thread count */
+  int64_t *cell_off = xcalloc(p->ncells + 1, sizeof(int64_t));
   for (int64_t c = 0; c < p->ncells; c++) {
     const int32_t *dofs = p->cell_dofs + c * nd;
     int64_t ni = 0, nj = 0;
     for (int a = 0; a < nd; a++) { ni += in_set(dofs[a], rows); nj += in_set(dofs[a], cols); }
-    ncoo += ni * nj;
+    cell_off[c + 1] = cell_off[c] + ni * nj;
   }
+  const int64_t ncoo = cell_off[p->ncells];
   coo *C = xcalloc(ncoo + 1, sizeof(coo));
+  double t1 = now_ms();
+#pragma omp parallel num_threads(g_threads)
+  {
   double *be = xcalloc((size_t)nd * nd, sizeof(double));
   double *gph = xcalloc((size_t)ns * 3, sizeof(double));
-  int64_t k = 0;
+#pragma omp for schedule(static)
   for (int64_t c = 0; c < p->ncells; c++) {
     const int32_t *dofs = p->cell_dofs + c * nd;
     memset(be, 0, sizeof(double) * nd * nd);
@@ -1279,6 +1298,7 @@ int orc_assemble_matrix(orc_problem *p, int form, const double *coef, int rows, 
       }
     }
     /* contribute!: trial j outer, test i inner (assembly.jl:189-208) */
+    int64_t k = cell_off[c];
     for (int j = 0; j < nd; j++) {
       if (!in_set(dofs[j], cols)) continue;
       for (int i = 0; i < nd; i++) {
@@ -1288,12 +1308,13 @@ int orc_assemble_matrix(orc_problem *p, int form, const double *coef, int rows, 
       }
     }
   }
-  free(be); free(gph); free_tables(&t);
+  free(be); free(gph);
+  }
+  free(cell_off); free_tables(&t);
+  double t2 = now_ms();
   /* compress: stable column/row order, duplicates summed in COO order */
-  int64_t nrow = rows == ORC_FREE ? p->nfree : p->ndiri;
   int64_t ncol = cols == ORC_FREE ? p->nfree : p->ndiri;
-  (void)nrow;
-  /* stable sort: counting sort by column, then merge sort rows per column */
+  /* stable sort: counting sort by column, then rows sorted per column */
   int64_t *cptr = xcalloc(ncol + 2, sizeof(int64_t));
   for (int64_t e = 0; e < ncoo; e++) cptr[C[e].j]++;
   for (int64_t j = 0; j < ncol; j++) cptr[j + 1] += cptr[j];
@@ -1305,12 +1326,14 @@ int orc_assemble_matrix(orc_problem *p, int form, const double *coef, int rows, 
     free(fill);
   }
   free(C);
-  /* insertion-sort rows inside each column keeping equal rows in COO order */
-  int64_t nnz = 0;
+  /* insertion-sort rows inside each column keeping equal rows in COO order;
+   * columns are independent */
   p->colptr = xcalloc(ncol + 1, sizeof(int32_t));
   /* cptr[j] holds the end of column j (1-based) */
   int64_t *cstart = xcalloc(ncol + 2, sizeof(int64_t));
+  int64_t *ucount = xcalloc(ncol + 2, sizeof(int64_t));
   for (int64_t j = 1; j <= ncol; j++) cstart[j] = (j == 1) ? 0 : cptr[j - 1];
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1024)
   for (int64_t j = 1; j <= ncol; j++) {
     int64_t b = cstart[j], e = cptr[j];
     for (int64_t a = b + 1; a < e; a++) {
@@ -1321,23 +1344,30 @@ int orc_assemble_matrix(orc_problem *p, int form, const double *coef, int rows, 
     }
     int64_t u = 0;
     for (int64_t a = b; a < e; a++) if (a == b || S[a].i != S[a - 1].i) u++;
-    nnz += u;
+    ucount[j] = u;
   }
+  /* ucount[j] becomes the offset of column j's first nonzero */
+  int64_t nnz = 0;
+  for (int64_t j = 1; j <= ncol; j++) { int64_t u = ucount[j]; ucount[j] = nnz; nnz += u; }
   p->rowval = xcalloc(nnz + 1, sizeof(int32_t));
   p->nzval = xcalloc(nnz + 1, sizeof(double));
-  int64_t w = 0;
   p->colptr[0] = 1;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1024)
   for (int64_t j = 1; j <= ncol; j++) {
-    int64_t b = cstart[j], e = cptr[j];
+    int64_t b = cstart[j], e = cptr[j], w = ucount[j];
     for (int64_t a = b; a < e; a++) {
       if (a == b || S[a].i != S[a - 1].i) { p->rowval[w] = S[a].i; p->nzval[w] = S[a].v; w++; }
       else p->nzval[w - 1] += S[a].v;
     }
     p->colptr[j] = (int32_t)(w + 1);
   }
-  free(S); free(cptr); free(cstart);
+  free(S); free(cptr); free(cstart); free(ucount);
   p->csc_nnz = nnz;
   p->csc_ncols = ncol;
+  double t3 = now_ms();
+  g_phase_ms[0] = t1 - t0;
+  g_phase_ms[1] = t2 - t1;
+  g_phase_ms[2] = t3 - t2;
   return 0;
 }
 

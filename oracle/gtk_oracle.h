@@ -66,6 +66,11 @@ int orc_tabulate(const orc_problem *p, int geometry, const double *xi, int npts,
  * ELASTICITY: (lambda, mu) per cell (ncells*2).  rows/cols: ORC_FREE/ORC_DIRICHLET.
  * The result is kept; fetch it with orc_get_csc. */
 int orc_assemble_matrix(orc_problem *p, int form, const double *coef, int rows, int cols);
+/* Threads of orc_assemble_matrix (default 1; the result does not depend on
+ * it) and the wall time in ms of its last call's phases: symbolic count,
+ * numeric cell loop, COO -> CSC compress. */
+int orc_set_threads(int n);
+int orc_phase_ms(double *out3);
 int orc_get_csc(const orc_problem *p, int32_t *colptr, int32_t *rowval, double *nzval);
 int64_t orc_csc_ncols(const orc_problem *p);
 

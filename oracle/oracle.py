@@ -56,6 +56,8 @@ def lib():
         L.orc_csc_ncols.argtypes = [vp]
         L.orc_assemble_vector.argtypes = [vp, C.c_int, f64p, C.c_int, f64p]
         L.orc_gauss_legendre.argtypes = [C.c_int, f64p, f64p]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_phase_ms.argtypes = [f64p]
         _lib = L
     return _lib
 
@@ -63,6 +65,19 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def set_threads(n):
+    """Threads of the matrix assembly (the result does not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def phase_ms():
+    """Wall time (ms) of the last matrix assembly's phases: symbolic count,
+    numeric cell loop, COO -> CSC compress."""
+    out = np.zeros(3)
+    lib().orc_phase_ms(out)
+    return {"symbolic": out[0], "numeric": out[1], "compress": out[2]}
 
 
 def gauss_legendre(n):
