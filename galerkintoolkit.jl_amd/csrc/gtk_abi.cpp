@@ -714,8 +714,9 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
   bool mfma = allow_mfma && gtk::find_matrix_kernel(key_e, form, precision, off_bytes, true, waves, ep->k) &&
               ep->k.mfma;
   if (mfma) {
-    // accumulator bound: a batch adds at most nd rows per slot
-    const size_t acc_bound = (size_t)waves * (gtk::kBatchSlots * nd + 64) * sizeof(double);
+    // accumulator bound: a batch adds at most nd rows per slot, and the
+    // write-out reads whole rounds of 64
+    const size_t acc_bound = (size_t)waves * (64 * ((gtk::kBatchSlots * nd + 63) / 64)) * sizeof(double);
     int max_cells = tile_cells_for(ep->k, acc_bound, ctx->lds_budget);
     if (max_cells_cap > 0) max_cells = std::max(8, std::min(max_cells, max_cells_cap));
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
@@ -735,8 +736,8 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
   if (mfma) {
     ep->lds_geo = (int)align16(ep->k.table_bytes);
     ep->acc_offset = (int)align16(ep->lds_geo + (size_t)ep->tiles.max_tile_cells * ep->k.geo_bytes_per_cell);
-    // batch entries + a private sink per lane; at least the write-out rounds
-    ep->acc_per_wave = (std::max(ep->bat.max_batch_acc + 64, 64 * ep->bat.rounds) + 1) & ~1;
+    // batch entries; at least the write-out rounds
+    ep->acc_per_wave = (std::max(ep->bat.max_batch_acc, 64 * ep->bat.rounds) + 1) & ~1;
     const size_t acc_bytes = (size_t)waves * ep->acc_per_wave * sizeof(double);
     // staged coordinates and coefficient of a cell; staging reuses the
     // accumulators and the LDS the budget leaves after them, so that a tile

@@ -51,6 +51,42 @@ __device__ inline double inverse(const double J[D][D], double Ji[D][D]) {
   }
 }
 
+template <int D>
+__device__ inline double determinant(const double J[D][D]) {
+  if constexpr (D == 2) {
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  } else {
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    return J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  }
+}
+
+// Adjugate A = det(J) J^-1 (the transposed cofactors, the same products as
+// inverse<D>); returns det J.
+template <int D>
+__device__ inline double adjugate(const double J[D][D], double A[D][D]) {
+  if constexpr (D == 2) {
+    A[0][0] = J[1][1];
+    A[0][1] = -J[0][1];
+    A[1][0] = -J[1][0];
+    A[1][1] = J[0][0];
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  } else {
+    A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    return J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+  }
+}
+
 // J at quadrature point q of `cell`; returns |det J| and J^-1.
 template <int D, int NV>
 __device__ inline double cell_jacobian(const double *__restrict__ xyz, const int32_t *__restrict__ cell_nodes,
@@ -646,16 +682,22 @@ struct BatchTile {
     return l;
   }
 
-  // reference table into LDS (rows q >= NQ zero)
-  __device__ static void table(const MatrixArgs &a, CT *tab, int tid) {
+  // reference table entry: direction d of local column b at point q (zero
+  // past the points and the directions)
+  __device__ static CT table_value(const MatrixArgs &a, int q, int b, int d) {
+    double v = 0.0;
+    if (q < NQ) {
+      if constexpr (FORM == kLaplace) v = d < D ? a.gphi[(q * D + d) * NS + b] : 0.0;
+      else v = a.phi[q * NS + b];
+    }
+    return (CT)v;
+  }
+
+  // reference table into LDS
+  __device__ static void table(const MatrixArgs &a, const Lds &l, int tid) {
     for (int i = tid; i < NQP * NS * TP; i += kB) {
       const int q = i / (NS * TP), rem = i - q * NS * TP, b = rem / TP, d = rem - b * TP;
-      double v = 0.0;
-      if (q < NQ) {
-        if constexpr (FORM == kLaplace) v = d < D ? a.gphi[(q * D + d) * NS + b] : 0.0;
-        else v = a.phi[q * NS + b];
-      }
-      tab[i] = (CT)v;
+      l.tab[i] = table_value(a, q, b, d);
     }
   }
 
@@ -712,9 +754,11 @@ struct BatchTile {
         for (int i = 0; i < D; i++)
 #pragma unroll
           for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
-      double Ji[D][D];
-      const double det = inverse<D>(J, Ji);
-      double kdV = l.K[sl] * fabs(det) * wq;
+      // Laplace: G = kappa |det J| w J^-1 J^-T = (kappa w / |det J|) A A^T
+      // with A = adj(J); mass: kappa |det J| w
+      double A[D][D];
+      const double det = FORM == kLaplace ? adjugate<D>(J, A) : determinant<D>(J);
+      double kdV = FORM == kLaplace ? l.K[sl] * wq / fabs(det) : l.K[sl] * fabs(det) * wq;
       if constexpr (kSplit) {
         // the cell's scale 2^e from kappa and its first edge h, the same in
         // every thread of the cell: |G| ~ kappa h^(D-2) w (Laplace),
@@ -747,7 +791,7 @@ struct BatchTile {
           for (int j = i; j < D; j++) {
             double gg = 0.0;
 #pragma unroll
-            for (int m = 0; m < D; m++) gg = fma(Ji[i][m], Ji[j][m], gg);
+            for (int m = 0; m < D; m++) gg = fma(A[i][m], A[j][m], gg);
             gv[sym_index<D>(i, j)] = (CT)(kdV * gg);
           }
         if constexpr (NGP > NG) gv[NGP - 1] = (CT)0;
@@ -799,7 +843,7 @@ struct BatchTile {
   // flat write-out
   __device__ static void batch(const MatrixArgs &a, const Lds &l, double *acc, const CT (&bfrag)[MB][kSplit ? 1 : KS],
                                const half8 (&bh)[MB][kSplit ? KT : 1], const half8 (&bl)[MB][kSplit ? KT : 1],
-                               const BatchRegs<OffT, MB> &cur, int lane, int sink) {
+                               const BatchRegs<OffT, MB> &cur, int lane) {
     const int g = lane >> 4;
     const uint32_t info = __builtin_amdgcn_readfirstlane(cur.info);
     const int tmax = (info >> 8) & 0xFF;
@@ -878,9 +922,8 @@ struct BatchTile {
         }
       }
     }
-    // destinations of the 4 slots x MB rows this lane holds; rows outside
-    // the row set and padding go to a private per-lane slot past the
-    // accumulators
+    // destinations of the 4 slots x MB rows this lane holds; -1: a row
+    // outside the row set or padding (not added)
     const uint32_t at4[4] = {cur.at.x, cur.at.y, cur.at.z, cur.at.w};
     int win[4];  // window of 4 ranks of each held slot (empty slots: none)
     int dst[MB][4];
@@ -891,7 +934,7 @@ struct BatchTile {
 #pragma unroll
       for (int mb = 0; mb < MB; mb++) {
         const int off = O4::get(cur.off[mb], r);
-        dst[mb][r] = off != (int)O4::none ? ab + off : sink;
+        dst[mb][r] = off != (int)O4::none ? ab + off : -1;
       }
     }
     // split product: the rows of slot 4g+r come scaled by its cell's 2^e;
@@ -911,16 +954,17 @@ struct BatchTile {
         if (win[r] == h) {
 #pragma unroll
           for (int mb = 0; mb < MB; mb++)
-            atomicAdd(&acc[dst[mb][r]], kSplit ? (double)c[mb][r] * us[r] : (double)c[mb][r]);
+            if (dst[mb][r] >= 0) atomicAdd(&acc[dst[mb][r]], kSplit ? (double)c[mb][r] * us[r] : (double)c[mb][r]);
         }
       }
     }
     // flat write-out of the batch's KR rounds of 64 accumulator entries:
     // entry e goes to e + delta[c], c = column ends before e (the padding
     // column sends the entries past the batch's columns to the value array's
-    // tail); all reads first, then the clears, then the stores, so every
-    // batch issues the same number of stores and the next batch's
-    // prefetched record is waited for without waiting for them
+    // tail); each entry is read and cleared by one LDS exchange, all
+    // exchanges before the stores, so every batch issues the same number of
+    // stores and the next batch's prefetched record is waited for without
+    // waiting for them
     int ends = 0;
     double v[KR];
     int32_t z[KR];
@@ -930,11 +974,9 @@ struct BatchTile {
       const uint32_t hi = __builtin_amdgcn_readlane(cur.em.y, k);
       const int cidx = ends + (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
       ends += __builtin_popcount(lo) + __builtin_popcount(hi);
-      v[k] = acc[k * 64 + lane];
+      v[k] = __hip_atomic_exchange(&acc[k * 64 + lane], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       z[k] = k * 64 + lane + __shfl(cur.delta, cidx);
     }
-#pragma unroll
-    for (int k = 0; k < KR; k++) acc[k * 64 + lane] = 0.0;
 #pragma unroll
     for (int k = 0; k < KR; k++) a.nzval[(int64_t)z[k]] = v[k];
   }
@@ -946,7 +988,6 @@ struct BatchTile {
   __device__ static void batches(const MatrixArgs &a, const Lds &l, int tile, int tid, F &&between) {
     const int wave = tid >> 6, lane = tid & 63;
     double *acc = l.acc + (size_t)wave * a.acc_per_wave;
-    const int sink = a.acc_per_wave - 64 + lane;
     for (int i = lane; i < a.acc_per_wave; i += 64) acc[i] = 0.0;
     CT bfrag[MB][kSplit ? 1 : KS];
     half8 bh[MB][kSplit ? KT : 1], bl[MB][kSplit ? KT : 1];
@@ -973,7 +1014,7 @@ struct BatchTile {
         load_batch<CT, OffT, NS, MB>(a.rec, a, bi, lane, nxt);
         pend = claim_issue();
       }
-      batch(a, l, acc, bfrag, bh, bl, cur, lane, sink);
+      batch(a, l, acc, bfrag, bh, bl, cur, lane);
       between();
     }
   }
@@ -1007,7 +1048,7 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
   constexpr int PF = 2;  // staging entries per thread prefetched for the next tile
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const typename BT::Lds l = BT::lds(lds_raw, args);
-  BT::table(args, l.tab, threadIdx.x);
+  BT::table(args, l, threadIdx.x);
   int32_t pc[PF], pn[PF];
   double pk[PF];
   int pre_tile = -1, npre = 0;
@@ -1079,7 +1120,7 @@ __device__ inline void batched_tile(unsigned char *raw, const MatrixArgs &a, int
   using BT = BatchTile<E::d, E::nv, E::ns, E::nq, FORM, CT, uint8_t, NW>;
   const typename BT::Lds l = BT::lds(raw, a);
   if (new_problem) {
-    BT::table(a, l.tab, threadIdx.x);
+    BT::table(a, l, threadIdx.x);
     __syncthreads();
   }
   BT::tile_simple(a, l, tile, threadIdx.x);
@@ -1091,18 +1132,6 @@ template <int N>
 struct VecCellStride {
   static constexpr int value = N % 2 ? N : N + 1;
 };
-
-template <int D>
-__device__ inline double determinant(const double J[D][D]) {
-  if constexpr (D == 2) {
-    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
-  } else {
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    return J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-  }
-}
 
 // One tile of the load vector.  Each cell's node coordinates (and, for a
 // discrete field, its nodal values) are read from HBM once into LDS; then
