@@ -122,6 +122,9 @@ struct gtk_ctx {
   // GTK_WAVES overrides
   int waves = 0;
   size_t lds_budget = 156 * 1024;  // LDS a workgroup may use (less gives several workgroups per CU)
+  // largest single device allocation the context makes (GTK_MEM_LIMIT_MB;
+  // 0: no cap): a larger request fails with GTK_ENOMEM before hipMalloc
+  size_t alloc_cap = 0;
   std::map<int, std::unique_ptr<MatrixPlan>> mplans;
   std::map<int, std::unique_ptr<VectorPlan>> vplans;
   bool has_mesh = false, has_space = false, has_ref = false;
@@ -181,11 +184,18 @@ __global__ void zero_fill_kernel(double *p, int64_t n) {
     }                                                                                       \
   } while (0)
 
+static bool over_cap(gtk_ctx *ctx, size_t bytes) {
+  if (!ctx->alloc_cap || bytes <= ctx->alloc_cap) return false;
+  ctx->err = "device allocation of " + std::to_string(bytes) + " bytes exceeds GTK_MEM_LIMIT_MB";
+  return true;
+}
+
 template <typename T>
 static int upload(gtk_ctx *ctx, DevBuf &b, const T *src, size_t count) {
   size_t bytes = count * sizeof(T);
   if (b.n < bytes || !b.p) {
     b.release();
+    if (over_cap(ctx, bytes)) return GTK_ENOMEM;
     if (bytes) {
       hipError_t e = hipMalloc(&b.p, bytes);
       if (e != hipSuccess) {
@@ -204,6 +214,7 @@ static int alloc(gtk_ctx *ctx, DevBuf &b, size_t bytes) {
   if (b.p && b.n >= bytes) return GTK_OK;
   b.release();
   if (!bytes) return GTK_OK;
+  if (over_cap(ctx, bytes)) return GTK_ENOMEM;
   hipError_t e = hipMalloc(&b.p, bytes);
   if (e != hipSuccess) {
     b.p = nullptr;
@@ -323,6 +334,10 @@ int gtk_create(int device, gtk_ctx **out) {
   if (const char *l = std::getenv("GTK_LDS_KB")) {
     const long kb = std::strtol(l, nullptr, 10);
     if (kb >= 16 && kb <= 160) ctx->lds_budget = (size_t)kb * 1024;
+  }
+  if (const char *l = std::getenv("GTK_MEM_LIMIT_MB")) {
+    const long mb = std::strtol(l, nullptr, 10);
+    if (mb > 0) ctx->alloc_cap = (size_t)mb << 20;
   }
   if (resolve_scratch(device) != GTK_OK) {
     delete ctx;

@@ -100,7 +100,7 @@ uint64_t spread2(uint64_t x) {
 
 }  // namespace
 
-std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, int nthreads) {
+std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, int nthreads, int64_t nnz_limit) {
   if (rows != 1 && rows != 2) return "rows must be GTK_FREE or GTK_DIRICHLET";
   if (cols != 1 && cols != 2) return "cols must be GTK_FREE or GTK_DIRICHLET";
   out.rows = rows;
@@ -138,7 +138,7 @@ std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, i
   out.colptr[0] = 1;
   for (int64_t j = 0; j < ncols; j++) {
     acc += len[j];
-    if (acc + 1 > INT32_MAX) return "nnz exceeds the Int32 range of SparseMatrixCSC{Float64,Int32}";
+    if (acc > nnz_limit) return "nnz exceeds the Int32 range of SparseMatrixCSC{Float64,Int32}";
     out.colptr[j + 1] = (int32_t)(acc + 1);
   }
   out.nnz = acc;

@@ -114,7 +114,10 @@ struct MeshView {
 };
 
 // Builds the CSC pattern for (rows, cols); returns an error string on failure.
-std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, int nthreads);
+// nnz_limit: the most nonzeros a pattern may hold (SparseMatrixCSC{Float64,
+// Int32} keeps nnz + 1 in its Int32 colptr); beyond it an error.
+std::string build_pattern(const MeshView &m, int rows, int cols, Pattern &out, int nthreads,
+                          int64_t nnz_limit = INT32_MAX - 1);
 // Builds the tile plan over set S with at most max_cells cells, max_cols
 // owned DOFs and max_ctr contributors per tile.
 std::string build_tiles(const MeshView &m, int set, int max_cells, int max_cols, int max_ctr, int chunk,

@@ -1,0 +1,381 @@
+"""
+    GTKHip
+
+Julia binding of `libgtk_hip.so` (the C ABI in `include/gtk_hip.h`) for
+GalerkinToolkit.jl v0.6.3.  It replaces the generated per-cell loop of
+`GT.assemble_matrix` / `GT.assemble_vector` (src/problems.jl:244-350,
+loop templates src/compiler.jl:1826-1997) and the COO/CSC allocation
+protocol (src/assembly.jl:75-208, :428-588) for the integrands the library
+recognises, and returns the same `SparseMatrixCSC{Float64,Int32}` and
+`Vector{Float64}`.
+
+The per-cell integrand of a GT weak form is a closure the binding cannot
+inspect, so the caller names it with a form descriptor (`Laplace`, `Mass`,
+`Elasticity`, `Load`), exactly as in the library's Python mirror.  Through
+the reference's own keyword seam, `assembly_method = (; matrix = HipAssembly(form))`
+(src/assembly.jl:3-25), `GTKHip.assemble_matrix` takes the device path;
+without a `HipAssembly` it calls `GT.assemble_matrix` unchanged.
+
+A device context is created once per (space, measure) and keeps the mesh,
+the cell -> DOF map, the reference tables and, per (rows, cols) block, the
+CSC pattern and the work plan on the GPU; later assemblies of any form on
+that block only refill values (the `reuse`/`update_matrix!` path of
+src/problems.jl:352-361).
+"""
+module GTKHip
+
+import GalerkinToolkit as GT
+import ForwardDiff
+using SparseArrays
+using LinearAlgebra
+
+const lib = get(ENV, "GTK_HIP_LIB", joinpath(@__DIR__, "..", "galerkintoolkit.jl_amd", "libgtk_hip.so"))
+
+const FREE, DIRICHLET = Cint(1), Cint(2)
+const LAPLACE, MASS, ELASTICITY = Cint(0), Cint(1), Cint(2)
+const FP64, MIXED = Cint(0), Cint(1)
+const F_CONST, F_CELL, F_NODAL = Cint(0), Cint(1), Cint(2)
+
+struct Stats
+    kernel_ms::Float64
+    total_ms::Float64
+    cells::Int64
+    nnz::Int64
+    bytes_alg::Int64
+    flops_alg::Int64
+end
+
+struct SolveStats
+    outer_iterations::Cint
+    inner_iterations::Cint
+    converged::Cint
+    rel_residual::Float64
+    solve_ms::Float64
+end
+
+# ---------------------------------------------------------------------------
+# context: the C ABI, one call each
+# ---------------------------------------------------------------------------
+
+mutable struct Ctx
+    h::Ptr{Cvoid}
+    patterns::Dict{Tuple{Cint,Cint},Tuple{Vector{Int32},Vector{Int32}}}
+    function Ctx(device::Integer=0)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:gtk_create, lib), Cint, (Cint, Ref{Ptr{Cvoid}}), device, r)
+        rc == 0 || error("gtk_create ($rc): ", unsafe_string(ccall((:gtk_last_error, lib), Cstring, (Ptr{Cvoid},), C_NULL)))
+        c = new(r[], Dict{Tuple{Cint,Cint},Tuple{Vector{Int32},Vector{Int32}}}())
+        finalizer(c) do x
+            x.h == C_NULL || ccall((:gtk_destroy, lib), Cint, (Ptr{Cvoid},), x.h)
+            x.h = C_NULL
+        end
+    end
+end
+
+function check(c::Ctx, rc, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:gtk_last_error, lib), Cstring, (Ptr{Cvoid},), c.h))
+    error("$what failed ($rc): $msg")
+end
+
+function set_mesh!(c::Ctx, mesh)
+    D = GT.num_dims(mesh)
+    x = collect(reinterpret(Float64, GT.node_coordinates(mesh)))   # node-major, D per node
+    cells = GT.face_nodes(mesh, D)                                  # JaggedArray{Int32}, 1-based
+    npc = length(cells[1])
+    check(c, ccall((:gtk_set_mesh, lib), Cint, (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Int64, Cint, Ptr{Int32}),
+                   c.h, D, length(GT.node_coordinates(mesh)), x, length(cells), npc, cells.data), "gtk_set_mesh")
+end
+
+function set_space!(c::Ctx, V)
+    dofs = GT.face_dofs(V)                                          # JaggedArray{Int32}, GT signed encoding
+    ndpc = length(dofs[1])
+    ts = GT.tensor_size(V)
+    ncomp = ts === :scalar ? 1 : prod(ts)
+    check(c, ccall((:gtk_set_space, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Int32}),
+                   c.h, ncomp, ndpc, GT.num_free_dofs(V), GT.num_dirichlet_dofs(V), dofs.data), "gtk_set_space")
+end
+
+# Reference tables at the points of the measure: w[nq], phi[nq*ns],
+# grad_phi[nq*ns*D], grad_geo[nq*nv*D], point-major (tabulator,
+# src/space.jl:960-970).
+function set_reference!(c::Ctx, V, dΩ)
+    mesh = GT.mesh(GT.domain(dΩ))
+    D = GT.num_dims(mesh)
+    q = first(GT.reference_quadratures(dΩ))
+    x, w = GT.coordinates(q), GT.weights(q)
+    fe = first(GT.reference_spaces(V))
+    geo = first(GT.reference_spaces(mesh, D))
+    tab, tgeo = GT.tabulator(fe), GT.tabulator(geo)
+    phi = collect(permutedims(tab(GT.value, x)))                  # ns x nq -> point-major
+    gphi = tab(ForwardDiff.gradient, x)                             # nq x ns of SVector{D}
+    ggeo = tgeo(ForwardDiff.gradient, x)
+    nq, ns, nv = length(w), size(gphi, 2), size(ggeo, 2)
+    grad_phi = [gphi[iq, s][d] for d in 1:D, s in 1:ns, iq in 1:nq]
+    grad_geo = [ggeo[iq, n][d] for d in 1:D, n in 1:nv, iq in 1:nq]
+    check(c, ccall((:gtk_set_reference, lib), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   c.h, nq, collect(Float64, w), ns, vec(phi), vec(grad_phi), vec(grad_geo)), "gtk_set_reference")
+end
+
+function set_cell_coefficients!(c::Ctx, coef, ncoef)
+    if coef === nothing
+        check(c, ccall((:gtk_set_cell_coefficients, lib), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), c.h, 0, C_NULL),
+              "gtk_set_cell_coefficients")
+    else
+        v = collect(Float64, coef)
+        check(c, ccall((:gtk_set_cell_coefficients, lib), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), c.h, ncoef, v),
+              "gtk_set_cell_coefficients")
+    end
+end
+
+# The CSC pattern of block (rows, cols), built once per context.
+function pattern!(c::Ctx, rows::Cint, cols::Cint)
+    get!(c.patterns, (rows, cols)) do
+        nnz, ncols = Ref{Int64}(0), Ref{Int64}(0)
+        check(c, ccall((:gtk_build_pattern, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Ref{Int64}, Ref{Int64}),
+                       c.h, rows, cols, nnz, ncols), "gtk_build_pattern")
+        colptr = Vector{Int32}(undef, ncols[] + 1)
+        rowval = Vector{Int32}(undef, nnz[])
+        check(c, ccall((:gtk_get_pattern, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Int32}, Ptr{Int32}),
+                       c.h, rows, cols, colptr, rowval), "gtk_get_pattern")
+        (colptr, rowval)
+    end
+end
+
+function assemble_values!(c::Ctx, nzval::Vector{Float64}, form::Cint, precision::Cint, rows::Cint, cols::Cint)
+    st = Ref{Stats}()
+    check(c, ccall((:gtk_assemble_matrix, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Float64}, Ref{Stats}),
+                   c.h, form, precision, rows, cols, nzval, st), "gtk_assemble_matrix")
+    st[]
+end
+
+function assemble_load!(c::Ctx, b::Vector{Float64}, fkind::Cint, f::Vector{Float64}, rows::Cint; noalloc=false)
+    st = Ref{Stats}()
+    sym = noalloc ? :gtk_assemble_vector_noalloc : :gtk_assemble_vector
+    rc = noalloc ?
+         ccall((:gtk_assemble_vector_noalloc, lib), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Ref{Stats}),
+               c.h, fkind, f, rows, b, st) :
+         ccall((:gtk_assemble_vector, lib), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Ref{Stats}),
+               c.h, fkind, f, rows, b, st)
+    check(c, rc, String(sym))
+    st[]
+end
+
+# b -= A_(rows, DIRICHLET) xd on the device (src/problems.jl:439-452)
+function lift_dirichlet!(c::Ctx, b::Vector{Float64}, xd::Vector{Float64}; rows=FREE)
+    check(c, ccall((:gtk_lift_dirichlet, lib), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}), c.h, rows, xd, b),
+          "gtk_lift_dirichlet")
+    b
+end
+
+# ---------------------------------------------------------------------------
+# forms the library recognises (the integrands of src/compiler.jl:1544-1606
+# the supported weak forms lower to)
+# ---------------------------------------------------------------------------
+
+abstract type Form end
+"∫ κ ∇u⋅∇v dΩ, κ per cell (a vector over the cells of Ω) or `nothing` for 1"
+struct Laplace <: Form
+    dΩ
+    kappa
+end
+Laplace(dΩ) = Laplace(dΩ, nothing)
+"∫ κ u v dΩ"
+struct Mass <: Form
+    dΩ
+    kappa
+end
+Mass(dΩ) = Mass(dΩ, nothing)
+"∫ σ(ε(u)):ε(v) dΩ, σ = λ tr(ε) I + 2μ ε, λ and μ per cell"
+struct Elasticity <: Form
+    dΩ
+    lambda
+    mu
+end
+"∫ f⋅v dΩ: f a constant (one value per component), per-cell values or a GT.DiscreteField"
+struct Load <: Form
+    dΩ
+    f
+end
+
+form_code(::Laplace) = LAPLACE
+form_code(::Mass) = MASS
+form_code(::Elasticity) = ELASTICITY
+coefficients(a::Union{Laplace,Mass}) = (a.kappa, 1)
+coefficients(a::Elasticity) = (vec(permutedims(hcat(collect(a.lambda), collect(a.mu)))), 2)
+
+"The assembly method that selects the device path: `assembly_method=(; matrix=HipAssembly(form))`."
+struct HipAssembly
+    form::Form
+    precision::Cint
+end
+HipAssembly(form; precision=FP64) = HipAssembly(form, precision)
+
+# device contexts, one per (space, measure), reused by every assembly on them
+const CONTEXTS = IdDict{Any,Ctx}()
+
+function context(V, dΩ; device=0)
+    get!(CONTEXTS, (V, dΩ)) do
+        c = Ctx(device)
+        set_mesh!(c, GT.mesh(GT.domain(dΩ)))
+        set_space!(c, V)
+        set_reference!(c, V, dΩ)
+        c
+    end
+end
+
+num_rows(V, rows) = rows == FREE ? GT.num_free_dofs(V) : GT.num_dirichlet_dofs(V)
+
+"""
+    assemble_matrix(a::Form, Float64, U, V; free_or_dirichlet=(FREE, FREE), precision=FP64, reuse=false)
+
+`GT.assemble_matrix` of the form on the device (src/problems.jl:319-350):
+same `SparseMatrixCSC{Float64,Int32}`; `reuse=true` also returns the cache
+of `update_matrix!`.
+"""
+function assemble_matrix(a::Form, ::Type{Float64}, U, V; free_or_dirichlet=(FREE, FREE), precision=FP64,
+                         reuse=false, device=0)
+    U === V || error("trial and test spaces must be the same space")
+    rows, cols = Cint.(free_or_dirichlet)
+    c = context(V, a.dΩ; device)
+    colptr, rowval = pattern!(c, rows, cols)
+    coef, ncoef = coefficients(a)
+    set_cell_coefficients!(c, coef, ncoef)
+    nzval = Vector{Float64}(undef, length(rowval))
+    assemble_values!(c, nzval, form_code(a), Cint(precision), rows, cols)
+    A = SparseMatrixCSC(num_rows(V, rows), length(colptr) - 1, colptr, rowval, nzval)  # layout of assembly.jl:853
+    reuse ? (A, (; c, a, rows, cols, precision)) : A
+end
+
+"""
+    update_matrix!(A, cache; form=cache.a)
+
+`GT.update_matrix!` (src/problems.jl:352-361): new coefficients (a form of
+the same kind over the same measure), same pattern, values refilled in place.
+"""
+function update_matrix!(A::SparseMatrixCSC, cache; form=cache.a)
+    coef, ncoef = coefficients(form)
+    set_cell_coefficients!(cache.c, coef, ncoef)
+    assemble_values!(cache.c, A.nzval, form_code(form), Cint(cache.precision), cache.rows, cache.cols)
+    A
+end
+
+function load_data(l::Load, V)
+    f = l.f
+    if f isa GT.DiscreteField
+        return F_NODAL, vcat(collect(Float64, GT.free_values(f)), collect(Float64, GT.dirichlet_values(f)))
+    elseif f isa AbstractVector && length(f) == GT.num_faces(GT.domain(l.dΩ))
+        return F_CELL, collect(Float64, f)
+    else
+        return F_CONST, collect(Float64, f isa Number ? (f,) : f)
+    end
+end
+
+"`GT.assemble_vector` of a load on the device (src/problems.jl:244-274)."
+function assemble_vector(l::Load, ::Type{Float64}, V; free_or_dirichlet=FREE, device=0, noalloc=false)
+    rows = Cint(free_or_dirichlet)
+    c = context(V, l.dΩ; device)
+    b = Vector{Float64}(undef, num_rows(V, rows))
+    kind, f = load_data(l, V)
+    assemble_load!(c, b, kind, f, rows; noalloc)
+    b
+end
+
+"""
+    solve!(x, A, b; precision=MIXED, rtol=1e-10)
+
+Solve with the `(FREE, FREE)` values the device holds for A's context (the
+place of PartitionedSolvers' solve, src/problems.jl:439-462): FP32
+Jacobi-preconditioned CG inside FP64 iterative refinement (MIXED) or FP64 CG.
+If A's values were changed on the host, `set_values!` hands them back first.
+"""
+function solve!(x::Vector{Float64}, c::Ctx, b::Vector{Float64}; precision=MIXED, rtol=1e-10, max_outer=30,
+                max_inner=20000, inner_rtol=1e-4)
+    st = Ref{SolveStats}()
+    check(c, ccall((:gtk_solve, lib), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Float64, Cint, Cint, Float64, Ref{SolveStats}),
+                   c.h, precision, b, x, rtol, max_outer, max_inner, inner_rtol, st), "gtk_solve")
+    st[].converged == 1 || error("solve did not reach rtol=$rtol: relative residual $(st[].rel_residual)")
+    x
+end
+
+function set_values!(c::Ctx, A::SparseMatrixCSC; rows=FREE, cols=FREE)
+    check(c, ccall((:gtk_set_values, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}), c.h, rows, cols, A.nzval),
+          "gtk_set_values")
+end
+
+"""
+    linear_problem(uhd, a::Form, l::Load; precision=FP64)
+
+The system of `GT.PartitionedSolvers_linear_problem(uhd, a, l)`
+(src/problems.jl:439-452): A = a on (FREE, FREE), b = l - A_(FREE,DIRICHLET) xd
+with the Dirichlet values of `uhd`, the lifting done on the device.
+"""
+function linear_problem(uhd, a::Form, l::Load; precision=FP64, device=0)
+    V = GT.space(uhd)
+    A = assemble_matrix(a, Float64, V, V; precision, device)
+    c = context(V, a.dΩ; device)
+    pattern!(c, FREE, DIRICHLET)
+    check(c, ccall((:gtk_assemble_matrix, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Float64}, Ptr{Stats}),
+                   c.h, form_code(a), Cint(precision), FREE, DIRICHLET, C_NULL, C_NULL), "gtk_assemble_matrix")  # values stay on the device
+    b = assemble_vector(l, Float64, V; device)
+    lift_dirichlet!(c, b, collect(Float64, GT.dirichlet_values(uhd)))
+    A, b
+end
+
+"""
+    assemble_matrix(f, Float64, U, V; assembly_method=(;), kwargs...)
+
+The reference's entry point with its keyword seam: `assembly_method = (;
+matrix = HipAssembly(form))` selects the device path for the named form
+(the closure `f` is then not evaluated); any other method goes to
+`GT.assemble_matrix` unchanged.
+"""
+function assemble_matrix(f, ::Type{Float64}, U, V; assembly_method=(;), kwargs...)
+    m = get(assembly_method, :matrix, nothing)
+    if m isa HipAssembly
+        fd = get(kwargs, :free_or_dirichlet, (GT.FREE, GT.FREE))
+        code(x) = x == GT.FREE ? FREE : DIRICHLET
+        return assemble_matrix(m.form, Float64, U, V; free_or_dirichlet=(code(fd[1]), code(fd[2])),
+                               precision=m.precision, reuse=get(kwargs, :reuse, false))
+    end
+    GT.assemble_matrix(f, Float64, U, V; assembly_method, kwargs...)
+end
+
+"""
+    assemble_batch!(ctxs, loads; form=LAPLACE, precision=MIXED, waves=12, tiles=(0, 256, 0))
+
+Many independent problems (different meshes, orders and coefficient fields)
+assembled by one graph-replayed launch, in place of a loop of
+`GT.assemble_matrix` / `GT.assemble_vector` calls (no counterpart in the
+reference).  Each context has its (FREE, FREE) pattern, reference tables and
+coefficients set; `loads[i]` are problem i's nodal load values (free then
+Dirichlet) or `nothing`.  The values land in each context
+(`gtk_get_values`); returns the batch handle (`gtk_batch_run` replays it,
+`gtk_batch_destroy` frees it).
+"""
+function assemble_batch!(ctxs::Vector{Ctx}, loads; form=LAPLACE, precision=MIXED, waves=12, tiles=(0, 256, 0),
+                         device=0)
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    ccall((:gtk_batch_create, lib), Cint, (Cint, Cint, Cint, Ref{Ptr{Cvoid}}), device, form, precision, b) == 0 ||
+        error("gtk_batch_create failed")
+    bcheck(rc, what) = rc == 0 ||
+        error("$what failed ($rc): ", unsafe_string(ccall((:gtk_batch_last_error, lib), Cstring, (Ptr{Cvoid},), b[])))
+    for (i, c) in enumerate(ctxs)
+        bcheck(ccall((:gtk_batch_add, lib), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b[], c.h), "gtk_batch_add")
+        f = loads[i]
+        f === nothing || bcheck(ccall((:gtk_batch_set_load, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}),
+                                      b[], i - 1, F_NODAL, collect(Float64, f)), "gtk_batch_set_load")
+    end
+    bcheck(ccall((:gtk_batch_set_waves, lib), Cint, (Ptr{Cvoid}, Cint), b[], waves), "gtk_batch_set_waves")
+    for (t, cells) in enumerate(tiles)  # Q1, Q2, Q3 quadrilaterals: cells per tile, 0 = as many as fit in LDS
+        bcheck(ccall((:gtk_batch_set_tile_cells, lib), Cint, (Ptr{Cvoid}, Cint, Cint), b[], t - 1, cells),
+               "gtk_batch_set_tile_cells")
+    end
+    bcheck(ccall((:gtk_batch_build, lib), Cint, (Ptr{Cvoid},), b[]), "gtk_batch_build")
+    bcheck(ccall((:gtk_batch_run, lib), Cint, (Ptr{Cvoid}, Ptr{Stats}), b[], C_NULL), "gtk_batch_run")
+    b[]
+end
+
+end # module

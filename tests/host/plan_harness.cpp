@@ -198,3 +198,21 @@ extern "C" int harness_transpose(int64_t n, const int32_t *colptr, const int32_t
   std::memcpy(perm, out.data(), out.size() * sizeof(int32_t));
   return 0;
 }
+
+// The symbolic phase alone with a given nonzero limit: 0 and nnz, or -1 and
+// the error (the Int32 overflow check of the pattern, forced with a small
+// limit).
+extern "C" int harness_pattern(int dim, int64_t n_nodes, const double *xyz, int64_t n_cells, int nv,
+                               const int32_t *cell_nodes0, int nd, int64_t n_free, int64_t n_diri,
+                               const int32_t *cell_dofs, int rows, int cols, int64_t nnz_limit, int64_t *nnz_out,
+                               char *err, int errlen) {
+  gtk::MeshView m;
+  m.dim = dim; m.n_nodes = n_nodes; m.n_cells = n_cells; m.nodes_per_cell = nv;
+  m.xyz = xyz; m.cell_nodes = cell_nodes0; m.cell_dofs = cell_dofs; m.dofs_per_cell = nd;
+  m.n_free = n_free; m.n_diri = n_diri;
+  gtk::Pattern p;
+  std::string e = gtk::build_pattern(m, rows, cols, p, 4, nnz_limit);
+  if (!e.empty()) { std::strncpy(err, e.c_str(), errlen - 1); return -1; }
+  *nnz_out = p.nnz;
+  return 0;
+}

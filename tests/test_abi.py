@@ -31,3 +31,17 @@ def test_errors_without_device(GT):
     assert rc != 0
     assert L.gtk_last_error(None)
     assert not h.value
+
+
+def test_julia_binding_matches_header():
+    """julia/GTKHip.jl (the binding a GalerkinToolkit.jl caller loads; no
+    Julia in this image, so it is checked as text) calls only entry points
+    the header declares, and binds the ones its assembly path needs."""
+    src = open(os.path.join(ROOT, "julia", "GTKHip.jl")).read()
+    called = set(re.findall(r"ccall\(\(:(gtk_[a-z_]+),\s*lib\)", src))
+    names = set(declared())
+    assert called and called <= names, sorted(called - names)
+    for need in ("gtk_create", "gtk_destroy", "gtk_last_error", "gtk_set_mesh", "gtk_set_space", "gtk_set_reference",
+                 "gtk_build_pattern", "gtk_get_pattern", "gtk_set_cell_coefficients", "gtk_assemble_matrix",
+                 "gtk_assemble_vector", "gtk_lift_dirichlet", "gtk_solve", "gtk_set_values"):
+        assert need in called, need

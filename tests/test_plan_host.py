@@ -154,3 +154,27 @@ def test_duplicate_cell_dof_rejected(harness, GT, mode):
                              cd.shape[1], V.nfree, V.ndiri, cd, 1, 1, 64, 8, C.byref(C.c_int64()), colptr,
                              np.zeros(cap, np.int32), np.zeros(cap), cap, C.byref(C.c_int()), mode, err, 256)
     assert rc == -1 and b"twice" in err.value, (rc, err.value)
+
+
+def test_nnz_over_int32_rejected(harness, GT):
+    """The pattern's nonzero count must fit SparseMatrixCSC{Float64,Int32}:
+    the symbolic phase refuses a pattern past its limit (forced here with a
+    limit one below this pattern's nnz) instead of wrapping colptr."""
+    m = GT.cartesian_mesh((0.0, 1.0) * 2, (4, 3))
+    V = GT.lagrange_space(GT.interior(m), 2)
+    cd = np.ascontiguousarray(V.cell_dofs, np.int32)
+    cn = np.ascontiguousarray(m.cells, np.int32)
+    H = harness
+    i32 = np.ctypeslib.ndpointer(np.int32, flags="C")
+    H.harness_pattern.argtypes = [C.c_int, C.c_int64, np.ctypeslib.ndpointer(np.float64, flags="C"), C.c_int64,
+                                  C.c_int, i32, C.c_int, C.c_int64, C.c_int64, i32, C.c_int, C.c_int, C.c_int64,
+                                  C.POINTER(C.c_int64), C.c_char_p, C.c_int]
+    args = (2, m.coords.shape[0], np.ascontiguousarray(m.coords), cn.shape[0], cn.shape[1], cn, cd.shape[1], V.nfree,
+            V.ndiri, cd, 1, 1)
+    nnz, err = C.c_int64(), C.create_string_buffer(256)
+    assert H.harness_pattern(*args, 2**31 - 2, C.byref(nnz), err, 256) == 0
+    ref = coo_reference(V.cell_dofs, 1, 1, V.nfree, V.nfree)
+    assert nnz.value == ref.nnz
+    assert H.harness_pattern(*args, nnz.value, C.byref(C.c_int64()), err, 256) == 0
+    assert H.harness_pattern(*args, nnz.value - 1, C.byref(C.c_int64()), err, 256) == -1
+    assert b"Int32" in err.value
