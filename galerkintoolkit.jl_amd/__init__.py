@@ -6,7 +6,7 @@ GT.assemble_matrix/assemble_vector, GT.update_matrix!) whose numeric work runs
 in libgtk_hip.so (HIP kernels for gfx950 behind the C ABI of include/gtk_hip.h).
 """
 from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, FormSum, Laplace, Load, Mass, Measure,
-                       MatrixCache, SparseMatrixCSC, VectorCache, assemble_matrix, assemble_matrix_and_vector,
+                       MatrixCache, Nitsche, NitscheLoad, SparseMatrixCSC, VectorCache, assemble_matrix, assemble_matrix_and_vector,
                        assemble_matrix_and_vector_with_free_and_dirichlet_columns,
                        assemble_matrix_with_free_and_dirichlet_columns, assemble_vector, face_constant_field,
                        linear_problem, measure, solve, update_matrix, update_matrix_and_vector,
@@ -19,7 +19,7 @@ from . import inputs, refcell
 
 __all__ = [
     "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "FormSum", "Laplace",
-    "Load",
+    "Load", "Nitsche", "NitscheLoad",
     "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
     "assemble_vector", "face_constant_field", "linear_problem", "measure", "solve", "update_matrix", "Mesh",
     "MatrixCache", "VectorCache", "assemble_matrix_and_vector",

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libgtk_hip.so")
 
 GTK_FREE, GTK_DIRICHLET = 1, 2
-GTK_LAPLACE, GTK_MASS, GTK_ELASTICITY = 0, 1, 2
+GTK_LAPLACE, GTK_MASS, GTK_ELASTICITY, GTK_NITSCHE, GTK_NITSCHE_LOAD = 0, 1, 2, 3, 4
 GTK_FP64, GTK_MIXED = 0, 1
 GTK_F_CONST, GTK_F_CELL, GTK_F_NODAL = 0, 1, 2
 GTK_NOALLOC_MAX_DOUBLES = 8 * 1024 * 1024  # scratch of gtk_assemble_vector_noalloc
@@ -25,7 +25,7 @@ EXPORTS = [
     "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_set_values", "gtk_values_device_ptr", "gtk_assemble_vector",
     "gtk_assemble_vector_noalloc",
-    "gtk_plan_info", "gtk_lift_dirichlet",
+    "gtk_plan_info", "gtk_lift_dirichlet", "gtk_block_multiply", "gtk_set_reference_face_gradients",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
     "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_waves", "gtk_batch_set_load", "gtk_batch_get_vector",
     "gtk_batch_build", "gtk_batch_run",
@@ -93,6 +93,8 @@ def load():
     L.gtk_get_values.argtypes = [vp, i, i, f64]
     L.gtk_set_values.argtypes = [vp, i, i, f64]
     L.gtk_lift_dirichlet.argtypes = [vp, i, f64, f64]
+    L.gtk_block_multiply.argtypes = [vp, i, i, C.c_double, f64, f64]
+    L.gtk_set_reference_face_gradients.argtypes = [vp, f64, f64]
     L.gtk_plan_info.argtypes = [vp, i, i, i, i, C.POINTER(GtkPlanInfo)]
     L.gtk_values_device_ptr.argtypes = [vp, i, i, C.POINTER(C.c_void_p)]
     L.gtk_assemble_vector.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
@@ -168,6 +170,18 @@ class Context:
                                                   np.ascontiguousarray(grad_geo, np.float64),
                                                   np.ascontiguousarray(face_jac, np.float64),
                                                   np.ascontiguousarray(item_face, np.int32)), "gtk_set_reference_faces")
+
+    def set_reference_face_gradients(self, grad_phi, normals):
+        self.check(self.L.gtk_set_reference_face_gradients(self.h, np.ascontiguousarray(grad_phi, np.float64),
+                                                           np.ascontiguousarray(normals, np.float64)),
+                   "gtk_set_reference_face_gradients")
+
+    def block_multiply(self, rows, cols, x, y, alpha=1.0):
+        """y + alpha A x for the (rows, cols) block, on the device."""
+        out = np.array(y, np.float64, copy=True)
+        self.check(self.L.gtk_block_multiply(self.h, rows, cols, float(alpha), np.ascontiguousarray(x, np.float64),
+                                             out), "gtk_block_multiply")
+        return out
 
     def add_block(self, rows, cols, src):
         self.check(self.L.gtk_add_block(self.h, rows, cols, src.h), "gtk_add_block")

@@ -117,6 +117,27 @@ class Load(_Form):
     f: object = 1.0
 
 
+@dataclass
+class Nitsche(_Form):
+    """int (gamma/h) v u - v n.grad(u) - n.grad(v) u dGamma over boundary
+    faces: the Nitsche terms of GalerkinToolkitExamples/src/poisson.jl:93-107
+    (scalar fields), h the face's diameter (largest distance between its
+    nodes; GT.face_diameter_field, src/field.jl:488-492,
+    src/accessors.jl:907-921), n the outward unit normal
+    (src/accessors.jl:1009-1030)."""
+    dOmega: Measure
+    gamma: float = 1.0
+
+
+@dataclass
+class NitscheLoad(_Form):
+    """int (gamma/h) v g - n.grad(v) g dGamma (poisson.jl:108-111), g a
+    DiscreteField of the space (its free then Dirichlet values)."""
+    dOmega: Measure
+    g: object
+    gamma: float = 1.0
+
+
 class SparseMatrixCSC:
     """Julia's SparseMatrixCSC{Float64,Int32} layout: 1-based colptr/rowval."""
 
@@ -218,12 +239,76 @@ class _BoundaryAssembler(_Assembler):
         self.values_owner = {}
 
 
-def _assembler(space, dOmega, device):
-    key = (id(dOmega), device)
+class _NitscheAssembler(_Assembler):
+    """The device context of Nitsche terms over a boundary measure: one item
+    per face of the domain with all DOFs of its cell (normal derivatives
+    involve every shape function), the cell's shape functions, reference
+    gradients and order-1 geometry gradients at the face rule's points mapped
+    into the cell, per local face, the reference outward normals, and the
+    face diameters."""
+
+    def __init__(self, space, dGamma, device=0):
+        mesh = space.mesh
+        D = mesh.D
+        if space.ncomp != 1:
+            raise NotImplementedError("Nitsche terms: scalar fields")
+        self.space = space
+        self.dOmega = dGamma
+        faces = dGamma.domain.faces
+        cf = mesh.cell_faces[D - 1]
+        nlf = cf.shape[1]
+        owner = np.full(mesh.num_faces(D - 1), -1, np.int64)
+        owner[cf.ravel()] = np.arange(cf.size)
+        hit = owner[faces]
+        if np.any(hit < 0):
+            raise ValueError("a face of the boundary domain has no cell")
+        cells, lface = hit // nlf, hit % nlf
+        g = R.GEO[mesh.cellkind]
+        ft = g.ftype[D - 1]
+        N = R.linear_shape(ft, dGamma.x)
+        _, dNf = R.Lagrange(ft, 1).tabulate(dGamma.x[:1])
+        geo = R.Lagrange(mesh.cellkind, 1)
+        fe = space.fe
+        phi, gphi, gN, A = [], [], [], []
+        for fv in g.faces[D - 1]:
+            X = np.asarray([g.verts[v - 1] for v in fv], np.float64)
+            xc = N @ X
+            val, grad = fe.tabulate(xc)
+            phi.append(val)
+            gphi.append(grad)
+            gN.append(geo.tabulate(xc)[1])
+            A.append(X.T @ dNf[0])
+        # face diameters: largest distance between the face's nodes
+        fn = np.asarray(g.faces[D - 1])[lface] - 1  # (items, face vertices) local vertices
+        Xf = mesh.coords[np.take_along_axis(mesh.cells[cells], fn, axis=1)]
+        diff = Xf[:, :, None, :] - Xf[:, None, :, :]
+        self.h = np.sqrt((diff ** 2).sum(axis=3)).max(axis=(1, 2))
+        self.ctx = L.Context(device)
+        self.ctx.set_mesh(D, mesh.coords, mesh.cell_nodes_1b()[cells])
+        self.ctx.set_space(1, space.cell_dofs[cells], space.nfree, space.ndiri)
+        self.ctx.set_reference_faces(dGamma.w, np.stack(phi), np.stack(gN), np.stack(A), lface)
+        self.ctx.set_reference_face_gradients(np.stack(gphi), R.face_normals(mesh.cellkind))
+        self.items = len(faces)
+        self.patterns = {}
+        self.values_owner = {}
+
+    def beta(self, gamma):
+        """gamma / h per item."""
+        return np.ascontiguousarray(float(gamma) / self.h)
+
+
+def _assembler(space, dOmega, device, kind=None):
+    """The context of (space, measure); kind "nitsche" (or "nitsche_load",
+    a second one so the load's matrix never replaces the Nitsche matrix's
+    device values) for Nitsche items over a boundary measure."""
+    key = (id(dOmega), device, kind)
     cache = space.__dict__.setdefault("_assemblers", {})
     a = cache.get(key)
     if a is None or a.dOmega is not dOmega:
-        a = (_BoundaryAssembler if dOmega.on_boundary else _Assembler)(space, dOmega, device)
+        if kind is not None:
+            a = _NitscheAssembler(space, dOmega, device)
+        else:
+            a = (_BoundaryAssembler if dOmega.on_boundary else _Assembler)(space, dOmega, device)
         cache[key] = a
     return a
 
@@ -235,6 +320,8 @@ def _form_code(form):
         return L.GTK_MASS
     if isinstance(form, Elasticity):
         return L.GTK_ELASTICITY
+    if isinstance(form, Nitsche):
+        return L.GTK_NITSCHE
     raise TypeError("unsupported bilinear form")
 
 
@@ -309,8 +396,17 @@ def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic
     """(assembler of the block's device values, nzval, stats, pattern)."""
     if isinstance(form, FormSum):
         return _assemble_matrix_sum(form, space, rows, cols, precision, device, symbolic)
-    if form.dOmega.on_boundary and not isinstance(form, Mass):
-        raise NotImplementedError("boundary integrals of matrices: the mass form (Robin terms)")
+    if form.dOmega.on_boundary and not isinstance(form, (Mass, Nitsche)):
+        raise NotImplementedError("boundary integrals of matrices: the mass form (Robin terms) and Nitsche terms")
+    if isinstance(form, Nitsche):
+        if not form.dOmega.on_boundary:
+            raise ValueError("Nitsche terms are integrals over boundary faces")
+        a = _assembler(space, form.dOmega, device, "nitsche")
+        pat = a.pattern(rows, cols, symbolic)
+        a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
+        prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+        nzval, stats = a.ctx.assemble_matrix(L.GTK_NITSCHE, prec, rows, cols, pat[0])
+        return a, nzval, stats, pat
     a = _assembler(space, form.dOmega, device)
     pat = a.pattern(rows, cols, symbolic)
     coef, ncoef = _coefficients(form, _ncoef_items(form, space))
@@ -402,9 +498,33 @@ def _assemble_matrix_sum(fs, space, rows, cols, precision, device, symbolic):
     return a_vol, nzval, stats, pat
 
 
+def _nitsche_load(form, space, rows, device):
+    """int beta v g - n.grad(v) g: the load's matrix (rows x FREE, and
+    rows x DIRICHLET where g has Dirichlet values) applied to g's nodal values
+    on the device."""
+    g = form.g
+    if not isinstance(g, DiscreteField) or g.space is not space:
+        raise TypeError("NitscheLoad: g is a DiscreteField of the space")
+    a = _assembler(space, form.dOmega, device, "nitsche_load")
+    a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
+    n = space.nfree if rows == FREE else space.ndiri
+    b = np.zeros(n)
+    for cols, x in ((FREE, g.values[:space.nfree]), (DIRICHLET, g.values[space.nfree:])):
+        if len(x) == 0 or n == 0:
+            continue
+        a.pattern(rows, cols)
+        a.ctx.assemble_matrix_async(L.GTK_NITSCHE_LOAD, L.GTK_FP64, rows, cols)  # values stay on the device
+        a.ctx.sync()
+        a.values_owner.pop((rows, cols), None)  # the block's device values are the load's matrix now
+        b = a.ctx.block_multiply(rows, cols, x, b, 1.0)
+    return b
+
+
 def _assemble_vector_values(form, space, rows, device):
     if isinstance(form, FormSum):
         return sum(_assemble_vector_values(t, space, rows, device) for t in form.terms)
+    if isinstance(form, NitscheLoad):
+        return _nitsche_load(form, space, rows, device)
     kind, data = _load_data(form, space)
     a = _assembler(space, form.dOmega, device)
     n = space.nfree if rows == FREE else space.ndiri

@@ -115,7 +115,8 @@ struct gtk_ctx {
   std::vector<double> face_jac;
   std::vector<int32_t> item_face;
   // device copies
-  DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef, d_face_jac, d_item_face;
+  DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef, d_face_jac, d_item_face, d_face_normal;
+  bool has_face_grad = false;  // face mode: reference gradients and normals set (Nitsche terms)
   // wavefronts per workgroup of the matrix-core kernels; 0 = per kernel:
   // 16 or 12 (four or three per SIMD) when that instantiation fits its
   // registers without scratch, else 8 (two per SIMD within 256 registers);
@@ -459,6 +460,7 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
       for (int d = 0; d < D; d++) ctx->gphi_t[((size_t)q * D + d) * nshape + s] = grad_phi[((size_t)q * nshape + s) * D + d];
   ctx->gN.assign(grad_geo, grad_geo + (size_t)nq * ctx->nv * D);
   ctx->nfaces = 0;
+  ctx->has_face_grad = false;
   for (auto &mp : ctx->mplans) mp.second->exec.clear();  // kernels depend on the tables
   ctx->vplans.clear();
   int rc;
@@ -502,6 +504,35 @@ int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, i
   if ((rc = upload(ctx, ctx->d_item_face, ctx->item_face.data(), ctx->item_face.size()))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_ref = true;
+  ctx->has_face_grad = false;
+  ctx->generation++;
+  return GTK_OK;
+}
+
+int gtk_set_reference_face_gradients(gtk_ctx *ctx, const double *grad_phi, const double *normals) {
+  if (!ctx) return GTK_EINVAL;
+  if (!ctx->has_ref || !ctx->nfaces) FAIL(ctx, GTK_ESTATE, "gtk_set_reference_faces first");
+  if (!grad_phi || !normals) FAIL(ctx, GTK_EINVAL, "grad_phi and normals are required");
+  const int D = ctx->dim, L = ctx->nfaces, nq = ctx->nq, ns = ctx->nshape;
+  for (int l = 0; l < L; l++) {
+    double nn = 0.0;
+    for (int d = 0; d < D; d++) nn += normals[l * D + d] * normals[l * D + d];
+    if (!(std::fabs(nn - 1.0) < 1e-12)) FAIL(ctx, GTK_EINVAL, "reference normals must have unit length");
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // [face][q][s][d] -> [face][q][d][s], the kernels' layout
+  ctx->gphi_t.assign((size_t)L * nq * D * ns, 0.0);
+  for (int l = 0; l < L; l++)
+    for (int q = 0; q < nq; q++)
+      for (int s = 0; s < ns; s++)
+        for (int d = 0; d < D; d++)
+          ctx->gphi_t[(((size_t)l * nq + q) * D + d) * ns + s] = grad_phi[(((size_t)l * nq + q) * ns + s) * D + d];
+  int rc;
+  if ((rc = upload(ctx, ctx->d_gphi, ctx->gphi_t.data(), ctx->gphi_t.size()))) return rc;
+  if ((rc = upload(ctx, ctx->d_face_normal, normals, (size_t)L * D))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto &mp : ctx->mplans) mp.second->exec.clear();
+  ctx->has_face_grad = true;
   ctx->generation++;
   return GTK_OK;
 }
@@ -951,6 +982,7 @@ static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, c
   if (ctx->nfaces) {
     a.item_face = ctx->d_item_face.as<int32_t>();
     a.face_jac = ctx->d_face_jac.as<double>();
+    a.face_normal = ctx->d_face_normal.as<double>();
   }
   return a;
 }
@@ -958,9 +990,11 @@ static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, c
 static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision) {
   if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (form < GTK_LAPLACE || form > GTK_ELASTICITY) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (form < GTK_LAPLACE || form > GTK_NITSCHE_LOAD) FAIL(ctx, GTK_EINVAL, "unknown form");
   if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
   if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
+  if ((form == GTK_NITSCHE || form == GTK_NITSCHE_LOAD) && !ctx->has_face_grad)
+    FAIL(ctx, GTK_ESTATE, "Nitsche terms: gtk_set_reference_faces and gtk_set_reference_face_gradients first");
   ExecPlan *ep = nullptr;
   int rc = get_exec(ctx, plan, form, precision, ep);
   if (rc) return rc;
@@ -1005,7 +1039,7 @@ int gtk_plan_info(gtk_ctx *ctx, int form, int precision, int rows, int cols, gtk
   MatrixPlan *p = find_plan(ctx, rows, cols);
   if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (form < GTK_LAPLACE || form > GTK_ELASTICITY) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (form < GTK_LAPLACE || form > GTK_NITSCHE_LOAD) FAIL(ctx, GTK_EINVAL, "unknown form");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ExecPlan *ep = nullptr;
   int rc = get_exec(ctx, *p, form, precision, ep);
@@ -1105,41 +1139,56 @@ int gtk_set_values(gtk_ctx *ctx, int rows, int cols, const double *nzval) {
   return GTK_OK;
 }
 
-int gtk_lift_dirichlet(gtk_ctx *ctx, int rows, const double *xd, double *b) {
-  if (!ctx) return GTK_EINVAL;
-  MatrixPlan *p = find_plan(ctx, rows, GTK_DIRICHLET);
-  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern(rows, GTK_DIRICHLET) and gtk_assemble_matrix first");
-  const int64_t nr = p->pat.nrows, nc = p->pat.ncols, nnz = p->pat.nnz;
-  if ((!xd && nc) || (!b && nr)) FAIL(ctx, GTK_EINVAL, "xd and b are required");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
+// Row view of a block (built once per block): columns ascending, so each
+// row lists its entries in ascending column order; with staging vectors for
+// a product with the block.
+static int row_view(gtk_ctx *ctx, MatrixPlan &P) {
+  if (P.lift_rptr.p) return GTK_OK;
+  const int64_t nr = P.pat.nrows, nc = P.pat.ncols, nnz = P.pat.nnz;
+  std::vector<int64_t> rptr(nr + 1, 0);
+  for (int64_t k = 0; k < nnz; k++) rptr[P.pat.rowval[k]]++;
+  for (int64_t i = 0; i < nr; i++) rptr[i + 1] += rptr[i];
+  std::vector<int64_t> next(rptr.begin(), rptr.end() - 1);
+  std::vector<int32_t> perm(std::max<int64_t>(1, nnz)), col(std::max<int64_t>(1, nnz));
+  for (int64_t j = 0; j < nc; j++)
+    for (int64_t k = P.pat.colptr[j] - 1; k < P.pat.colptr[j + 1] - 1; k++) {
+      const int64_t slot = next[P.pat.rowval[k] - 1]++;
+      perm[slot] = (int32_t)k;
+      col[slot] = (int32_t)j;
+    }
   int rc;
-  if (!p->lift_rptr.p) {
-    // row view, built once per block: columns ascending, so each row lists
-    // its entries in ascending column order
-    std::vector<int64_t> rptr(nr + 1, 0);
-    for (int64_t k = 0; k < nnz; k++) rptr[p->pat.rowval[k]]++;
-    for (int64_t i = 0; i < nr; i++) rptr[i + 1] += rptr[i];
-    std::vector<int64_t> next(rptr.begin(), rptr.end() - 1);
-    std::vector<int32_t> perm(std::max<int64_t>(1, nnz)), col(std::max<int64_t>(1, nnz));
-    for (int64_t j = 0; j < nc; j++)
-      for (int64_t k = p->pat.colptr[j] - 1; k < p->pat.colptr[j + 1] - 1; k++) {
-        const int64_t slot = next[p->pat.rowval[k] - 1]++;
-        perm[slot] = (int32_t)k;
-        col[slot] = (int32_t)j;
-      }
-    if ((rc = upload(ctx, p->lift_rptr, rptr.data(), rptr.size())) || (rc = upload(ctx, p->lift_perm, perm.data(), perm.size())) ||
-        (rc = upload(ctx, p->lift_col, col.data(), col.size())) ||
-        (rc = alloc(ctx, p->lift_x, std::max<int64_t>(1, nc) * sizeof(double))) ||
-        (rc = alloc(ctx, p->lift_b, std::max<int64_t>(1, nr) * sizeof(double))))
-      return rc;
-  }
-  if (nc) HIPCHK(ctx, hipMemcpyAsync(p->lift_x.p, xd, nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  if (nr) HIPCHK(ctx, hipMemcpyAsync(p->lift_b.p, b, nr * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = upload(ctx, P.lift_rptr, rptr.data(), rptr.size())) || (rc = upload(ctx, P.lift_perm, perm.data(), perm.size())) ||
+      (rc = upload(ctx, P.lift_col, col.data(), col.size())) ||
+      (rc = alloc(ctx, P.lift_x, std::max<int64_t>(1, nc) * sizeof(double))) ||
+      (rc = alloc(ctx, P.lift_b, std::max<int64_t>(1, nr) * sizeof(double))))
+    return rc;
+  return GTK_OK;
+}
+
+int gtk_block_multiply(gtk_ctx *ctx, int rows, int cols, double alpha, const double *x, double *y) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern(rows, cols) and gtk_assemble_matrix first");
+  const int64_t nr = p->pat.nrows, nc = p->pat.ncols;
+  if ((!x && nc) || (!y && nr)) FAIL(ctx, GTK_EINVAL, "x and y are required");
+  if (!std::isfinite(alpha)) FAIL(ctx, GTK_EINVAL, "alpha must be finite");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc = row_view(ctx, *p);
+  if (rc) return rc;
+  if (nc) HIPCHK(ctx, hipMemcpyAsync(p->lift_x.p, x, nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if (nr) HIPCHK(ctx, hipMemcpyAsync(p->lift_b.p, y, nr * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   gtk::launch_lift(nr, p->lift_rptr.as<int64_t>(), p->lift_perm.as<int32_t>(), p->lift_col.as<int32_t>(),
-                   p->nzval.as<double>(), p->lift_x.as<double>(), p->lift_b.as<double>(), ctx->stream);
-  if (nr) HIPCHK(ctx, hipMemcpyAsync(b, p->lift_b.p, nr * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+                   p->nzval.as<double>(), alpha, p->lift_x.as<double>(), p->lift_b.as<double>(), ctx->stream);
+  if (nr) HIPCHK(ctx, hipMemcpyAsync(y, p->lift_b.p, nr * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return GTK_OK;
+}
+
+int gtk_lift_dirichlet(gtk_ctx *ctx, int rows, const double *xd, double *b) {
+  if (!ctx) return GTK_EINVAL;
+  if (!find_plan(ctx, rows, GTK_DIRICHLET))
+    FAIL(ctx, GTK_ESTATE, "gtk_build_pattern(rows, GTK_DIRICHLET) and gtk_assemble_matrix first");
+  return gtk_block_multiply(ctx, rows, GTK_DIRICHLET, -1.0, xd, b);
 }
 
 int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr) {

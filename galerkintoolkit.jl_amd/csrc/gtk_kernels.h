@@ -18,7 +18,9 @@ struct ValuBlock {
   static constexpr int value = sizeof(CT) == 4 ? 768 : 512;
 };
 
-enum { kLaplace = 0, kMass = 1, kElasticity = 2 };
+// kNitsche: (beta v u - v n.grad u - n.grad v u) over boundary items;
+// kNitscheLoad: (beta v u - n.grad v u), the load's matrix applied to nodal g
+enum { kLaplace = 0, kMass = 1, kElasticity = 2, kNitsche = 3, kNitscheLoad = 4 };
 enum { kFConst = 0, kFCell = 1, kFNodal = 2 };
 
 struct MatrixArgs {
@@ -44,6 +46,7 @@ struct MatrixArgs {
   int ntiles;        // tiles of the plan (matrix-core kernels loop over them)
   const int32_t *item_face;  // face mode (boundary items), as in VectorArgs
   const double *face_jac;
+  const double *face_normal;  // Nitsche: reference outward normal of each local face, [face][D]
 };
 
 // Many problems in one launch (matrix-core kernels of 2D quadrilaterals):

@@ -155,7 +155,9 @@ __device__ inline double surface_measure(const double J[D][D], const double *__r
 // number of staged geometry factors per (cell, point)
 template <int D, int FORM>
 struct NGeo {
-  static constexpr int value = FORM == kLaplace ? D * (D + 1) / 2 : (FORM == kMass ? 1 : D * D + 3);
+  static constexpr int value =
+      FORM == kLaplace ? D * (D + 1) / 2
+                       : (FORM == kMass ? 1 : (FORM == kNitsche || FORM == kNitscheLoad ? D + 1 : D * D + 3));
 };
 
 // lanes per column of the VALU matrix kernel: one per local row; for
@@ -192,14 +194,21 @@ __device__ inline int sym_index(int i, int j) {
   else return i == 0 ? j : (i == 1 ? 2 + j : 5);
 }
 
-// NL > 0: boundary items (face mode, mass only): the tables are per local
-// face, [NL][NQ][...], and the measure is the face's.
+// NL > 0: boundary items (face mode: mass, Nitsche): the tables are per
+// local face, [NL][NQ][...], and the measure is the face's.  Nitsche items
+// list all DOFs of their cell (normal derivatives involve every shape
+// function); per (item, point) the staged factors are beta dG and dG J^-1 n,
+// with dG the face measure times the weight, beta the item's coefficient
+// and n = J^-T n_ref / |J^-T n_ref| the physical unit normal
+// (src/accessors.jl:1009-1030), so that n . grad phi = (J^-1 n) . grad_ref phi.
 // BS: threads per workgroup (ValuBlock<CT>, or 256 where the accumulators of
 // that many waves would leave no LDS for a tile)
 template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT, int NL = 0, int BS = ValuBlock<CT>::value>
 __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
   constexpr int kVB = BS;
-  static_assert(NL == 0 || FORM == kMass, "boundary items: mass form only");
+  constexpr bool kNit = FORM == kNitsche || FORM == kNitscheLoad;
+  static_assert(NL == 0 || FORM == kMass || kNit, "boundary items: mass and Nitsche forms");
+  static_assert(!kNit || (NL > 0 && NC == 1), "Nitsche terms: scalar fields on boundary items");
   constexpr int NG = NGeo<D, FORM>::value;
   constexpr int NDOF = NS * NC;
   // lanes per column: one per local row, or (elasticity) one per node, that
@@ -219,9 +228,10 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
   constexpr bool kCellGeo = CellGeo<D, NV, FORM, NL>::value;
   constexpr int GQ = kCellGeo ? 1 : NQ;  // factor sets per cell
   constexpr int NGS = CellGeoStride<D, FORM, NL, NV>::value;  // values per factor set
-  constexpr int NGPHI = NL > 0 ? 0 : (kCellGeo ? NS * NS * D * D : NQ * D * NS);  // gradient table, or R
+  // gradient table ([NT][NQ][D][NS]; none for the face mass), or R
+  constexpr int NGPHI = NL > 0 ? (kNit ? NT * NQ * D * NS : 0) : (kCellGeo ? NS * NS * D * D : NQ * D * NS);
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NQ][D][NS], or R [NS][NS][D][D]
+  CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NT][NQ][D][NS], or R [NS][NS][D][D]
   CT *phi_s = gphi_s + NGPHI;                      // [NT][NQ][NS]
   CT *G_s = phi_s + NT * NQ * NS;                  // [cells][GQ][NGS]
   double *acc_base = reinterpret_cast<double *>(lds_raw + args.acc_offset);
@@ -259,6 +269,31 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
       double J[D][D];
       cell_jacobian_only<D, NV>(args.xyz, args.cell_nodes, args.gN + (size_t)l * NQ * NV * D, cell, q, J);
       dV = surface_measure<D>(J, args.face_jac + l * D * (D - 1)) * args.w[q];
+      if constexpr (kNit) {
+        // v = J^-T n_ref, n = v / |v|, m = J^-1 n
+        (void)inverse<D>(J, Ji);
+        const double *nr = args.face_normal + l * D;
+        double v[D], vv = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; j++) t = fma(Ji[j][i], nr[j], t);
+          v[i] = t;
+          vv = fma(t, t, vv);
+        }
+        const double inv = vv > 0.0 ? 1.0 / sqrt(vv) : 0.0;
+        CT *G = G_s + (s * GQ + q) * NGS;
+        G[0] = (CT)((args.coef ? args.coef[cell] : 1.0) * dV);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          double m = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; j++) m = fma(Ji[i][j], v[j] * inv, m);
+          G[1 + i] = (CT)(dV * m);
+        }
+        continue;
+      }
     } else {
       dV = cell_jacobian<D, NV>(args.xyz, args.cell_nodes, args.gN, cell, q, Ji);
       if constexpr (!kCellGeo) dV *= args.w[q];
@@ -366,6 +401,27 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
         const CT *ph = phi_s + (NL > 0 ? lf_s[s] * NQ * NS : 0);
 #pragma unroll
         for (int q = 0; q < NQ; q++) v[0] = fma(ph[q * NS + an], Gs[q * NG] * ph[q * NS + bn], v[0]);
+      } else if constexpr (kNit) {
+        // row a (test v = phi_a), column b (trial u = phi_b):
+        //   beta dG phi_a phi_b - phi_a (m . gref_b) - (m . gref_a) phi_b
+        // (the load's matrix drops the middle term)
+        const int lf = lf_s[s];
+        const CT *ph = phi_s + lf * NQ * NS;
+        const CT *gp = gphi_s + lf * NQ * D * NS;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          const CT *G = Gs + q * NGS;
+          CT ta = 0, tb = 0;
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            ta = fma(G[1 + d], gp[(q * D + d) * NS + an], ta);
+            tb = fma(G[1 + d], gp[(q * D + d) * NS + bn], tb);
+          }
+          const CT pa = ph[q * NS + an], pb = ph[q * NS + bn];
+          CT t = G[0] * pa * pb - ta * pb;
+          if constexpr (FORM == kNitsche) t -= pa * tb;
+          v[0] += t;
+        }
       } else {
         // sigma(eps(phi_b e_bc)) : eps(phi_a e_ac)
         //   = lambda (g_b)_bc (g_a)_ac + mu (delta_{ac,bc} g_a.g_b + (g_a)_bc (g_b)_ac)
@@ -1375,7 +1431,9 @@ static KernelInfo matrix_info_bs() {
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl, BS>);
   constexpr bool cell_geo = CellGeo<E::d, E::nv, FORM, E::nl>::value;
-  constexpr int ngphi = E::nl > 0 ? 0 : (cell_geo ? E::ns * E::ns * E::d * E::d : E::nq * E::d * E::ns);
+  constexpr bool nit = FORM == kNitsche || FORM == kNitscheLoad;
+  constexpr int ngphi = E::nl > 0 ? (nit ? NT * E::nq * E::d * E::ns : 0)
+                                  : (cell_geo ? E::ns * E::ns * E::d * E::d : E::nq * E::d * E::ns);
   k.table_bytes = (size_t)(ngphi + NT * E::nq * E::ns) * sizeof(CT);
   constexpr int GQ = cell_geo ? 1 : E::nq;
   k.geo_bytes_per_cell = (size_t)GQ * CellGeoStride<E::d, FORM, E::nl, E::nv>::value * sizeof(CT) +
@@ -1415,9 +1473,21 @@ static bool try_matrix(const ElemKey &key, int form, int precision, int off_byte
                        KernelInfo &out) {
   if (!match<E>(key)) return false;
   if constexpr (E::nl > 0) {
-    // boundary items: the mass form on the VALU kernel
-    if (form != kMass) return false;
-    out = precision == 0 ? matrix_info<E, kMass, double>(waves) : matrix_info<E, kMass, float>(waves);
+    // boundary items on the VALU kernel: the mass form, and on scalar fields
+    // the Nitsche terms
+    if (form == kMass) {
+      out = precision == 0 ? matrix_info<E, kMass, double>(waves) : matrix_info<E, kMass, float>(waves);
+    } else if constexpr (E::nc == 1) {
+      if (form == kNitsche)
+        out = precision == 0 ? matrix_info<E, kNitsche, double>(waves) : matrix_info<E, kNitsche, float>(waves);
+      else if (form == kNitscheLoad)
+        out = precision == 0 ? matrix_info<E, kNitscheLoad, double>(waves)
+                             : matrix_info<E, kNitscheLoad, float>(waves);
+      else
+        return false;
+    } else {
+      return false;
+    }
   } else if constexpr (E::nc == 1) {
     if (form != kLaplace && form != kMass) return false;
     if (allow_mfma) {
@@ -1465,6 +1535,13 @@ using QuadEdge3 = Elem<2, 4, 4, 4, 1, 4>;
 using HexFace1 = Elem<3, 8, 4, 4, 1, 6>;
 using HexFace2 = Elem<3, 8, 9, 9, 1, 6>;
 using TetFace2v = Elem<3, 4, 6, 9, 3, 4>;
+// Nitsche items: boundary faces with all DOFs of their cell, Gauss rules of
+// k+1 points per direction on the face
+using QuadNitsche1 = Elem<2, 4, 4, 2, 1, 4>;
+using QuadNitsche2 = Elem<2, 4, 9, 3, 1, 4>;
+using QuadNitsche3 = Elem<2, 4, 16, 4, 1, 4>;
+using HexNitsche1 = Elem<3, 8, 8, 4, 1, 6>;
+using HexNitsche2 = Elem<3, 8, 27, 9, 1, 6>;
 
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out) {
@@ -1479,7 +1556,12 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<QuadEdge3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexFace1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexFace2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
-         try_matrix<TetFace2v>(key, form, precision, off_bytes, allow_mfma, waves, out);
+         try_matrix<TetFace2v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadNitsche1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadNitsche3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexNitsche1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out);
 }
 
 int batched_type(const ElemKey &key) {

@@ -58,10 +58,11 @@ void launch_block3_check(const SolveMat &m, int *bad, hipStream_t st);
 void launch_block3_build(const SolveMat &m, const float *val_rows, int32_t *bcol, float *bval, bool with_cols,
                          hipStream_t st);
 void launch_gather_values(int64_t n, const int32_t *perm, const double *a, double *v64, float *v32, hipStream_t st);
-// b -= A xd for a CSC block A (values a) through its row view: row i's
-// entries rptr[i] .. rptr[i+1]-1 (0-based), CSC position perm[k] and column
-// col[k] (0-based), ascending columns
+// b_i += sum_k A_ik (alpha xd_k) for a CSC block A (values a) through its
+// row view: row i's entries rptr[i] .. rptr[i+1]-1 (0-based), CSC position
+// perm[k] and column col[k] (0-based), ascending columns, unfused products
+// and sums (alpha = -1: the Dirichlet lifting b -= A xd)
 void launch_lift(int64_t nrows, const int64_t *rptr, const int32_t *perm, const int32_t *col, const double *a,
-                 const double *xd, double *b, hipStream_t st);
+                 double alpha, const double *xd, double *b, hipStream_t st);
 
 }  // namespace gtk

@@ -339,11 +339,12 @@ __global__ __launch_bounds__(kRB) void gather_values_kernel(int64_t n, const int
 // and rounding of mul!(b, Ad, xd, -1, 1) on a CSC matrix
 __global__ __launch_bounds__(kRB) void lift_kernel(int64_t n, const int64_t *__restrict__ rptr,
                                                    const int32_t *__restrict__ perm, const int32_t *__restrict__ col,
-                                                   const double *__restrict__ a, const double *__restrict__ xd,
-                                                   double *__restrict__ b) {
+                                                   const double *__restrict__ a, double alpha,
+                                                   const double *__restrict__ xd, double *__restrict__ b) {
   for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
     double acc = b[i];
-    for (int64_t k = rptr[i]; k < rptr[i + 1]; k++) acc = __dadd_rn(acc, __dmul_rn(a[perm[k]], -xd[col[k]]));
+    for (int64_t k = rptr[i]; k < rptr[i + 1]; k++)
+      acc = __dadd_rn(acc, __dmul_rn(a[perm[k]], __dmul_rn(alpha, xd[col[k]])));
     b[i] = acc;
   }
 }
@@ -351,10 +352,10 @@ __global__ __launch_bounds__(kRB) void lift_kernel(int64_t n, const int64_t *__r
 }  // namespace
 
 void launch_lift(int64_t nrows, const int64_t *rptr, const int32_t *perm, const int32_t *col, const double *a,
-                 const double *xd, double *b, hipStream_t st) {
+                 double alpha, const double *xd, double *b, hipStream_t st) {
   if (nrows == 0) return;
   const int64_t blocks = (nrows + kRB - 1) / kRB;
-  lift_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), kRB, 0, st>>>(nrows, rptr, perm, col, a, xd, b);
+  lift_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), kRB, 0, st>>>(nrows, rptr, perm, col, a, alpha, xd, b);
 }
 
 int solve_rows_per_lane_group(double mean_row) {

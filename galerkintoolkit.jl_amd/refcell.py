@@ -78,6 +78,30 @@ def linear_shape(kind, xi):
     return N
 
 
+def face_normals(kind):
+    """Outward unit normal of each local (D-1)-face of the reference cell, in
+    the local face order of GEO (the lists of src/domain.jl:226, :258 for the
+    unit square and cube, :428, :460 for the simplices): orthogonal to the
+    face's tangents, pointing away from the cell's centroid."""
+    g = GEO[kind]
+    D = g.dim
+    V = np.asarray(g.verts, np.float64)
+    cen = V.mean(axis=0)
+    out = []
+    for fv in g.faces[D - 1]:
+        X = V[np.asarray(fv) - 1]
+        T = (X[1:] - X[0]).T  # D x (number of face vertices - 1) tangents
+        if D == 2:
+            n = np.array([T[1, 0], -T[0, 0]])
+        else:
+            n = np.cross(T[:, 0], T[:, 1])
+        n = n / np.linalg.norm(n)
+        if n @ (X.mean(axis=0) - cen) < 0:
+            n = -n
+        out.append(n)
+    return np.array(out)
+
+
 def vertex_permutations(kind):
     """Admissible vertex permutations (src/domain.jl:53-100), 1-based:
     identity for D == 0 and D > 2, all permutations (lexicographic) for

@@ -51,6 +51,14 @@ typedef struct gtk_ctx gtk_ctx;
 #define GTK_LAPLACE 0     /* int kappa grad(u).grad(v), kappa per cell (or 1) */
 #define GTK_MASS 1        /* int kappa u v */
 #define GTK_ELASTICITY 2  /* int sigma(eps(u)) : eps(v), (lambda, mu) per cell */
+/* Nitsche terms on boundary items with all DOFs of their cell (scalar
+ * fields; GalerkinToolkitExamples/src/poisson.jl:93-112), beta per item
+ * (gamma / h of the face) as the item's coefficient, n the outward unit
+ * normal: */
+#define GTK_NITSCHE 3       /* int beta v u - v n.grad(u) - n.grad(v) u */
+#define GTK_NITSCHE_LOAD 4  /* int beta v u - n.grad(v) u: applied to the nodal
+                               values of g (gtk_block_multiply) it is the load
+                               int beta v g - n.grad(v) g of the same method */
 
 /* arithmetic of the per-cell contraction */
 #define GTK_FP64 0   /* everything in FP64: the plain drop-in */
@@ -125,6 +133,17 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
 int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, int nshape, const double *phi,
                             const double *grad_geo, const double *face_jac, const int32_t *item_face);
 
+/* Face mode, Nitsche terms: reference gradients of the item's shape
+ * functions at the face rule's points, grad_phi[nfaces][nq][nshape][dim]
+ * (the points of gtk_set_reference_faces, mapped into the cell), and the
+ * outward unit normal of each local face of the reference cell,
+ * normals[nfaces][dim] (src/domain.jl:226, :258).  The physical normal at a
+ * point is J^-T n / |J^-T n| (map_unit_normal, src/accessors.jl:1009-1030).
+ * Replaces the n.grad terms of the generated boundary loops
+ * (src/compiler.jl:513-529) for the Nitsche method of
+ * GalerkinToolkitExamples/src/poisson.jl:93-112. */
+int gtk_set_reference_face_gradients(gtk_ctx *ctx, const double *grad_phi, const double *normals);
+
 /* Symbolic phase for a (rows, cols) block: builds the CSC pattern and the
  * column-owner work plan.  Equals the pattern compress() produces for the
  * COO triplets of allocate_matrix (src/assembly.jl:75-157, :571-575);
@@ -183,6 +202,13 @@ int gtk_set_values(gtk_ctx *ctx, int rows, int cols, const double *nzval);
  * n_diri values.  Replaces mul!(b, Ad, xd, -1, 1) in
  * PartitionedSolvers_linear_problem (src/problems.jl:439-452). */
 int gtk_lift_dirichlet(gtk_ctx *ctx, int rows, const double *xd, double *b);
+/* y += A x for the (rows, cols) block with the values last assembled into
+ * it, on the device: y_i += sum_k A_ik (alpha x_k), each row in ascending
+ * column order with unfused multiply and add; x (n cols), y (n rows, in/out)
+ * host arrays.  gtk_lift_dirichlet is alpha = -1 on (rows, GTK_DIRICHLET).
+ * Replaces mul!(y, A, x, alpha, 1) on the assembled SparseMatrixCSC (the
+ * reference's lifting and the Nitsche load from nodal boundary data). */
+int gtk_block_multiply(gtk_ctx *ctx, int rows, int cols, double alpha, const double *x, double *y);
 /* Device address of the value array of a (rows, cols) block (for callers
  * that keep the matrix on the GPU). */
 int gtk_values_device_ptr(gtk_ctx *ctx, int rows, int cols, double **dptr);
