@@ -14,7 +14,7 @@ from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField
                        update_matrix_with_free_and_dirichlet_columns, update_vector)
 from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
 from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space
-from .batch import MatrixBatch, assemble_matrices, autotune
+from .batch import MatrixBatch, assemble_matrices, autotune, autotune_matrix
 from . import inputs, refcell
 
 __all__ = [
@@ -26,6 +26,6 @@ __all__ = [
     "assemble_matrix_and_vector_with_free_and_dirichlet_columns", "update_matrix_and_vector",
     "update_matrix_and_vector_with_free_and_dirichlet_columns", "update_matrix_with_free_and_dirichlet_columns",
     "update_vector",
-    "cartesian_mesh", "perturb_interior_nodes", "MatrixBatch", "assemble_matrices", "autotune", "Domain",
+    "cartesian_mesh", "perturb_interior_nodes", "MatrixBatch", "assemble_matrices", "autotune", "autotune_matrix", "Domain",
     "LagrangeSpace", "boundary", "interior", "lagrange_space", "inputs", "refcell",
 ]

@@ -25,7 +25,7 @@ EXPORTS = [
     "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_set_values", "gtk_values_device_ptr", "gtk_assemble_vector",
     "gtk_assemble_vector_noalloc",
-    "gtk_plan_info", "gtk_lift_dirichlet", "gtk_block_multiply", "gtk_set_reference_face_gradients",
+    "gtk_plan_info", "gtk_set_launch", "gtk_lift_dirichlet", "gtk_block_multiply", "gtk_set_reference_face_gradients",
     "gtk_assemble_matrix_async", "gtk_sync", "gtk_time_matrix", "gtk_solve", "gtk_batch_create", "gtk_batch_destroy",
     "gtk_batch_last_error", "gtk_batch_add", "gtk_batch_set_tile_cells", "gtk_batch_set_waves", "gtk_batch_set_load", "gtk_batch_get_vector",
     "gtk_batch_build", "gtk_batch_run",
@@ -44,11 +44,11 @@ class GtkSolveStats(C.Structure):
 
 class GtkPlanInfo(C.Structure):
     _fields_ = [("matrix_cores", C.c_int), ("threads", C.c_int), ("grid", C.c_int), ("tiles", C.c_int),
-                ("max_tile_cells", C.c_int), ("stage_cells", C.c_int), ("lds_bytes", C.c_int), ("pad", C.c_int),
+                ("max_tile_cells", C.c_int), ("stage_cells", C.c_int), ("lds_bytes", C.c_int), ("tile_cap", C.c_int),
                 ("batches", C.c_int64), ("tile_cells", C.c_int64), ("cells", C.c_int64)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
         d["halo_redundancy"] = self.tile_cells / max(1, self.cells)
         return d
 
@@ -96,6 +96,7 @@ def load():
     L.gtk_block_multiply.argtypes = [vp, i, i, C.c_double, f64, f64]
     L.gtk_set_reference_face_gradients.argtypes = [vp, f64, f64]
     L.gtk_plan_info.argtypes = [vp, i, i, i, i, C.POINTER(GtkPlanInfo)]
+    L.gtk_set_launch.argtypes = [vp, i, i, i, i, i, i]
     L.gtk_values_device_ptr.argtypes = [vp, i, i, C.POINTER(C.c_void_p)]
     L.gtk_assemble_vector.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_assemble_vector_noalloc.argtypes = [vp, i, f64, i, C.c_void_p, C.POINTER(GtkStats)]
@@ -278,6 +279,9 @@ class Context:
         info = GtkPlanInfo()
         self.check(self.L.gtk_plan_info(self.h, form, precision, rows, cols, C.byref(info)), "gtk_plan_info")
         return info.as_dict()
+
+    def set_launch(self, form, precision, rows, cols, waves=0, max_cells=0):
+        self.check(self.L.gtk_set_launch(self.h, form, precision, rows, cols, waves, max_cells), "gtk_set_launch")
 
     def values_device_ptr(self, rows, cols):
         p = C.c_void_p()

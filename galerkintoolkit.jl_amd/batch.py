@@ -15,8 +15,8 @@ import numpy as np
 
 from . import _lib as L
 from . import refcell as R
-from .assembly import (FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _coefficients, _form_code,
-                       _load_data)
+from .assembly import (FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _assembler, _coefficients,
+                       _form_code, _load_data, _ncoef_items)
 
 # element types of the batched kernel: scalar quadrilaterals of order 1, 2, 3
 # with (order+1)^2 Gauss points
@@ -129,6 +129,68 @@ def autotune(problems, form_code, precision, device=0, cache=None, candidates=TI
     report["waves"] = w
     report["predicted_ms"] = {str(k): round(v * 1e-6, 4) for k, v in predicted.items()}
     return w, tiles, report
+
+
+# launch shapes the single-problem tuner tries: waves per workgroup
+# (matrix-core kernels; the VALU kernel: its default, 0, or 4) and tile caps
+SINGLE_WAVES = {True: (8, 12, 16), False: (0, 4)}
+SINGLE_CAPS = (0, 64)
+_KIND_NAMES = {R.QUAD: "quad", R.HEX: "hex", R.TRI: "tri", R.TET: "tet"}
+
+
+def element_name(space, dOmega):
+    """Element type of a single-problem work plan: cell kind, order,
+    components and quadrature points."""
+    return f"{_KIND_NAMES[space.mesh.cellkind]}-k{space.order}-c{space.ncomp}-q{len(dOmega.w)}"
+
+
+def autotune_matrix(form, space, precision=FP64, rows=FREE, cols=FREE, device=0, cache=None, retune=False,
+                    repeats=3, launches=5):
+    """Launch shape of one problem's (form, precision) work plan on the
+    (rows, cols) block: waves per workgroup and a cap on the cells of a tile.
+    For an element type (cell kind, order, components, quadrature points)
+    without a cached entry, every candidate is built on this problem and its
+    kernel timed (HIP events, best of `repeats` runs of `launches` launches);
+    the fastest is cached under (library build, form, precision, element
+    type) in the batch tuner's file and applied to the problem.  Later
+    problems of the same type take the cached shape.  Returns a report."""
+    code = _form_code(form)
+    prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+    a = _assembler(space, form.dOmega, device)
+    a.pattern(rows, cols)
+    coef, ncoef = _coefficients(form, _ncoef_items(form, space))
+    a.ctx.set_cell_coefficients(coef, ncoef)
+    ver = L.load().gtk_version().decode()
+    key = f"{ver}|single|form{code}|prec{prec}|{element_name(space, form.dOmega)}"
+    path = default_cache_path() if cache is None else cache
+    table = _read_cache(path)
+    e = table.get(key)
+    report = {"element": element_name(space, form.dOmega)}
+    if retune or not (isinstance(e, dict) and "waves" in e and "max_cells" in e):
+        mfma = bool(a.ctx.plan_info(code, prec, rows, cols)["matrix_cores"])
+        times = {}
+        for w in SINGLE_WAVES[mfma]:
+            for cap in SINGLE_CAPS:
+                try:
+                    a.ctx.set_launch(code, prec, rows, cols, w, cap)
+                except L.GtkError:
+                    continue  # no such instantiation, or the tile does not fit
+                a.ctx.time_matrix(code, prec, rows, cols, 1)
+                times[(w, cap)] = min(a.ctx.time_matrix(code, prec, rows, cols, launches) for _ in range(repeats))
+        best = min(times, key=lambda k: times[k])
+        e = {"waves": best[0], "max_cells": best[1], "ms": times[best],
+             "candidates": {f"{w}/{c}": round(t, 5) for (w, c), t in times.items()}}
+        table[key] = e
+        _write_cache(path, table)
+        report["cached"] = False
+        report["candidates_ms"] = e["candidates"]
+    else:
+        report["cached"] = True
+    a.ctx.set_launch(code, prec, rows, cols, int(e["waves"]), int(e["max_cells"]))
+    a.values_owner.pop((rows, cols), None)  # the timing launches replaced the block's device values
+    report.update(waves=int(e["waves"]), max_cells=int(e["max_cells"]),
+                  plan=a.ctx.plan_info(code, prec, rows, cols))
+    return report
 
 
 class MatrixBatch:

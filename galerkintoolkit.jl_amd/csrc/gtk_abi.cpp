@@ -56,6 +56,7 @@ struct ExecPlan {
   size_t lds = 0;
   int lds_geo = 0, acc_offset = 0, acc_per_wave = 0, stage_cells = 0, lds_claim = 0;
   int resident = 1;  // workgroups of the kernel resident on the device at once
+  int tile_cap = 0;  // cells-per-tile cap the plan was built with (0: as many as fit in LDS)
 };
 
 // Device work space of the solve on a (FREE, FREE) block.
@@ -806,13 +807,14 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
     // room for a tile of 8 cells; then the 4-wave workgroup
     // (GTK_VALU_WAVES=4 forces it: tests)
     const char *vw = std::getenv("GTK_VALU_WAVES");
-    if ((vw && std::atoi(vw) == 4) ||
+    if (waves == 4 || (vw && std::atoi(vw) == 4) ||
         align16(ep->k.table_bytes) + acc_of(ep->k) + 1024 + 8 * ep->k.geo_bytes_per_cell > kLdsBudget) {
       gtk::KernelInfo k4;
       if (gtk::find_matrix_kernel(key_e, form, precision, off_bytes, false, 4, k4)) ep->k = k4;
     }
     const size_t acc_bytes = acc_of(ep->k);
-    const int max_cells = tile_cells_for(ep->k, acc_bytes, kLdsBudget);
+    int max_cells = tile_cells_for(ep->k, acc_bytes, kLdsBudget);
+    if (max_cells_cap > 0) max_cells = std::max(8, std::min(max_cells, max_cells_cap));
     e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     e = gtk::build_offsets(m, plan.pat, ep->tiles, ep->offs, nt);
@@ -822,6 +824,7 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
     ep->lds = ep->acc_offset + acc_bytes;
   }
   if (ep->lds > 160 * 1024) FAIL(ctx, GTK_EUNSUP, "tile does not fit in LDS");
+  ep->tile_cap = max_cells_cap > 0 ? max_cells_cap : 0;
   const gtk::TilePlan &t = ep->tiles;
   int rc;
   if ((rc = upload(ctx, ep->tile_cell_ptr, t.tile_cell_ptr.data(), t.tile_cell_ptr.size()))) return rc;
@@ -850,6 +853,23 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
   return GTK_OK;
 }
 
+// Default wavefronts per workgroup of the matrix-core kernel of (form,
+// precision) on a block: the most (16: four per SIMD within 128 registers;
+// 12: three within 168) whose instantiation needs no scratch; else 8 (two
+// per SIMD within 256).  The context's GTK_WAVES setting wins.
+static int default_waves(gtk_ctx *ctx, const MatrixPlan &plan, int form, int precision) {
+  if (ctx->waves) return ctx->waves;
+  const int off_bytes = plan.max_seg < 0xFF ? 1 : 2;
+  for (const int w : {16, 12}) {
+    gtk::KernelInfo kw;
+    hipFuncAttributes fa;
+    if (gtk::find_matrix_kernel(elem_key(ctx), form, precision, off_bytes, true, w, kw) && kw.mfma &&
+        hipFuncGetAttributes(&fa, kw.fn) == hipSuccess && fa.localSizeBytes == 0)
+      return w;
+  }
+  return 8;
+}
+
 // Work plan of (form, precision) over a block, built on first use with the
 // default workgroup shape of the precision.
 static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, ExecPlan *&out) {
@@ -859,25 +879,8 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
     out = it->second.get();
     return GTK_OK;
   }
-  int waves = ctx->waves;
-  if (!waves) {
-    // the most wavefronts per workgroup (16: four per SIMD within 128
-    // registers; 12: three within 168) whose instantiation needs no
-    // scratch; else 8 (two per SIMD within 256)
-    waves = 8;
-    const int off_bytes = plan.max_seg < 0xFF ? 1 : 2;
-    for (const int w : {16, 12}) {
-      gtk::KernelInfo kw;
-      hipFuncAttributes fa;
-      if (gtk::find_matrix_kernel(elem_key(ctx), form, precision, off_bytes, true, w, kw) && kw.mfma &&
-          hipFuncGetAttributes(&fa, kw.fn) == hipSuccess && fa.localSizeBytes == 0) {
-        waves = w;
-        break;
-      }
-    }
-  }
   std::unique_ptr<ExecPlan> ep;
-  int rc = build_exec(ctx, plan, form, precision, waves, 0, ep);
+  int rc = build_exec(ctx, plan, form, precision, default_waves(ctx, plan, form, precision), 0, ep);
   if (rc) return rc;
   out = ep.get();
   plan.exec[key] = std::move(ep);
@@ -1055,6 +1058,28 @@ int gtk_plan_info(gtk_ctx *ctx, int form, int precision, int rows, int cols, gtk
   out->batches = ep->k.mfma ? ep->bat.nbatches : 0;
   out->tile_cells = ep->tiles.tile_cells.size();
   out->cells = ctx->n_cells;
+  out->tile_cap = ep->tile_cap;
+  return GTK_OK;
+}
+
+int gtk_set_launch(gtk_ctx *ctx, int form, int precision, int rows, int cols, int waves, int max_cells) {
+  if (!ctx) return GTK_EINVAL;
+  MatrixPlan *p = find_plan(ctx, rows, cols);
+  if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
+  if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
+  if (form < GTK_LAPLACE || form > GTK_NITSCHE_LOAD) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (waves != 0 && waves != 4 && waves != 8 && waves != 12 && waves != 16)
+    FAIL(ctx, GTK_EINVAL, "waves must be 0 (default), 4, 8, 12 or 16");
+  if (max_cells < 0 || max_cells > 4095) FAIL(ctx, GTK_EINVAL, "max_cells must be 0 (LDS-bound) .. 4095");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<ExecPlan> ep;
+  const int w = waves == 0 || waves == 4 ? default_waves(ctx, *p, form, precision) : waves;
+  int rc = build_exec(ctx, *p, form, precision, waves == 4 ? 4 : w, max_cells, ep);
+  if (rc) return rc;
+  if (ep->k.mfma && waves == 4) FAIL(ctx, GTK_EINVAL, "the matrix-core kernels take 8, 12 or 16 waves");
+  if (!ep->k.mfma && waves != 0 && waves != 4) FAIL(ctx, GTK_EINVAL, "the VALU kernel takes 0 (its default) or 4 waves");
+  p->exec[form * 2 + precision] = std::move(ep);
+  ctx->generation++;
   return GTK_OK;
 }
 

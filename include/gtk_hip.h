@@ -253,12 +253,19 @@ typedef struct gtk_plan_info_t {
   int max_tile_cells;  /* most cells (owned and halo) of one tile */
   int stage_cells;     /* cells staged per geometry pass (matrix-core kernel) */
   int lds_bytes;       /* LDS per workgroup */
-  int pad;
+  int tile_cap;        /* cells-per-tile cap the plan was built with (0: as many as fit in LDS) */
   int64_t batches;     /* batch records (matrix-core kernel) */
   int64_t tile_cells;  /* cells summed over the tiles: tile_cells / cells is the halo redundancy */
   int64_t cells;       /* cells of the mesh */
 } gtk_plan_info_t;
 int gtk_plan_info(gtk_ctx *ctx, int form, int precision, int rows, int cols, gtk_plan_info_t *out);
+/* Rebuild the work plan of (form, precision) on a built block with a given
+ * launch shape: waves per workgroup (matrix-core kernels 8, 12 or 16; the
+ * VALU kernel 4, or 0 for its default; 0 everywhere picks the default) and
+ * a cap on the cells of a tile (0: as many as fit in LDS).  The knobs of the
+ * single-problem autotuner; no counterpart in the reference.  The next
+ * assembly of that form and precision uses the new plan. */
+int gtk_set_launch(gtk_ctx *ctx, int form, int precision, int rows, int cols, int waves, int max_cells);
 
 /* Enqueue the matrix assembly of a (rows, cols) block on the context's
  * stream without synchronising (for timing loops); gtk_sync waits. */
