@@ -143,6 +143,17 @@ def time_form(ctx, L, form, prec, steps, warmup):
     return wall_ms, ctx.time_matrix(form, prec, L.GTK_FREE, L.GTK_FREE, steps)
 
 
+def tuned_launch(GT, args, ctx, form, V, code):
+    """Launch shape of both precisions: the autotuner's choice (tuned on the
+    problem unless cached), or with --no-tune the library's defaults."""
+    from gtk_amd import _lib as L
+    if args.no_tune:
+        return {name: {"tuned": False, "plan": ctx.plan_info(code, prec, L.GTK_FREE, L.GTK_FREE)}
+                for name, prec in (("fp64", L.GTK_FP64), ("mixed", L.GTK_MIXED))}
+    return {name: GT.autotune_matrix(form, V, precision=prec, cache=args.tune_cache or None)
+            for name, prec in (("fp64", GT.FP64), ("mixed", GT.MIXED))}
+
+
 def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
     """Laplace assembly on an n^3 hex mesh: FP64 and mixed forms."""
     from gtk_amd import _lib as L
@@ -162,9 +173,7 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
     rel_err = float(np.linalg.norm(Amx.nzval - ref) / np.linalg.norm(ref))
     setup_s = time.perf_counter() - t_setup
     ctx = A64.device[0].ctx
-    tune_cache = args.tune_cache or None
-    launch = {name: GT.autotune_matrix(form, V, precision=prec, cache=tune_cache)
-              for name, prec in (("fp64", GT.FP64), ("mixed", GT.MIXED))}
+    launch = tuned_launch(GT, args, ctx, form, V, L.GTK_LAPLACE)
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib << 20)
     wall64 = kern64 = wallmx = kernmx = float("nan")
@@ -189,8 +198,8 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
     pcie_ms = min(GT.update_matrix(Amx, cache_mx).stats.total_ms for _ in range(3))
     nd = V.fe.nnodes
     nq = len(dO.w)
-    headline = (order, degree, perturb, lognormal_kappa) == (2, 4, 0.2, False)
-    traffic = measured_traffic("q2hex64", cells_per_dir=n) if headline else None
+    traffic = measured_traffic(f"q{order}hex{n}", cells_per_dir=n, order=order, degree=degree, perturb=perturb,
+                               lognormal_kappa=lognormal_kappa)
     bytes_alg = alg_bytes(3, mesh.coords.shape[0], cells, 8, nd, 1 if lognormal_kappa else 0, A64.nnz)
     flops_alg = cells * nq * (2 * 3 * nd * nd + 2 * 9 * nd + 2 * 8 * 9)
     achieved = bytes_alg / (kernmx * 1e-3) / 1e9
@@ -255,8 +264,7 @@ def bench_elasticity(GT, args, n):
     b = GT.assemble_vector(GT.Load(dO, (0.0, 0.0, -1.0)), float, V)
     setup_s = time.perf_counter() - t_setup
     ctx = A.device[0].ctx
-    launch = {name: GT.autotune_matrix(form, V, precision=prec, cache=args.tune_cache or None)
-              for name, prec in (("fp64", GT.FP64), ("mixed", GT.MIXED))}
+    launch = tuned_launch(GT, args, ctx, form, V, L.GTK_ELASTICITY)
     wall64, kern64 = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_FP64, args.steps, args.warmup)
     wallmx, kernmx = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_MIXED, args.steps, args.warmup)
     # the FP32 assembly is the one solved (mixed refinement to 1e-10); the
@@ -454,6 +462,9 @@ def main():
     ap.add_argument("--problems", type=int, default=256, help="problems of the batch workload")
     ap.add_argument("--batch-cpu-stride", type=int, default=2, help="CPU baseline: every k-th problem of the batch")
     ap.add_argument("--tune-cache", default="", help="autotuner cache file (default: the user cache)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="single problems keep the library's default launch shapes (no tuning launches, for "
+                         "profiling runs)")
     ap.add_argument("--cells", type=int, default=0, help="cells per direction (default per workload)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the single-thread CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
