@@ -614,11 +614,22 @@ __device__ inline v4f mfma16x16x32(half8 a, half8 b, v4f c) {
 }
 
 // x = hi + lo + e with hi, lo FP16 (round to nearest), |e| <= 2^-22 |x| for
-// x in FP16's normal range; two values per conversion
-__device__ inline void split_f16(float x0, float x1, f16x2 &hi, f16x2 &lo) {
+// x in FP16's normal range; two values per conversion.  x - hi is exact in
+// FP32; m1 = -1 held in a register the compiler cannot see through, so that
+// x + m1 hi stays one mixed-precision FMA (FP16 operand, FP32 result)
+// instead of a conversion and a subtraction
+__device__ inline void split_f16(float x0, float x1, float m1, f16x2 &hi, f16x2 &lo) {
   const f32x2 x = {x0, x1};
   hi = __builtin_convertvector(x, f16x2);
-  lo = __builtin_convertvector(x - __builtin_convertvector(hi, f32x2), f16x2);
+  const f32x2 r = {__builtin_fmaf((float)hi[0], m1, x0), __builtin_fmaf((float)hi[1], m1, x1)};
+  lo = __builtin_convertvector(r, f16x2);
+}
+
+// -1.0f that the compiler cannot constant-fold (see split_f16)
+__device__ inline float opaque_minus_one() {
+  float m1 = -1.0f;
+  asm volatile("" : "+v"(m1));
+  return m1;
 }
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -786,30 +797,97 @@ struct BatchTile {
     }
   }
 
+  // multilinear geometry (quadrilaterals, hexahedra: vertex n at the corner
+  // whose coordinate d is bit d of n): x(xi) = sum_S c_S prod_{d in S} xi_d
+  // over the bit sets S, so that dx/dxi_j = c_{j} + sum_{S > {j}} c_S
+  // prod_{d in S, d != j} xi_d needs D - 1 fused multiply-adds per entry of
+  // J with the point's products (the gradients' sums over the vertices
+  // above S) instead of one per vertex
+  static constexpr bool kMultilinear = NV == (1 << D);
+  static constexpr int NSUP = (1 << (D - 1)) - 1;  // sets S > {j} for each j
+
+  // S > {j} in a fixed order: the k-th set with bit j that is not {j}
+  __device__ static constexpr int super_set(int j, int k) {
+    int seen = 0;
+    for (int S = 0; S < NV; S++)
+      if ((S >> j & 1) && S != (1 << j)) {
+        if (seen == k) return S;
+        seen++;
+      }
+    return 0;
+  }
+
   // geometry factors of the nch staged cells (slots c0 ..) from the staged
   // coordinates; thread tid takes quadrature point tid % NQ
   __device__ static void geometry(const MatrixArgs &a, const Lds &l, int c0, int nch, int tid) {
+    if constexpr (kMultilinear) {
+      // the staged vertex coordinates become the coefficients c_S (Moebius
+      // transform over the bits of the vertex number; c_0 = x_0), one
+      // thread per (cell, direction)
+      for (int i = tid; i < nch * D; i += kB) {
+        double *X = l.X + (i / D) * NV * D + i % D;
+        double v[NV];
+#pragma unroll
+        for (int n = 0; n < NV; n++) v[n] = X[n * D];
+#pragma unroll
+        for (int b = 1; b < NV; b <<= 1)
+#pragma unroll
+          for (int n = 0; n < NV; n++)
+            if (n & b) v[n] -= v[n ^ b];
+#pragma unroll
+        for (int n = 1; n < NV; n++) X[n * D] = v[n];
+      }
+      __syncthreads();
+    }
     const int tq = tid % NQ, tcs = tid / NQ;
     if (tcs >= CPR) return;
-    double gq[NV][D];
+    constexpr int NGQ = kMultilinear ? 1 : NV;
+    double gq[NGQ][D];   // general map: the vertices' gradients at the point
+    double cw[D][NSUP];  // multilinear map: the point's products for S > {j}
+    if constexpr (kMultilinear) {
 #pragma unroll
-    for (int n = 0; n < NV; n++)
+      for (int j = 0; j < D; j++)
 #pragma unroll
-      for (int d = 0; d < D; d++) gq[n][d] = a.gN[(tq * NV + n) * D + d];
+        for (int k = 0; k < NSUP; k++) {
+          const int S = super_set(j, k);
+          double s = 0.0;
+#pragma unroll
+          for (int n = 0; n < NV; n++)
+            if ((n & S) == S) s += a.gN[(tq * NV + n) * D + j];
+          cw[j][k] = s;
+        }
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; n++)
+#pragma unroll
+        for (int d = 0; d < D; d++) gq[n][d] = a.gN[(tq * NV + n) * D + d];
+    }
     const double wq = a.w[tq];
     for (int sl = tcs; sl < nch; sl += CPR) {
       const double *X = l.X + sl * NV * D;
       double J[D][D];
-#pragma unroll
-      for (int i = 0; i < D; i++)
-#pragma unroll
-        for (int j = 0; j < D; j++) J[i][j] = 0.0;
-#pragma unroll
-      for (int n = 0; n < NV; n++)
+      if constexpr (kMultilinear) {
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
-          for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
+          for (int j = 0; j < D; j++) {
+            double s = X[(1 << j) * D + i];
+#pragma unroll
+            for (int k = 0; k < NSUP; k++) s = fma(X[super_set(j, k) * D + i], cw[j][k], s);
+            J[i][j] = s;
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+          for (int j = 0; j < D; j++) J[i][j] = 0.0;
+#pragma unroll
+        for (int n = 0; n < NV; n++)
+#pragma unroll
+          for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) J[i][j] = fma(X[n * D + i], gq[n][j], J[i][j]);
+      }
       // Laplace: G = kappa |det J| w J^-1 J^-T = (kappa w / |det J|) A A^T
       // with A = adj(J); mass: kappa |det J| w
       double A[D][D];
@@ -822,7 +900,8 @@ struct BatchTile {
         double h2 = 0.0;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          const double e = X[D + d] - X[d];
+          // x_1 - x_0 (the multilinear coefficient c_{0} holds it already)
+          const double e = kMultilinear ? X[D + d] : X[D + d] - X[d];
           h2 += e * e;
         }
         const double h = sqrt(h2);
@@ -887,7 +966,7 @@ struct BatchTile {
             v[u] = (i < KS && a < NS && q < NQ) ? (float)tab[(q * NS + a) * TP + d] : 0.0f;
           }
           f16x2 hi, lo;
-          split_f16(v[0], v[1], hi, lo);
+          split_f16(v[0], v[1], -1.0f, hi, lo);
           bh[mb][t][2 * p] = hi[0];
           bh[mb][t][2 * p + 1] = hi[1];
           bl[mb][t][2 * p] = lo[0];
@@ -912,6 +991,7 @@ struct BatchTile {
 #pragma unroll
     for (int mb = 0; mb < MB; mb++) c[mb] = V4{0, 0, 0, 0};
     if constexpr (kSplit) {
+      const float m1 = opaque_minus_one();
       float w[KT * 8];  // this lane's W values in the split product's K order
 #pragma unroll
       for (int j = 0; j < QG; j++) {
@@ -939,7 +1019,7 @@ struct BatchTile {
 #pragma unroll
         for (int p = 0; p < 4; p++) {
           f16x2 hi, lo;
-          split_f16(w[8 * t + 2 * p], w[8 * t + 2 * p + 1], hi, lo);
+          split_f16(w[8 * t + 2 * p], w[8 * t + 2 * p + 1], m1, hi, lo);
           wh[2 * p] = hi[0];
           wh[2 * p + 1] = hi[1];
           wl[2 * p] = lo[0];
@@ -998,7 +1078,10 @@ struct BatchTile {
     double us[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (kSplit) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) us[r] = (double)l.G[__shfl((int)(cur.sb & 0xFFFF), 4 * g + r) * GS + GS - 1];
+      for (int r = 0; r < 4; r++) {
+        const int sr = __shfl((int)cur.sb, 4 * g + r) & 0xFFFF;  // 16 bits: a 24-bit multiply
+        us[r] = (double)l.G[sr * GS + GS - 1];
+      }
     }
     // step (h, r) adds slot 4g+r of rank 4h..4h+3: the slots of one column
     // in one step are 4 ranks apart, never in one window, so no two lanes of
