@@ -165,6 +165,11 @@ static int resolve_scratch(int device) {
 
 // Zero fill for the no-alloc path (a kernel, not a memset node, so callers
 // may capture the call in a graph).
+// clears a batched launch's item counter: the first node of its graph
+__global__ void claim_reset_kernel(int *counter) {
+  if (threadIdx.x == 0) counter[threadIdx.x] = 0;
+}
+
 __global__ void zero_fill_kernel(double *p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = 0.0;
@@ -1571,7 +1576,7 @@ struct gtk_batch {
   };
   std::vector<std::unique_ptr<Load>> loads;  // per problem
   std::vector<gtk::VectorArgs> vargs;       // host copy of d_loads
-  DevBuf d_problems, d_loads, d_types, d_fkinds, d_items;
+  DevBuf d_problems, d_loads, d_types, d_fkinds, d_items, d_counter;
   gtk::KernelInfo k;
   size_t lds = 0;
   int nitems = 0, grid = 0;
@@ -1649,6 +1654,7 @@ int gtk_batch_destroy(gtk_batch *b) {
   b->d_types.release();
   b->d_fkinds.release();
   b->d_items.release();
+  b->d_counter.release();
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1731,7 +1737,10 @@ int gtk_batch_build(gtk_batch *b) {
   std::vector<gtk::MatrixArgs> args;
   std::vector<gtk::VectorArgs> vargs(b->ctxs.size());
   std::vector<int32_t> types, fkinds(b->ctxs.size(), -1);
-  std::vector<int2> items;
+  // work items with their estimated cost (cells x shape functions per
+  // cell): matrix tiles, then load-vector tiles, each by descending cost so
+  // that the claims of the last workgroups are short items
+  std::vector<std::pair<int64_t, int2>> mitems, vitems;
   b->lds = 0;
   b->cells = b->nnz = 0;
   for (size_t p = 0; p < b->ctxs.size(); p++) {
@@ -1746,7 +1755,9 @@ int gtk_batch_build(gtk_batch *b) {
     b->lds = std::max(b->lds, ep->lds);
     args.push_back(matrix_args(ctx, *P, *ep));
     types.push_back(type);
-    for (int t = 0; t < ep->tiles.ntiles; t++) items.push_back(make_int2((int)p, t));
+    const int64_t ns = elem_key(ctx).ns;
+    for (int t = 0; t < ep->tiles.ntiles; t++)
+      mitems.push_back({ns * (ep->tiles.tile_cell_ptr[t + 1] - ep->tiles.tile_cell_ptr[t]), make_int2((int)p, t)});
     b->cells += ctx->n_cells;
     b->nnz += P->pat.nnz;
     b->plans.push_back(std::move(ep));
@@ -1765,8 +1776,21 @@ int gtk_batch_build(gtk_batch *b) {
     b->lds = std::max(b->lds, vk.table_bytes + (size_t)vp->tiles.max_tile_cells * vk.geo_bytes_per_cell);
     vargs[p] = vector_args(ctx, *vp, l.f.as<double>());
     fkinds[p] = l.fkind;
-    for (int t = 0; t < vp->tiles.ntiles; t++) items.push_back(make_int2((int)p, ~t));
+    const int64_t ns = elem_key(ctx).ns;
+    for (int t = 0; t < vp->tiles.ntiles; t++)
+      vitems.push_back({ns * (vp->tiles.tile_cell_ptr[t + 1] - vp->tiles.tile_cell_ptr[t]), make_int2((int)p, ~t)});
   }
+  auto by_cost = [](const std::pair<int64_t, int2> &x, const std::pair<int64_t, int2> &y) { return x.first > y.first; };
+  std::stable_sort(mitems.begin(), mitems.end(), by_cost);
+  std::stable_sort(vitems.begin(), vitems.end(), by_cost);
+  std::vector<int2> items;
+  items.reserve(mitems.size() + vitems.size());
+  for (const auto &m : mitems) items.push_back(m.second);
+  for (const auto &v : vitems) items.push_back(v.second);
+  // the claim slots after the largest tile's LDS
+  const int claim_offset = (int)((b->lds + 15) & ~(size_t)15);
+  b->lds = (size_t)claim_offset + 16;
+  const int zero = 0;
   int rc;
   b->args = args;
   b->vargs = vargs;
@@ -1775,6 +1799,7 @@ int gtk_batch_build(gtk_batch *b) {
   if ((rc = batch_upload(b, b->d_fkinds, fkinds.data(), fkinds.size()))) return rc;
   if ((rc = batch_upload(b, b->d_types, types.data(), types.size()))) return rc;
   if ((rc = batch_upload(b, b->d_items, items.data(), items.size()))) return rc;
+  if ((rc = batch_upload(b, b->d_counter, &zero, 1))) return rc;
   b->nitems = (int)items.size();
   if (b->lds > 160 * 1024) BFAIL(b, GTK_EUNSUP, "a problem's tile does not fit in LDS");
   if (b->lds > 64 * 1024)
@@ -1791,9 +1816,13 @@ int gtk_batch_build(gtk_batch *b) {
   ba.fkinds = b->d_fkinds.as<int32_t>();
   ba.items = b->d_items.as<int2>();
   ba.nitems = b->nitems;
+  ba.counter = b->d_counter.as<int>();
+  ba.claim_offset = claim_offset;
   void *kargs[] = {&ba};
   BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t le = hipLaunchKernel(b->k.fn, dim3(b->grid), dim3(b->k.block), kargs, b->lds, b->stream);
+  hipLaunchKernelGGL(claim_reset_kernel, dim3(1), dim3(64), 0, b->stream, ba.counter);
+  hipError_t le = hipGetLastError();
+  if (le == hipSuccess) le = hipLaunchKernel(b->k.fn, dim3(b->grid), dim3(b->k.block), kargs, b->lds, b->stream);
   hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
   if (le != hipSuccess) BFAIL(b, GTK_EDEVICE, std::string("hipLaunchKernel: ") + hipGetErrorString(le));
   if (ce != hipSuccess) BFAIL(b, GTK_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));

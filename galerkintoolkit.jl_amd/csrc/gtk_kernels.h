@@ -76,8 +76,10 @@ struct BatchedArgs {
   const VectorArgs *loads;  // per problem (read for vector items only)
   const int32_t *types;     // 0: Q1, 1: Q2, 2: Q3 quadrilaterals
   const int32_t *fkinds;    // load kind per problem (kFConst, kFCell, kFNodal)
-  const int2 *items;
+  const int2 *items;  // in descending order of estimated cost within matrix and load-vector items
   int nitems;
+  int *counter;      // items claimed so far in this launch (cleared by the launch's first graph node)
+  int claim_offset;  // byte offset in LDS of the workgroup's two claim slots
 };
 
 // nl: local faces of the cell for boundary items (face mode), 0 for cells
