@@ -267,6 +267,9 @@ def bench_elasticity(GT, args, n):
     launch = tuned_launch(GT, args, ctx, form, V, L.GTK_ELASTICITY)
     wall64, kern64 = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_FP64, args.steps, args.warmup)
     wallmx, kernmx = time_form(ctx, L, L.GTK_ELASTICITY, L.GTK_MIXED, args.steps, args.warmup)
+    if args.no_solve:
+        return elasticity_result(args, n, cells, V, A, launch, wall64, kern64, wallmx, kernmx, rel_err, mesh,
+                                 None, None, setup_s)
     # the FP32 assembly is the one solved (mixed refinement to 1e-10); the
     # FP64 solve of the FP64-assembled system to 1e-11 is the reference
     A = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
@@ -287,6 +290,13 @@ def bench_elasticity(GT, args, n):
            "note": "x: mixed assembly, FP32 Jacobi-PCG inside FP64 refinement to |b - A x| <= 1e-10 |b|; "
                    "reference: FP64 assembly, FP64 Jacobi-PCG to 1e-11; the difference includes the assembly's "
                    "FP32 rounding (rel_fro_err_vs_fp64 of the matrix) amplified by the condition number"}
+    return elasticity_result(args, n, cells, V, A, launch, wall64, kern64, wallmx, kernmx, rel_err, mesh, st, sol,
+                             setup_s)
+
+
+def elasticity_result(args, n, cells, V, A, launch, wall64, kern64, wallmx, kernmx, rel_err, mesh, st, sol,
+                      setup_s):
+    """The p2tet line; st/sol None: assembly only (--no-solve)."""
     nd = 30
     bytes_alg = alg_bytes(3, mesh.coords.shape[0], cells, 4, nd, 2, A.nnz)
     achieved = bytes_alg / (kernmx * 1e-3) / 1e9
@@ -314,8 +324,8 @@ def bench_elasticity(GT, args, n):
         "fp64": {"ms_per_step": wall64, "kernel_ms": kern64, "cells_per_s": cells / (wall64 * 1e-3)},
         "mixed": {"kernel_ms": kernmx, "rel_fro_err_vs_fp64": rel_err, "speedup_vs_fp64": kern64 / kernmx},
         "bytes_alg_per_launch": bytes_alg,
-        "solve": {"rel_residual": st.rel_residual, "outer_iterations": st.outer_iterations,
-                  "inner_iterations": st.inner_iterations, "ms": st.solve_ms, **sol},
+        "solve": None if st is None else {"rel_residual": st.rel_residual, "outer_iterations": st.outer_iterations,
+                                          "inner_iterations": st.inner_iterations, "ms": st.solve_ms, **sol},
         "setup_s": setup_s,
     }
 
@@ -468,6 +478,7 @@ def main():
     ap.add_argument("--cells", type=int, default=0, help="cells per direction (default per workload)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the single-thread CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="p2tet: assembly only (profiling runs)")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (HIP runtime) so hardware "
