@@ -16,7 +16,8 @@ struct SolveMat {
   const int32_t *idx = nullptr;  // rowval, 1-based
   int lanes = 16;                // lanes per row of the SpMV
   // BSR(3) copy for the FP32 iterations (null: none): block columns and the
-  // FP32 values by block entry, bval[(3i+j) * nb + block]
+  // FP32 values: entries e = 3i+j < 8 of a block at bval[8 * block + e]
+  // (two 16-byte loads), entry 8 at bval[8 * nb + block]
   const int32_t *bcol = nullptr;
   const float *bval = nullptr;
   int64_t nb = 0;        // blocks

@@ -227,13 +227,16 @@ __global__ __launch_bounds__(kRB) void block3_build_kernel(int64_t nbr, int64_t 
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) bval[(3 * i + j) * nb + b] = val[ptr[3 * I + i] - 1 + 3 * m + j];
+        for (int j = 0; j < 3; j++) {
+          const int e = 3 * i + j;
+          bval[e < 8 ? 8 * b + e : 8 * nb + b] = val[ptr[3 * I + i] - 1 + 3 * m + j];
+        }
     }
   }
 }
 
 // y = A x over the BSR(3) copy, FP32; S lanes per block row, two blocks per
-// lane in flight
+// lane in flight, a block's first 8 values in two 16-byte loads
 template <int S>
 __global__ __launch_bounds__(kRB) void bsr3_spmv_kernel(int64_t nbr, int64_t nb, const int32_t *__restrict__ ptr,
                                                         const int32_t *__restrict__ bcol,
@@ -252,8 +255,18 @@ __global__ __launch_bounds__(kRB) void bsr3_spmv_kernel(int64_t nbr, int64_t nb,
       for (int u = 0; u < 2; u++) {
         const bool in = bb[u] < b1;
         c[u] = in ? 3 * bcol[bb[u]] : 0;
-#pragma unroll
-        for (int e = 0; e < 9; e++) v[u][e] = in ? bval[e * nb + bb[u]] : 0.f;
+        const int64_t bk = in ? bb[u] : 0;  // past the row: block 0, values unused
+        const float4 *q = reinterpret_cast<const float4 *>(bval + 8 * bk);
+        const float4 lo = q[0], hi = q[1];
+        v[u][0] = in ? lo.x : 0.f;
+        v[u][1] = in ? lo.y : 0.f;
+        v[u][2] = in ? lo.z : 0.f;
+        v[u][3] = in ? lo.w : 0.f;
+        v[u][4] = in ? hi.x : 0.f;
+        v[u][5] = in ? hi.y : 0.f;
+        v[u][6] = in ? hi.z : 0.f;
+        v[u][7] = in ? hi.w : 0.f;
+        v[u][8] = in ? bval[8 * nb + bk] : 0.f;
       }
 #pragma unroll
       for (int u = 0; u < 2; u++)
