@@ -1234,6 +1234,37 @@ struct BatchTile {
       if (lane == 0) v = atomicAdd(l.claim, 1);
       return v;
     };
+    if constexpr (D == 2 && sizeof(CT) == 4) {
+      // records two batches ahead: a 2D batch of the FP32 product is too
+      // short to cover a record load's latency (256-problem 2D batch,
+      // matrices: 3.50 -> 3.20 ms; the FP64 form unchanged, Q1 hexahedra
+      // 0.745 -> 0.823 ms, so only here)
+      int b1 = bt0 + wave, b2 = bt1, pend = bt1 - bt0;
+      BatchRegs<OffT, MB> n1, n2;
+      if (b1 < bt1) {
+        load_batch<CT, OffT, NS, MB>(a.rec, a, b1, lane, n1);
+        b2 = bt0 + (int)__builtin_amdgcn_readfirstlane(claim_issue());
+        if (b2 < bt1) {
+          load_batch<CT, OffT, NS, MB>(a.rec, a, b2, lane, n2);
+          pend = claim_issue();
+        }
+      }
+      while (b1 < bt1) {
+        const BatchRegs<OffT, MB> cur = n1;
+        n1 = n2;
+        b1 = b2;
+        if (b2 < bt1) {
+          b2 = bt0 + (int)__builtin_amdgcn_readfirstlane(pend);
+          if (b2 < bt1) {
+            load_batch<CT, OffT, NS, MB>(a.rec, a, b2, lane, n2);
+            pend = claim_issue();
+          }
+        }
+        batch(a, l, acc, bfrag, bh, bl, cur, lane);
+        between();
+      }
+      return;
+    }
     int bi = bt0 + wave, pend = bt1 - bt0;
     BatchRegs<OffT, MB> nxt;
     if (bi < bt1) {
