@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libgtk_oracle.so")
+# GTK_ORACLE_LIB: another build of the same sources (the sanitizer test's)
+LIB = os.environ.get("GTK_ORACLE_LIB") or os.path.join(HERE, "libgtk_oracle.so")
 
 POINT, SEG, QUAD, HEX, TRI, TET = range(6)
 LAPLACE, MASS, ELASTICITY = 0, 1, 2
