@@ -117,6 +117,10 @@ struct gtk_ctx {
   std::vector<int32_t> item_face;
   // device copies
   DevBuf d_xyz, d_cell_nodes, d_cell_dofs, d_w, d_phi, d_gphi, d_gN, d_coef, d_face_jac, d_item_face, d_face_normal;
+  // affine simplices: R[a][b][m][n] = sum_q w_q d_m phi_a(q) d_n phi_b(q),
+  // the reference integrals of every node pair (FP64, formed once per
+  // reference table set)
+  DevBuf d_rtab;
   bool has_face_grad = false;  // face mode: reference gradients and normals set (Nitsche terms)
   // wavefronts per workgroup of the matrix-core kernels; 0 = per kernel:
   // 16 or 12 (four or three per SIMD) when that instantiation fits its
@@ -465,6 +469,19 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
     for (int s = 0; s < nshape; s++)
       for (int d = 0; d < D; d++) ctx->gphi_t[((size_t)q * D + d) * nshape + s] = grad_phi[((size_t)q * nshape + s) * D + d];
   ctx->gN.assign(grad_geo, grad_geo + (size_t)nq * ctx->nv * D);
+  std::vector<double> rtab;
+  if (ctx->nv == D + 1) {
+    // in the order a kernel forming it per workgroup would: r = fma(w_q d_m
+    // phi_a, d_n phi_b, r) over q ascending
+    rtab.assign((size_t)nshape * nshape * D * D, 0.0);
+    for (size_t i = 0; i < rtab.size(); i++) {
+      const size_t ab = i / (D * D), mn = i % (D * D), ia = ab / nshape, ib = ab % nshape, m = mn / D, n = mn % D;
+      double r = 0.0;
+      for (int q = 0; q < nq; q++)
+        r = std::fma(ctx->w[q] * ctx->gphi_t[((size_t)q * D + m) * nshape + ia], ctx->gphi_t[((size_t)q * D + n) * nshape + ib], r);
+      rtab[i] = r;
+    }
+  }
   ctx->nfaces = 0;
   ctx->has_face_grad = false;
   for (auto &mp : ctx->mplans) mp.second->exec.clear();  // kernels depend on the tables
@@ -474,6 +491,7 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
   if ((rc = upload(ctx, ctx->d_phi, ctx->phi.data(), ctx->phi.size()))) return rc;
   if ((rc = upload(ctx, ctx->d_gphi, ctx->gphi_t.data(), ctx->gphi_t.size()))) return rc;
   if ((rc = upload(ctx, ctx->d_gN, ctx->gN.data(), ctx->gN.size()))) return rc;
+  if (!rtab.empty() && (rc = upload(ctx, ctx->d_rtab, rtab.data(), rtab.size()))) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->has_ref = true;
   ctx->generation++;
@@ -959,6 +977,7 @@ static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, c
   a.phi = ctx->d_phi.as<double>();
   a.gphi = ctx->d_gphi.as<double>();
   a.gN = ctx->d_gN.as<double>();
+  a.rtab = ctx->d_rtab.as<double>();
   a.tile_cell_ptr = ep->tile_cell_ptr.as<int32_t>();
   a.tile_cells = ep->tile_cells.as<int32_t>();
   a.tile_col_ptr = ep->tile_col_ptr.as<int32_t>();

@@ -28,6 +28,7 @@ struct MatrixArgs {
   const int32_t *cell_nodes;
   const double *coef;
   const double *w, *phi, *gphi, *gN;
+  const double *rtab;  // affine simplices: reference integrals R[a][b][m][n] (NS*NS*D*D)
   const int32_t *tile_cell_ptr, *tile_cells, *tile_col_ptr, *col_id, *col_ctr_ptr;
   const int32_t *tile_batch_ptr;
   const uint16_t *ctr_slot, *ctr_b;

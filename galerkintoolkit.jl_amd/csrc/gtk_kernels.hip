@@ -244,13 +244,9 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
   int32_t *lf_s = reinterpret_cast<int32_t *>(G_s + ncell * GQ * NGS);  // [cells] local face (NL > 0)
 
   if constexpr (kCellGeo) {
-    // R[a][b][m][n] = sum_q w_q d_m phi_a(q) d_n phi_b(q), in FP64
-    for (int i = threadIdx.x; i < NGPHI; i += kVB) {
-      const int ab = i / (D * D), mn = i - ab * D * D, ia = ab / NS, ib = ab - ia * NS, m = mn / D, n = mn - m * D;
-      double r = 0.0;
-      for (int q = 0; q < NQ; q++) r = fma(args.w[q] * args.gphi[(q * D + m) * NS + ia], args.gphi[(q * D + n) * NS + ib], r);
-      gphi_s[i] = (CT)r;
-    }
+    // R[a][b][m][n] = sum_q w_q d_m phi_a(q) d_n phi_b(q), formed in FP64
+    // once per reference table set (gtk_set_reference)
+    for (int i = threadIdx.x; i < NGPHI; i += kVB) gphi_s[i] = (CT)args.rtab[i];
   } else {
     for (int i = threadIdx.x; i < NGPHI; i += kVB) gphi_s[i] = (CT)args.gphi[i];
   }
