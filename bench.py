@@ -240,7 +240,40 @@ def bench_laplace(GT, args, n, order, degree, perturb, lognormal_kappa, label):
                         "note": "mixed assembly with the values copied to host memory (not the value)"},
         "bytes_alg_per_launch": bytes_alg,
         "setup_s": setup_s,
+        "refined_solve": (refined_solve(GT, mesh, order, degree, kappa)
+                          if label == "q2hex" and not args.no_solve else None),
     }
+
+
+def refined_solve(GT, mesh, order, degree, kappa):
+    """The refined-solution check of the mixed-precision path on the same
+    mesh with Dirichlet values on the whole boundary and f = 1: the FP64
+    reference (FP64 assembly, FP64 Jacobi-PCG to 1e-11); the mixed
+    refinement of the FP64 assembly (FP32 Jacobi-PCG on its FP32 copy inside
+    FP64 residuals with the FP64 values, to 1e-10); the mixed refinement of
+    the mixed assembly (its residuals use the mixed values)."""
+    Om = GT.interior(mesh)
+    V = GT.lagrange_space(Om, order, dirichlet_boundary=GT.boundary(mesh))
+    dO = GT.measure(Om, degree)
+    form = GT.Laplace(dO, GT.face_constant_field(kappa, Om) if kappa is not None else None)
+    b = GT.assemble_vector(GT.Load(dO, 1.0), float, V)
+    A64 = GT.assemble_matrix(form, float, V, V, precision=GT.FP64)
+    x64, st64 = GT.solve(A64, b, precision=GT.FP64, rtol=1e-11)
+    xr, str_ = GT.solve(A64, b, precision=GT.MIXED, rtol=1e-10)
+    Amx = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+    xm, stm = GT.solve(Amx, b, precision=GT.MIXED, rtol=1e-10)
+    nx = np.linalg.norm(x64)
+
+    def rep(x, st):
+        return {"rel_err_vs_fp64_solve": float(np.linalg.norm(x - x64) / nx), "rel_residual": st.rel_residual,
+                "outer_iterations": st.outer_iterations, "inner_iterations": st.inner_iterations, "ms": st.solve_ms}
+    return {"dofs": int(V.nfree), "nnz": int(A64.nnz),
+            "fp64_reference": {"rel_residual": st64.rel_residual, "inner_iterations": st64.inner_iterations,
+                               "ms": st64.solve_ms},
+            "mixed_refinement_on_fp64_assembly": rep(xr, str_),
+            "mixed_refinement_on_mixed_assembly": rep(xm, stm),
+            "note": "Dirichlet on the whole boundary, f = 1; the error of the second against the FP64 solve is the "
+                    "assembly's FP32 contraction rounding amplified by the condition number"}
 
 
 def bench_elasticity(GT, args, n):
@@ -478,7 +511,8 @@ def main():
     ap.add_argument("--cells", type=int, default=0, help="cells per direction (default per workload)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="cells per direction of the single-thread CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-solve", action="store_true", help="p2tet: assembly only (profiling runs)")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="assembly only: no refined solves (q2hex, p2tet; profiling runs)")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (HIP runtime) so hardware "
