@@ -1,0 +1,209 @@
+"""Pins of the discontinuous-space and skeleton oracle (oracle/skeleton.py),
+on the CPU.
+
+- DG numbering: a hand-derived 2x2 Q1 case (cell-by-cell ids, the Dirichlet
+  tags of the left face's closure), and on larger meshes: Dirichlet exactly
+  at the nodes on the tagged boundary, free ids ascending in (cell, node)
+  order.
+- The DG volume matrix, gathered through the map from continuous to DG
+  DOFs (P^T A_dg P), equals the C oracle's continuous assembly.
+- The penalty term on 2x2 unit squares equals the closed form: the face
+  mass of the linear traces, [[1/3, 1/6], [1/6, 1/3]], with sign n_i . n_j
+  per side pair; the skeleton matrix is linear in gamma.
+- On a continuous space every jump vanishes, so the whole skeleton matrix
+  is zero up to rounding while its face blocks are not: a normal that does
+  not flip between the sides, or a mean without the factor 1/2, fails it.
+- The matrix is symmetric (the third term is the transpose of the second).
+- The reference's own patch tests for the method
+  (GalerkinToolkitExamples/test/poisson_tests.jl:31-44: interior penalty with
+  Nitsche and with strong Dirichlet conditions, a Neumann side, Q2, exact
+  solution to 1e-10): the discrete solution equals u = x + y at every DOF,
+  and u quadratic on affine meshes; the consistency term
+  -jump(v).mean(grad u) with a wrong sign or factor breaks it."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from boundary import Boundary
+from nitsche import NitscheFaces
+from skeleton import DGSpace, Skeleton
+
+
+def problem(GT, O, cells, order, box, perturb, seed=7):
+    m = GT.cartesian_mesh(tuple(box), cells)
+    x = GT.perturb_interior_nodes(m, perturb, seed) if perturb else m.coords
+    p = O.Problem(list(cells), box=list(box)).space(order, dirichlet=0)
+    p.set_coords(x)
+    p.quadrature(2 * order)
+    return m, x, p
+
+
+def test_dg_numbering_hand_derived(GT, O):
+    # 2x2 Q1 cells of side 1 (c0 [0,1]^2, c1 right of it, c2 above it, c3),
+    # local nodes (0,0) (1,0) (0,1) (1,1); ids before the split: cell c
+    # gets 4c+1 .. 4c+4; the face x = 0 tags nodes 0 and 2 of c0 and c2 (ids
+    # 1, 3, 9, 11): the free ids 2,4,5,6,7,8,10,12,13..16 become 1..12 and
+    # the tagged ones -1..-4
+    _, _, p = problem(GT, O, (2, 2), 1, [0.0, 2.0, 0.0, 2.0], 0.0)
+    dg = DGSpace(p, 1, dirichlet=lambda y: abs(y[0]) < 1e-12)
+    assert dg.cd.tolist() == [[-1, 1, -2, 2], [3, 4, 5, 6], [-3, 7, -4, 8], [9, 10, 11, 12]]
+    assert (dg.nfree, dg.ndiri) == (12, 4)
+    dg0 = DGSpace(p, 1)
+    assert dg0.cd.tolist() == [[1, 2, 3, 4], [5, 6, 7, 8], [9, 10, 11, 12], [13, 14, 15, 16]]
+
+
+@pytest.mark.parametrize("cells,order", [((3, 2), 2), ((2, 2, 2), 1), ((3, 3), 3)])
+def test_dg_numbering_rules(GT, O, cells, order):
+    D = len(cells)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, 0.0)
+    on_bnd = lambda y: np.any(np.abs(y) < 1e-12) or np.any(np.abs(y - 1.0) < 1e-12)
+    dg = DGSpace(p, order, dirichlet=lambda y: on_bnd(y))
+    pos = dg.geo.cell_pos
+    diri = dg.cd < 0
+    for c in range(dg.cd.shape[0]):
+        for a in range(dg.nd):
+            assert diri[c, a] == on_bnd(pos[c, a])
+    free = dg.cd[~diri]
+    assert np.array_equal(free, np.arange(1, dg.nfree + 1))  # row-major (cell, node) order
+    assert np.array_equal(-dg.cd[diri], np.arange(1, dg.ndiri + 1))
+
+
+@pytest.mark.parametrize("cells,order,perturb", [((3, 3), 2, 0.2), ((3, 2), 1, 0.2), ((2, 2, 2), 1, 0.15)])
+def test_dg_volume_gathers_to_continuous(GT, O, cells, order, perturb):
+    D = len(cells)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb)
+    cp, rv, nz = p.assemble_matrix(O.LAPLACE)
+    n = len(cp) - 1
+    K = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(n, n)).toarray()
+    dg = DGSpace(p, order)
+    Kdg = dg.laplace(2 * order).toarray()
+    cd = p.cell_dofs()
+    P = np.zeros((dg.nfree, n))
+    for c in range(cd.shape[0]):
+        for a in range(cd.shape[1]):
+            P[dg.cd[c, a] - 1, cd[c, a] - 1] = 1.0
+    assert np.abs(P.T @ Kdg @ P - K).max() <= 1e-13 * np.abs(K).max()
+
+
+def test_penalty_closed_form(GT, O):
+    # 2x2 unit Q1 squares: every interior face has length 1 (h = 1) and the
+    # penalty block of a side pair (i, j) is n_i . n_j times the face mass of
+    # the linear traces at the face's two nodes
+    _, _, p = problem(GT, O, (2, 2), 1, [0.0, 2.0, 0.0, 2.0], 0.0)
+    dg = DGSpace(p, 1)
+    sk = Skeleton(p, 1, dg.cd)
+    S = sk.matrix(2, 1.0, shape=(16, 16), terms=(1, 0, 0)).toarray()
+    M = np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+    ref = np.zeros((16, 16))
+    # (cell, its nodes on the face) for both sides, nodes in matching order
+    faces = [((0, [1, 3]), (1, [0, 2])), ((2, [1, 3]), (3, [0, 2])),
+             ((0, [2, 3]), (2, [0, 1])), ((1, [2, 3]), (3, [0, 1]))]
+    for (ca, na), (cb, nb) in faces:
+        A = [4 * ca + k for k in na]
+        B = [4 * cb + k for k in nb]
+        ref[np.ix_(A, A)] += M
+        ref[np.ix_(B, B)] += M
+        ref[np.ix_(A, B)] -= M
+        ref[np.ix_(B, A)] -= M
+    assert np.abs(S - ref).max() < 1e-15
+    S3 = sk.matrix(2, 3.0, shape=(16, 16)).toarray()
+    S1 = sk.matrix(2, 1.0, shape=(16, 16)).toarray()
+    assert np.abs((S3 - S1) / 2.0 - ref).max() < 1e-14
+
+
+@pytest.mark.parametrize("cells,order,perturb", [((3, 3), 2, 0.2), ((3, 2), 1, 0.2), ((2, 2, 2), 1, 0.1)])
+def test_continuous_jumps_vanish(GT, O, cells, order, perturb):
+    D = len(cells)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb)
+    sk = Skeleton(p, order, p.cell_dofs())
+    n = O.Problem.q(p, O.Q_NFREE)
+    S = sk.matrix(2 * order, 4.0, shape=(n, n)).toarray()
+    block = max(np.abs(sk.face_blocks(f, 2 * order, 4.0)[1]).max() for f in sk.geo.interior[:4])
+    assert block > 1e-2
+    assert np.abs(S).max() <= 1e-13 * block
+
+
+@pytest.mark.parametrize("cells,order,perturb", [((3, 3), 2, 0.2), ((2, 2, 2), 1, 0.1)])
+def test_symmetry(GT, O, cells, order, perturb):
+    D = len(cells)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb)
+    dg = DGSpace(p, order)
+    S = Skeleton(p, order, dg.cd).matrix(2 * order, 2.0, shape=(dg.nfree, dg.nfree)).toarray()
+    assert np.abs(S - S.T).max() <= 1e-13 * np.abs(S).max()
+
+
+def dg_patch(GT, O, cells, order, perturb, kind, dirichlet_method, degree):
+    """Interior penalty on the box (0,2)^D, Dirichlet on every side but
+    y = 2 (Neumann there), as GalerkinToolkitExamples/test/poisson_tests.jl:
+    31-44 (tags 1-face-1, -3, -4 Dirichlet, 1-face-2 Neumann)."""
+    D = len(cells)
+    _, _, p = problem(GT, O, cells, order, [0.0, 2.0] * D, perturb)
+    if kind == "linear":
+        u = lambda y: float(np.sum(y))  # params[:u] = sum
+        f = lambda y: 0.0
+        gn = 1.0  # n . grad u on y = 2
+    else:
+        u = lambda y: 1.0 + y[0] - y[1] + y[0] ** 2 - 0.5 * y[0] * y[1] + 0.25 * y[1] ** 2
+        f = lambda y: -2.5
+        gn = None
+    neumann = lambda y: abs(y[1] - 2.0) < 1e-12
+    # Dirichlet sides x = 0, x = 2, y = 0 (a face is selected when all its
+    # vertices satisfy the predicate)
+    dirichlet = lambda y: abs(y[0]) < 1e-12 or abs(y[0] - 2.0) < 1e-12 or abs(y[1]) < 1e-12
+    # gamma = degree (degree + 1) / 10 (poisson.jl:62)
+    gamma = degree * (degree + 1) / 10.0
+    strong = dirichlet_method == "strong"
+    diri_sel = dirichlet if strong else None
+    dg = DGSpace(p, order, dirichlet=diri_sel)
+    n = dg.nfree
+    K = dg.laplace(degree)
+    S = Skeleton(p, order, dg.cd).matrix(degree, gamma, shape=(n, n))
+    b = dg.load(degree, f)
+    # Neumann side: g = n . grad u (the outward normal of y = 2 is e_y)
+    bd = Boundary(p, order, 1, select=neumann)
+    bd.cd, bd.nfree = dg.cd, dg.nfree
+    if gn is not None:
+        b = b + bd.vector(degree, 0, [gn], n=n)
+    else:
+        # quadratic u: g = d u / dy = -1 - 0.5 x + 0.5 y at y = 2, nodal in Q2
+        pos = dg.geo.cell_pos
+        gvals = np.zeros(dg.nfree + dg.ndiri)
+        for c in range(dg.cd.shape[0]):
+            for a in range(dg.nd):
+                y = pos[c, a]
+                d = dg.cd[c, a]
+                gvals[d - 1 if d > 0 else dg.nfree - d - 1] = -1.0 - 0.5 * y[0] + 0.5 * y[1]
+        b = b + bd.vector(degree, 2, gvals, n=n)
+    A = K + S
+    ue_all = np.zeros(dg.nfree + dg.ndiri)
+    pos = dg.geo.cell_pos
+    for c in range(dg.cd.shape[0]):
+        for a in range(dg.nd):
+            d = dg.cd[c, a]
+            ue_all[d - 1 if d > 0 else dg.nfree - d - 1] = u(pos[c, a])
+    if strong:
+        Kd = dg.laplace(degree, cols=2)
+        Sd = Skeleton(p, order, dg.cd).matrix(degree, gamma, cols=2, shape=(n, dg.ndiri))
+        b = b - (Kd + Sd) @ ue_all[n:]
+    else:
+        nf = NitscheFaces(p, order, select=dirichlet)
+        nf.cd = dg.cd
+        A = A + nf.matrix(degree, gamma, shape=(n, n))
+        b = b + nf.load(degree, gamma, u, n=n)
+    x = sla.spsolve(A.tocsc(), b)
+    return x, ue_all[:n]
+
+
+# degree 2 with strong conditions is the Examples' own case: the test leaves
+# integration_degree at its default, 2 x the default interpolation degree 1
+# (poisson.jl:484-495).  With Nitsche conditions that rule makes the system
+# singular (cond ~1e17 here: a quadratic jump vanishing at both Gauss points
+# of every face is seen by no face term), so those cases use 2 x order.
+@pytest.mark.parametrize("cells,order,perturb,kind,method,degree", [
+    ((2, 2), 2, 0.0, "linear", "strong", 2), ((2, 2), 2, 0.0, "linear", "nitsche", 4),
+    ((3, 3), 2, 0.2, "linear", "nitsche", 4), ((3, 3), 1, 0.2, "linear", "strong", 2),
+    ((3, 3), 2, 0.0, "quadratic", "nitsche", 4), ((3, 3), 2, 0.0, "quadratic", "strong", 4)])
+def test_dg_patch_exact(GT, O, cells, order, perturb, kind, method, degree):
+    x, ue = dg_patch(GT, O, cells, order, perturb, kind, method, degree)
+    assert np.abs(x - ue).max() < 1e-10 * max(1.0, np.abs(ue).max())
