@@ -5,7 +5,8 @@ Host-side mirror of the reference's interface for the assembly path
 GT.assemble_matrix/assemble_vector, GT.update_matrix!) whose numeric work runs
 in libgtk_hip.so (HIP kernels for gfx950 behind the C ABI of include/gtk_hip.h).
 """
-from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, FormSum, Laplace, Load, Mass, Measure,
+from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField, FormSum, InteriorPenalty, Laplace, Load,
+                       Mass, Measure,
                        MatrixCache, Nitsche, NitscheLoad, SparseMatrixCSC, VectorCache, assemble_matrix, assemble_matrix_and_vector,
                        assemble_matrix_and_vector_with_free_and_dirichlet_columns,
                        assemble_matrix_with_free_and_dirichlet_columns, assemble_vector, face_constant_field,
@@ -13,13 +14,13 @@ from .assembly import (FP64, MIXED, DiscreteField, Elasticity, FaceConstantField
                        update_matrix_and_vector_with_free_and_dirichlet_columns,
                        update_matrix_with_free_and_dirichlet_columns, update_vector)
 from .mesh import Mesh, cartesian_mesh, perturb_interior_nodes
-from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space
+from .space import DIRICHLET, FREE, Domain, LagrangeSpace, boundary, interior, lagrange_space, skeleton
 from .batch import MatrixBatch, assemble_matrices, autotune, autotune_matrix
 from . import inputs, refcell
 
 __all__ = [
     "FP64", "MIXED", "FREE", "DIRICHLET", "DiscreteField", "Elasticity", "FaceConstantField", "FormSum", "Laplace",
-    "Load", "Nitsche", "NitscheLoad",
+    "Load", "Nitsche", "NitscheLoad", "InteriorPenalty", "skeleton",
     "Mass", "Measure", "SparseMatrixCSC", "assemble_matrix", "assemble_matrix_with_free_and_dirichlet_columns",
     "assemble_vector", "face_constant_field", "linear_problem", "measure", "solve", "update_matrix", "Mesh",
     "MatrixCache", "VectorCache", "assemble_matrix_and_vector",

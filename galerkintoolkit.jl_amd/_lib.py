@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libgtk_hip.so")
 
 GTK_FREE, GTK_DIRICHLET = 1, 2
-GTK_LAPLACE, GTK_MASS, GTK_ELASTICITY, GTK_NITSCHE, GTK_NITSCHE_LOAD = 0, 1, 2, 3, 4
+GTK_LAPLACE, GTK_MASS, GTK_ELASTICITY, GTK_NITSCHE, GTK_NITSCHE_LOAD, GTK_SKELETON = 0, 1, 2, 3, 4, 5
 GTK_FP64, GTK_MIXED = 0, 1
 GTK_F_CONST, GTK_F_CELL, GTK_F_NODAL = 0, 1, 2
 GTK_NOALLOC_MAX_DOUBLES = 8 * 1024 * 1024  # scratch of gtk_assemble_vector_noalloc

@@ -43,7 +43,11 @@ class Measure:
 
     @property
     def on_boundary(self):
-        return self.domain.dim == self.domain.mesh.D - 1
+        return self.domain.dim == self.domain.mesh.D - 1 and not self.on_skeleton
+
+    @property
+    def on_skeleton(self):
+        return self.domain.is_skeleton
 
 
 def measure(domain, degree):
@@ -127,6 +131,23 @@ class Nitsche(_Form):
     (src/accessors.jl:1009-1030)."""
     dOmega: Measure
     gamma: float = 1.0
+
+
+@dataclass
+class InteriorPenalty(_Form):
+    """int (gamma/h) jump(v).jump(u) - jump(v).mean(grad u)
+    - mean(grad v).jump(u) dLambda over the interior faces (GT.skeleton):
+    the interior-penalty terms of GalerkinToolkitExamples/src/poisson.jl:
+    99-102 with jump(u) = u[1] n[1] + u[2] n[2] and mean(f) = (f[1] + f[2]) / 2
+    (poisson.jl:40-41), h the face's diameter, n[i] the unit normal out of
+    side i's cell (scalar fields on discontinuous spaces: on a continuous
+    space every jump vanishes)."""
+    dLambda: Measure
+    gamma: float = 1.0
+
+    @property
+    def dOmega(self):
+        return self.dLambda
 
 
 @dataclass
@@ -297,6 +318,102 @@ class _NitscheAssembler(_Assembler):
         return np.ascontiguousarray(float(gamma) / self.h)
 
 
+class _SkeletonAssembler(_Assembler):
+    """The device context of interior-penalty terms over a skeleton measure:
+    one item per interior face with both its cells (their nodes and DOFs,
+    the lower cell id first), and point sets of the face rule: for each
+    side, the rule's points in that cell's reference coordinates, side 1's
+    listed in the physical order of side 0's through the vertices the two
+    local faces share; the reference gradients and normals of each set's
+    local face; the face diameters."""
+
+    def __init__(self, space, dL, device=0):
+        mesh = space.mesh
+        D = mesh.D
+        if space.ncomp != 1:
+            raise NotImplementedError("skeleton terms: scalar fields")
+        if space.continuous:
+            # an item would list each DOF of the face twice, and a continuous
+            # basis function has no jump, so every term is rounding
+            raise NotImplementedError("skeleton terms on discontinuous spaces (lagrange_space(...; continuous=False))")
+        if mesh.cellkind not in (R.QUAD, R.HEX):
+            raise NotImplementedError("skeleton terms: quadrilaterals and hexahedra")
+        self.space = space
+        self.dOmega = dL
+        faces = dL.domain.faces
+        cf = mesh.cell_faces[D - 1]
+        nlf = cf.shape[1]
+        flat = cf.ravel()
+        order = np.argsort(flat, kind="stable")  # (cell, local face) slots grouped by face, cells ascending
+        sf = flat[order]
+        start = np.searchsorted(sf, faces)
+        if np.any(np.searchsorted(sf, faces, side="right") - start != 2):
+            raise ValueError("a face of the skeleton does not have two cells")
+        p0, p1 = order[start], order[start + 1]
+        cells = np.stack([p0 // nlf, p1 // nlf], axis=1)  # (items, 2)
+        lfaces = np.stack([p0 % nlf, p1 % nlf], axis=1)
+        # every cell must lie on an interior face: the skeleton pattern then
+        # holds the volume and boundary blocks of a sum
+        if not np.all(np.bincount(cells.ravel(), minlength=mesh.cells.shape[0]) > 0):
+            raise NotImplementedError("skeleton terms: every cell needs an interior face")
+        g = R.GEO[mesh.cellkind]
+        fv = np.asarray(g.faces[D - 1]) - 1  # (nlf, face vertices)
+        V0 = np.take_along_axis(mesh.cells[cells[:, 0]], fv[lfaces[:, 0]], axis=1)  # side 0's face, mesh nodes
+        V1 = np.take_along_axis(mesh.cells[cells[:, 1]], fv[lfaces[:, 1]], axis=1)
+        # perm[k]: position in side 1's local face of side 0's k-th vertex
+        perm = np.argmax(V1[:, None, :] == V0[:, :, None], axis=2)
+        if not np.all(np.take_along_axis(V1, perm, axis=1) == V0):
+            raise ValueError("the two sides of a face do not share its vertices")
+        nfv = fv.shape[1]
+        ident = tuple(range(nfv))
+        keys, item_sets = {}, np.zeros((len(faces), 2), np.int64)
+        for i in range(len(faces)):
+            for side, pk in ((0, ident), (1, tuple(int(v) for v in perm[i]))):
+                item_sets[i, side] = keys.setdefault((int(lfaces[i, side]), pk), len(keys))
+        nsets = 8 if D == 2 else 12
+        if len(keys) > nsets:
+            raise NotImplementedError(f"skeleton terms: {len(keys)} point sets (at most {nsets})")
+        ft = g.ftype[D - 1]
+        N = R.linear_shape(ft, dL.x)
+        _, dNf = R.Lagrange(ft, 1).tabulate(dL.x[:1])
+        geo = R.Lagrange(mesh.cellkind, 1)
+        fe = space.fe
+        normals = R.face_normals(mesh.cellkind)
+        nq, nv = len(dL.w), len(g.verts)
+        phi, gphi, gN, A, nrm = [], [], [], [], []
+        for (l, pk) in sorted(keys, key=keys.get):
+            X = np.asarray([g.verts[v] for v in fv[l]], np.float64)[list(pk)]
+            xc = N @ X
+            val, grad = fe.tabulate(xc)
+            phi.append(val)
+            gphi.append(grad)
+            # one cell's geometry gradients, then zeros: a set spans the
+            # item's two cells' worth of nodes
+            gN.append(np.concatenate([geo.tabulate(xc)[1].ravel(), np.zeros(nq * nv * D)]))
+            A.append(X.T @ dNf[0])
+            nrm.append(normals[l])
+        while len(phi) < nsets:  # unused sets: copies of the first
+            phi.append(phi[0]), gphi.append(gphi[0]), gN.append(gN[0]), A.append(A[0]), nrm.append(nrm[0])
+        Xf = mesh.coords[V0]
+        diff = Xf[:, :, None, :] - Xf[:, None, :, :]
+        self.h = np.sqrt((diff ** 2).sum(axis=3)).max(axis=(1, 2))
+        item_nodes = mesh.cell_nodes_1b()[cells].reshape(len(faces), -1)
+        item_dofs = space.cell_dofs[cells].reshape(len(faces), -1)
+        item_face = (item_sets[:, 0] | (item_sets[:, 1] << 16)).astype(np.int32)
+        self.ctx = L.Context(device)
+        self.ctx.set_mesh(D, mesh.coords, item_nodes)
+        self.ctx.set_space(1, item_dofs, space.nfree, space.ndiri)
+        self.ctx.set_reference_faces(dL.w, np.stack(phi), np.stack(gN), np.stack(A), item_face)
+        self.ctx.set_reference_face_gradients(np.stack(gphi), np.stack(nrm))
+        self.items = len(faces)
+        self.patterns = {}
+        self.values_owner = {}
+
+    def beta(self, gamma):
+        """gamma / h per item."""
+        return np.ascontiguousarray(float(gamma) / self.h)
+
+
 def _assembler(space, dOmega, device, kind=None):
     """The context of (space, measure); kind "nitsche" (or "nitsche_load",
     a second one so the load's matrix never replaces the Nitsche matrix's
@@ -305,7 +422,9 @@ def _assembler(space, dOmega, device, kind=None):
     cache = space.__dict__.setdefault("_assemblers", {})
     a = cache.get(key)
     if a is None or a.dOmega is not dOmega:
-        if kind is not None:
+        if dOmega.on_skeleton:
+            a = _SkeletonAssembler(space, dOmega, device)
+        elif kind is not None:
             a = _NitscheAssembler(space, dOmega, device)
         else:
             a = (_BoundaryAssembler if dOmega.on_boundary else _Assembler)(space, dOmega, device)
@@ -322,13 +441,15 @@ def _form_code(form):
         return L.GTK_ELASTICITY
     if isinstance(form, Nitsche):
         return L.GTK_NITSCHE
+    if isinstance(form, InteriorPenalty):
+        return L.GTK_SKELETON
     raise TypeError("unsupported bilinear form")
 
 
 def _ncoef_items(form, space):
     """Cells (or boundary items) the form's coefficients run over."""
     dom = form.dOmega.domain
-    return len(dom.faces) if form.dOmega.on_boundary else space.mesh.cells.shape[0]
+    return len(dom.faces) if form.dOmega.domain.dim < space.mesh.D else space.mesh.cells.shape[0]
 
 
 def _coefficients(form, ncells):
@@ -398,6 +519,15 @@ def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic
         return _assemble_matrix_sum(form, space, rows, cols, precision, device, symbolic)
     if form.dOmega.on_boundary and not isinstance(form, (Mass, Nitsche)):
         raise NotImplementedError("boundary integrals of matrices: the mass form (Robin terms) and Nitsche terms")
+    if form.dOmega.on_skeleton != isinstance(form, InteriorPenalty):
+        raise NotImplementedError("skeleton integrals: the interior-penalty terms (over GT.skeleton)")
+    if isinstance(form, InteriorPenalty):
+        a = _assembler(space, form.dOmega, device)
+        pat = a.pattern(rows, cols, symbolic)
+        a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
+        prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+        nzval, stats = a.ctx.assemble_matrix(L.GTK_SKELETON, prec, rows, cols, pat[0])
+        return a, nzval, stats, pat
     if isinstance(form, Nitsche):
         if not form.dOmega.on_boundary:
             raise ValueError("Nitsche terms are integrals over boundary faces")
@@ -483,19 +613,25 @@ def _load_data(form, space):
 
 
 def _assemble_matrix_sum(fs, space, rows, cols, precision, device, symbolic):
-    """One volume integral plus boundary mass integrals: the volume block is
-    assembled, then each boundary block is added into its values on the
-    device (the boundary pattern lies inside the volume pattern)."""
-    vol = [t for t in fs.terms if not t.dOmega.on_boundary]
-    bnd = [t for t in fs.terms if t.dOmega.on_boundary]
-    if len(vol) != 1:
-        raise NotImplementedError("a sum of matrix integrals: one over the cells plus boundary integrals")
-    a_vol, _, stats, pat = _assemble_matrix_values(vol[0], space, rows, cols, precision, device, symbolic)
-    for t in bnd:
-        a_b, _, _, _ = _assemble_matrix_values(t, space, rows, cols, precision, device)
-        a_vol.ctx.add_block(rows, cols, a_b.ctx)
-    nzval = a_vol.ctx.get_values(rows, cols, pat[0])
-    return a_vol, nzval, stats, pat
+    """One volume integral plus boundary integrals and at most one skeleton
+    integral: the block whose pattern holds the others' (the skeleton's,
+    which couples the cells on either side of each interior face, else the
+    volume's) is assembled, then each other block is added into its values
+    on the device."""
+    vol = [t for t in fs.terms if t.dOmega.domain.dim == space.mesh.D]
+    skel = [t for t in fs.terms if t.dOmega.on_skeleton]
+    if len(vol) != 1 or len(skel) > 1:
+        raise NotImplementedError("a sum of matrix integrals: one over the cells, boundary integrals and at most "
+                                  "one skeleton integral")
+    base = skel[0] if skel else vol[0]
+    a_base, _, stats, pat = _assemble_matrix_values(base, space, rows, cols, precision, device, symbolic)
+    for t in fs.terms:
+        if t is base:
+            continue
+        a_t, _, _, _ = _assemble_matrix_values(t, space, rows, cols, precision, device)
+        a_base.ctx.add_block(rows, cols, a_t.ctx)
+    nzval = a_base.ctx.get_values(rows, cols, pat[0])
+    return a_base, nzval, stats, pat
 
 
 def _nitsche_load(form, space, rows, device):

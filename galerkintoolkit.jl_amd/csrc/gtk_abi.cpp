@@ -122,6 +122,7 @@ struct gtk_ctx {
   // reference table set)
   DevBuf d_rtab;
   bool has_face_grad = false;  // face mode: reference gradients and normals set (Nitsche terms)
+  bool faces_packed = false;   // item_face holds two point sets per item (skeleton items)
   // wavefronts per workgroup of the matrix-core kernels; 0 = per kernel:
   // 16 or 12 (four or three per SIMD) when that instantiation fits its
   // registers without scratch, else 8 (two per SIMD within 256 registers);
@@ -384,7 +385,7 @@ int gtk_set_mesh(gtk_ctx *ctx, int dim, int64_t n_nodes, const double *xyz, int6
   if (!ctx) return GTK_EINVAL;
   if (dim < 2 || dim > 3) FAIL(ctx, GTK_EINVAL, "dim must be 2 or 3");
   if (n_nodes <= 0 || n_cells <= 0 || !xyz || !cell_nodes) FAIL(ctx, GTK_EINVAL, "empty mesh or NULL array");
-  if (nodes_per_cell < dim + 1 || nodes_per_cell > 8) FAIL(ctx, GTK_EINVAL, "nodes_per_cell out of range");
+  if (nodes_per_cell < dim + 1 || nodes_per_cell > 16) FAIL(ctx, GTK_EINVAL, "nodes_per_cell out of range");
   if (n_nodes > INT32_MAX || n_cells > INT32_MAX) FAIL(ctx, GTK_EINVAL, "mesh too large for Int32 ids");
   std::vector<int32_t> cn((size_t)n_cells * nodes_per_cell);
   for (size_t i = 0; i < cn.size(); i++) {
@@ -502,12 +503,16 @@ int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, i
                             const double *grad_geo, const double *face_jac, const int32_t *item_face) {
   if (!ctx) return GTK_EINVAL;
   if (!ctx->has_mesh) FAIL(ctx, GTK_ESTATE, "gtk_set_mesh first");
-  if (nfaces < 1 || nfaces > 8 || nq < 1 || nq > 128 || nshape < 1 || nshape > 64 || !w || !phi || !grad_geo ||
+  if (nfaces < 1 || nfaces > 16 || nq < 1 || nq > 128 || nshape < 1 || nshape > 64 || !w || !phi || !grad_geo ||
       !face_jac || !item_face)
     FAIL(ctx, GTK_EINVAL, "invalid face tables");
   const int D = ctx->dim;
-  for (int64_t i = 0; i < ctx->n_cells; i++)
-    if (item_face[i] < 0 || item_face[i] >= nfaces) FAIL(ctx, GTK_EINVAL, "item_face out of range");
+  bool packed = false;
+  for (int64_t i = 0; i < ctx->n_cells; i++) {
+    const int32_t lo = item_face[i] & 0xFFFF, hi = item_face[i] >> 16;
+    if (item_face[i] < 0 || lo >= nfaces || hi >= nfaces) FAIL(ctx, GTK_EINVAL, "item_face out of range");
+    packed = packed || hi != 0;
+  }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ctx->nfaces = nfaces;
   ctx->nq = nq;
@@ -518,6 +523,7 @@ int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, i
   ctx->gN.assign(grad_geo, grad_geo + (size_t)nfaces * nq * ctx->nv * D);
   ctx->face_jac.assign(face_jac, face_jac + (size_t)nfaces * D * (D - 1));
   ctx->item_face.assign(item_face, item_face + ctx->n_cells);
+  ctx->faces_packed = packed;
   for (auto &mp : ctx->mplans) mp.second->exec.clear();
   ctx->vplans.clear();
   int rc;
@@ -1017,11 +1023,15 @@ static gtk::MatrixArgs matrix_args(const gtk_ctx *ctx, const MatrixPlan &plan, c
 static int launch_matrix(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision) {
   if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (form < GTK_LAPLACE || form > GTK_NITSCHE_LOAD) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (form < GTK_LAPLACE || form > GTK_SKELETON) FAIL(ctx, GTK_EINVAL, "unknown form");
   if (form == GTK_ELASTICITY && ctx->ncoef != 2) FAIL(ctx, GTK_ESTATE, "elasticity needs (lambda, mu) per cell");
   if (form != GTK_ELASTICITY && ctx->ncoef == 2) FAIL(ctx, GTK_ESTATE, "scalar forms take one coefficient per cell");
-  if ((form == GTK_NITSCHE || form == GTK_NITSCHE_LOAD) && !ctx->has_face_grad)
+  if ((form == GTK_NITSCHE || form == GTK_NITSCHE_LOAD || form == GTK_SKELETON) && !ctx->has_face_grad)
     FAIL(ctx, GTK_ESTATE, "Nitsche terms: gtk_set_reference_faces and gtk_set_reference_face_gradients first");
+  if (ctx->faces_packed && form != GTK_SKELETON)
+    FAIL(ctx, GTK_ESTATE, "item_face holds point-set pairs (skeleton items): GTK_SKELETON only");
+  if (form == GTK_SKELETON && (ctx->nv % 2 || ctx->dofs_per_cell != 2 * ctx->nshape * ctx->ncomp))
+    FAIL(ctx, GTK_EINVAL, "skeleton items: two cells' nodes and DOFs per item");
   ExecPlan *ep = nullptr;
   int rc = get_exec(ctx, plan, form, precision, ep);
   if (rc) return rc;
@@ -1066,7 +1076,7 @@ int gtk_plan_info(gtk_ctx *ctx, int form, int precision, int rows, int cols, gtk
   MatrixPlan *p = find_plan(ctx, rows, cols);
   if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (form < GTK_LAPLACE || form > GTK_NITSCHE_LOAD) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (form < GTK_LAPLACE || form > GTK_SKELETON) FAIL(ctx, GTK_EINVAL, "unknown form");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ExecPlan *ep = nullptr;
   int rc = get_exec(ctx, *p, form, precision, ep);
@@ -1091,7 +1101,7 @@ int gtk_set_launch(gtk_ctx *ctx, int form, int precision, int rows, int cols, in
   MatrixPlan *p = find_plan(ctx, rows, cols);
   if (!p) FAIL(ctx, GTK_ESTATE, "gtk_build_pattern first");
   if (precision != GTK_FP64 && precision != GTK_MIXED) FAIL(ctx, GTK_EINVAL, "unknown precision");
-  if (form < GTK_LAPLACE || form > GTK_NITSCHE_LOAD) FAIL(ctx, GTK_EINVAL, "unknown form");
+  if (form < GTK_LAPLACE || form > GTK_SKELETON) FAIL(ctx, GTK_EINVAL, "unknown form");
   if (waves != 0 && waves != 4 && waves != 8 && waves != 12 && waves != 16)
     FAIL(ctx, GTK_EINVAL, "waves must be 0 (default), 4, 8, 12 or 16");
   if (max_cells < 0 || max_cells > 4095) FAIL(ctx, GTK_EINVAL, "max_cells must be 0 (LDS-bound) .. 4095");
@@ -1256,6 +1266,7 @@ int gtk_assemble_vector(gtk_ctx *ctx, int fkind, const double *f, int rows, doub
   if (!ctx->has_ref) FAIL(ctx, GTK_ESTATE, "gtk_set_reference first");
   if (rows != GTK_FREE && rows != GTK_DIRICHLET) FAIL(ctx, GTK_EINVAL, "rows must be GTK_FREE or GTK_DIRICHLET");
   if (fkind < GTK_F_CONST || fkind > GTK_F_NODAL || !f) FAIL(ctx, GTK_EINVAL, "invalid load");
+  if (ctx->faces_packed) FAIL(ctx, GTK_ESTATE, "skeleton items carry no load vector");
   gtk::KernelInfo k;
   if (!gtk::find_vector_kernel(elem_key(ctx), fkind, k)) FAIL(ctx, GTK_EUNSUP, "no vector kernel for this element");
   HIPCHK(ctx, hipSetDevice(ctx->device));

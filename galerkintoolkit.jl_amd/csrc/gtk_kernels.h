@@ -19,8 +19,10 @@ struct ValuBlock {
 };
 
 // kNitsche: (beta v u - v n.grad u - n.grad v u) over boundary items;
-// kNitscheLoad: (beta v u - n.grad v u), the load's matrix applied to nodal g
-enum { kLaplace = 0, kMass = 1, kElasticity = 2, kNitsche = 3, kNitscheLoad = 4 };
+// kNitscheLoad: (beta v u - n.grad v u), the load's matrix applied to nodal g;
+// kSkeleton: the interior-penalty terms over interior faces, items of two
+// cells (beta jump(v).jump(u) - jump(v).mean(grad u) - mean(grad v).jump(u))
+enum { kLaplace = 0, kMass = 1, kElasticity = 2, kNitsche = 3, kNitscheLoad = 4, kSkeleton = 5 };
 enum { kFConst = 0, kFCell = 1, kFNodal = 2 };
 
 struct MatrixArgs {

@@ -157,14 +157,18 @@ template <int D, int FORM>
 struct NGeo {
   static constexpr int value =
       FORM == kLaplace ? D * (D + 1) / 2
-                       : (FORM == kMass ? 1 : (FORM == kNitsche || FORM == kNitscheLoad ? D + 1 : D * D + 3));
+                       : (FORM == kMass ? 1
+                                        : (FORM == kNitsche || FORM == kNitscheLoad
+                                               ? D + 1
+                                               : (FORM == kSkeleton ? 2 * D + 1 : D * D + 3)));
 };
 
-// lanes per column of the VALU matrix kernel: one per local row; for
-// elasticity one per node (NC = D rows each)
+// lanes per column of the VALU matrix kernel: one per local row (skeleton
+// items: the rows of both cells); for elasticity one per node (NC = D rows
+// each)
 template <int NS, int NC, int FORM>
 struct ValuLanes {
-  static constexpr int value = FORM == kElasticity ? NS : NS * NC;
+  static constexpr int value = FORM == kElasticity ? NS : (FORM == kSkeleton ? 2 * NS * NC : NS * NC);
 };
 
 // elasticity on affine cells (order-1 simplices, J the same at every
@@ -201,16 +205,28 @@ __device__ inline int sym_index(int i, int j) {
 // with dG the face measure times the weight, beta the item's coefficient
 // and n = J^-T n_ref / |J^-T n_ref| the physical unit normal
 // (src/accessors.jl:1009-1030), so that n . grad phi = (J^-1 n) . grad_ref phi.
+// Skeleton items (kSkeleton): an interior face with its two cells; NV is
+// both cells' nodes (NV / 2 each, side 0 first), the item's DOFs are both
+// cells' (NS each), and the NL tables are point sets: item_face packs the
+// set of side 0 (low 16 bits) and of side 1, each the face rule's points in
+// that cell's reference coordinates, in the same physical order.  Per
+// (item, point): beta dG and dG/2 J_i^-1 n_i per side i, n_i the unit normal
+// out of side i's cell; with s = n_i . n_j = +-1 for rows of side i and
+// columns of side j, the entry is
+//   s (beta phi_a phi_b - phi_a (m_j . gref_b) - (m_i . gref_a) phi_b)
+// (jump(v) = v_i n_i, mean(grad u) = grad u_j / 2, m_i = J_i^-1 n_i / 2).
 // BS: threads per workgroup (ValuBlock<CT>, or 256 where the accumulators of
 // that many waves would leave no LDS for a tile)
 template <int D, int NV, int NS, int NQ, int NC, int FORM, typename CT, int NL = 0, int BS = ValuBlock<CT>::value>
 __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
   constexpr int kVB = BS;
   constexpr bool kNit = FORM == kNitsche || FORM == kNitscheLoad;
-  static_assert(NL == 0 || FORM == kMass || kNit, "boundary items: mass and Nitsche forms");
-  static_assert(!kNit || (NL > 0 && NC == 1), "Nitsche terms: scalar fields on boundary items");
+  constexpr bool kSkel = FORM == kSkeleton;
+  static_assert(NL == 0 || FORM == kMass || kNit || kSkel, "boundary items: mass and Nitsche forms");
+  static_assert(!(kNit || kSkel) || (NL > 0 && NC == 1), "Nitsche and skeleton terms: scalar fields on face items");
+  constexpr int NVC = kSkel ? NV / 2 : NV;  // nodes of one cell
   constexpr int NG = NGeo<D, FORM>::value;
-  constexpr int NDOF = NS * NC;
+  constexpr int NDOF = (kSkel ? 2 : 1) * NS * NC;
   // lanes per column: one per local row, or (elasticity) one per node, that
   // lane forming the node's NC rows from one pair of physical gradients
   constexpr int LPC = ValuLanes<NS, NC, FORM>::value;
@@ -229,7 +245,7 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
   constexpr int GQ = kCellGeo ? 1 : NQ;  // factor sets per cell
   constexpr int NGS = CellGeoStride<D, FORM, NL, NV>::value;  // values per factor set
   // gradient table ([NT][NQ][D][NS]; none for the face mass), or R
-  constexpr int NGPHI = NL > 0 ? (kNit ? NT * NQ * D * NS : 0) : (kCellGeo ? NS * NS * D * D : NQ * D * NS);
+  constexpr int NGPHI = NL > 0 ? (kNit || kSkel ? NT * NQ * D * NS : 0) : (kCellGeo ? NS * NS * D * D : NQ * D * NS);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   CT *gphi_s = reinterpret_cast<CT *>(lds_raw);   // [NT][NQ][D][NS], or R [NS][NS][D][D]
   CT *phi_s = gphi_s + NGPHI;                      // [NT][NQ][NS]
@@ -260,7 +276,44 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
     const int64_t cell = args.tile_cells[cell0 + s];
     double Ji[D][D];
     double dV;
-    if constexpr (NL > 0) {
+    if constexpr (kSkel) {
+      // both sides' maps at the point (the same physical point); the face
+      // measure from side 0's
+      const int sets = args.item_face[cell];
+      CT *G = G_s + (s * GQ + q) * NGS;
+      G[0] = 0;
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        const int l = side ? sets >> 16 : sets & 0xFFFF;
+        double J[D][D];
+        cell_jacobian_only<D, NVC>(args.xyz, args.cell_nodes, args.gN + (size_t)l * NQ * NV * D, 2 * cell + side, q,
+                                   J);
+        if (side == 0) {
+          dV = surface_measure<D>(J, args.face_jac + l * D * (D - 1)) * args.w[q];
+          G[0] = (CT)((args.coef ? args.coef[cell] : 1.0) * dV);
+        }
+        (void)inverse<D>(J, Ji);
+        const double *nr = args.face_normal + l * D;
+        double v[D], vv = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; j++) t = fma(Ji[j][i], nr[j], t);
+          v[i] = t;
+          vv = fma(t, t, vv);
+        }
+        const double inv = vv > 0.0 ? 0.5 / sqrt(vv) : 0.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          double m = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; j++) m = fma(Ji[i][j], v[j] * inv, m);
+          G[1 + side * D + i] = (CT)(dV * m);
+        }
+      }
+      continue;
+    } else if constexpr (NL > 0) {
       const int l = args.item_face[cell];
       double J[D][D];
       cell_jacobian_only<D, NV>(args.xyz, args.cell_nodes, args.gN + (size_t)l * NQ * NV * D, cell, q, J);
@@ -477,6 +530,26 @@ __global__ __launch_bounds__(BS) void matrix_tile_kernel(MatrixArgs args) {
         const CT *ph = phi_s + (NL > 0 ? lf_s[s] * NQ * NS : 0);
 #pragma unroll
         for (int q = 0; q < NQ; q++) v[0] = fma(ph[q * NS + an], Gs[q * NG] * ph[q * NS + bn], v[0]);
+      } else if constexpr (kSkel) {
+        const int sets = lf_s[s];
+        const int sa = an >= NS, sb = bn >= NS;  // sides of row a and column b
+        const int la = sa ? sets >> 16 : sets & 0xFFFF, lb = sb ? sets >> 16 : sets & 0xFFFF;
+        const int ia = an - sa * NS, ib = bn - sb * NS;
+        const CT *pha = phi_s + la * NQ * NS, *phb = phi_s + lb * NQ * NS;
+        const CT *gpa = gphi_s + la * NQ * D * NS, *gpb = gphi_s + lb * NQ * D * NS;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          const CT *G = Gs + q * NGS;
+          CT ta = 0, tb = 0;
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            ta = fma(G[1 + sa * D + d], gpa[(q * D + d) * NS + ia], ta);
+            tb = fma(G[1 + sb * D + d], gpb[(q * D + d) * NS + ib], tb);
+          }
+          const CT pa = pha[q * NS + ia], pb = phb[q * NS + ib];
+          v[0] += G[0] * pa * pb - pa * tb - ta * pb;
+        }
+        if (sa != sb) v[0] = -v[0];
       } else if constexpr (kNit) {
         // row a (test v = phi_a), column b (trial u = phi_b):
         //   beta dG phi_a phi_b - phi_a (m . gref_b) - (m . gref_a) phi_b
@@ -1647,7 +1720,7 @@ static KernelInfo matrix_info_bs() {
   KernelInfo k;
   k.fn = reinterpret_cast<const void *>(&matrix_tile_kernel<E::d, E::nv, E::ns, E::nq, E::nc, FORM, CT, E::nl, BS>);
   constexpr bool cell_geo = CellGeo<E::d, E::nv, FORM, E::nl>::value;
-  constexpr bool nit = FORM == kNitsche || FORM == kNitscheLoad;
+  constexpr bool nit = FORM == kNitsche || FORM == kNitscheLoad || FORM == kSkeleton;
   constexpr int ngphi = E::nl > 0 ? (nit ? NT * E::nq * E::d * E::ns : 0)
                                   : (cell_geo ? E::ns * E::ns * E::d * E::d : E::nq * E::d * E::ns);
   k.table_bytes = (size_t)(ngphi + NT * E::nq * E::ns) * sizeof(CT);
@@ -1684,11 +1757,21 @@ static bool match(const ElemKey &key) {
          key.nl == E::nl;
 }
 
+// skeleton items: two cells' nodes, a point set per (local face, vertex
+// order) pair of the two sides
+template <typename E>
+constexpr bool is_skeleton() {
+  return E::nl == (E::d == 2 ? 8 : 12);
+}
+
 template <typename E>
 static bool try_matrix(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                        KernelInfo &out) {
   if (!match<E>(key)) return false;
-  if constexpr (E::nl > 0) {
+  if constexpr (is_skeleton<E>()) {
+    if (form != kSkeleton) return false;
+    out = precision == 0 ? matrix_info<E, kSkeleton, double>(waves) : matrix_info<E, kSkeleton, float>(waves);
+  } else if constexpr (E::nl > 0) {
     // boundary items on the VALU kernel: the mass form, and on scalar fields
     // the Nitsche terms
     if (form == kMass) {
@@ -1743,6 +1826,9 @@ using QuadQ3 = Elem<2, 4, 16, 16, 1>;
 using HexQ1 = Elem<3, 8, 8, 8, 1>;
 using HexQ2 = Elem<3, 8, 27, 27, 1>;
 using TetP2v = Elem<3, 4, 10, 14, 3>;
+// Q2 quadrilaterals with integration degree 2 (2x2 Gauss points): the rule
+// GalerkinToolkitExamples/src/poisson.jl keeps by default
+using QuadQ2d2 = Elem<2, 4, 9, 4, 1>;
 // boundary items: edges of Q1..Q3 quadrilaterals (k+1 Gauss points), faces
 // of Q1/Q2 hexahedra ((k+1)^2), faces of P2 vector tetrahedra (9-point rule)
 using QuadEdge1 = Elem<2, 4, 2, 2, 1, 4>;
@@ -1751,6 +1837,7 @@ using QuadEdge3 = Elem<2, 4, 4, 4, 1, 4>;
 using HexFace1 = Elem<3, 8, 4, 4, 1, 6>;
 using HexFace2 = Elem<3, 8, 9, 9, 1, 6>;
 using TetFace2v = Elem<3, 4, 6, 9, 3, 4>;
+using QuadEdge2d2 = Elem<2, 4, 3, 2, 1, 4>;
 // Nitsche items: boundary faces with all DOFs of their cell, Gauss rules of
 // k+1 points per direction on the face
 using QuadNitsche1 = Elem<2, 4, 4, 2, 1, 4>;
@@ -1758,6 +1845,15 @@ using QuadNitsche2 = Elem<2, 4, 9, 3, 1, 4>;
 using QuadNitsche3 = Elem<2, 4, 16, 4, 1, 4>;
 using HexNitsche1 = Elem<3, 8, 8, 4, 1, 6>;
 using HexNitsche2 = Elem<3, 8, 27, 9, 1, 6>;
+// skeleton items: interior faces with both cells (their nodes and DOFs),
+// Gauss rules of k+1 points per direction on the face, and for Q2
+// quadrilaterals also 2 (integration degree 2)
+using QuadSkeleton1 = Elem<2, 8, 4, 2, 1, 8>;
+using QuadSkeleton2a = Elem<2, 8, 9, 2, 1, 8>;
+using QuadSkeleton2 = Elem<2, 8, 9, 3, 1, 8>;
+using QuadSkeleton3 = Elem<2, 8, 16, 4, 1, 8>;
+using HexSkeleton1 = Elem<3, 16, 8, 4, 1, 12>;
+using HexSkeleton2 = Elem<3, 16, 27, 9, 1, 12>;
 
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out) {
@@ -1767,6 +1863,8 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<HexQ1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexQ2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TetP2v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadQ2d2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadEdge2d2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<QuadEdge1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<QuadEdge2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<QuadEdge3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
@@ -1777,7 +1875,13 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<QuadNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<QuadNitsche3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexNitsche1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
-         try_matrix<HexNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out);
+         try_matrix<HexNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadSkeleton2a>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadSkeleton3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out);
 }
 
 int batched_type(const ElemKey &key) {
@@ -1816,7 +1920,8 @@ bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {
          try_vector<HexQ2>(key, fkind, out) || try_vector<TetP2v>(key, fkind, out) ||
          try_vector<QuadEdge1>(key, fkind, out) || try_vector<QuadEdge2>(key, fkind, out) ||
          try_vector<QuadEdge3>(key, fkind, out) || try_vector<HexFace1>(key, fkind, out) ||
-         try_vector<HexFace2>(key, fkind, out) || try_vector<TetFace2v>(key, fkind, out);
+         try_vector<HexFace2>(key, fkind, out) || try_vector<TetFace2v>(key, fkind, out) ||
+         try_vector<QuadQ2d2>(key, fkind, out) || try_vector<QuadEdge2d2>(key, fkind, out);
 }
 
 }  // namespace gtk

@@ -19,11 +19,12 @@ FREE, DIRICHLET = 1, 2
 class Domain:
     """A set of d-faces of a mesh (a GT.MeshDomain)."""
 
-    def __init__(self, mesh, dim, faces, name=""):
+    def __init__(self, mesh, dim, faces, name="", is_skeleton=False):
         self.mesh = mesh
         self.dim = dim
         self.faces = np.asarray(faces, np.int64)  # 0-based face ids
         self.name = name
+        self.is_skeleton = is_skeleton  # interior faces integrated from both sides
 
     def num_dims(self):
         return self.dim
@@ -34,6 +35,14 @@ class Domain:
 
 def interior(mesh):
     return Domain(mesh, mesh.D, np.arange(mesh.cells.shape[0]), "interior")
+
+
+def skeleton(mesh):
+    """GT.skeleton(mesh) (src/mesh.jl:236-247): the interior (D-1)-faces,
+    those with two incident cells (group_interior_faces!)."""
+    d = mesh.D - 1
+    cnt = np.bincount(mesh.cell_faces[d].ravel(), minlength=mesh.num_faces(d))
+    return Domain(mesh, d, np.nonzero(cnt == 2)[0], "skeleton", is_skeleton=True)
 
 
 def boundary(mesh, group_names=None):
@@ -48,9 +57,13 @@ def boundary(mesh, group_names=None):
 
 
 class LagrangeSpace:
-    """GT.lagrange_space(domain, order; tensor_size, dirichlet_boundary)."""
+    """GT.lagrange_space(domain, order; tensor_size, dirichlet_boundary,
+    continuous).  continuous=False: every DOF belongs to its cell
+    (reference_face_own_dofs, src/space.jl:860-880), so step 1 numbers them
+    cell by cell (cell c's local DOF a is c*nd + a + 1); the Dirichlet step is
+    the same."""
 
-    def __init__(self, domain, order, tensor_size=None, dirichlet_boundary=None):
+    def __init__(self, domain, order, tensor_size=None, dirichlet_boundary=None, continuous=True):
         mesh = domain.mesh
         if domain.dim != mesh.D or len(domain.faces) != mesh.cells.shape[0]:
             raise NotImplementedError("spaces on the whole interior of the mesh only")
@@ -60,6 +73,7 @@ class LagrangeSpace:
         self.ncomp = 1 if tensor_size is None else int(np.prod(tensor_size))
         self.fe = R.Lagrange(mesh.cellkind, self.order, self.ncomp)
         self.dirichlet_boundary = dirichlet_boundary
+        self.continuous = bool(continuous)
         self._number()
 
     def _number(self):
@@ -76,7 +90,10 @@ class LagrangeSpace:
             base += mesh.num_faces(d) * cnt
         ndofs = base
         cell_dofs = np.zeros((ncell, fe.ndofs), np.int64)
-        for d in range(D + 1):
+        if not self.continuous:
+            ndofs = ncell * fe.ndofs
+            cell_dofs[:] = np.arange(ndofs, dtype=np.int64).reshape(ncell, fe.ndofs) + 1
+        for d in range(D + 1 if self.continuous else 0):
             b0, cnt = offsets[d]
             own = fe.own_nodes[d]  # (nlf, n_own)
             P = fe.own_perms[d]  # (np, n_own)
@@ -133,5 +150,6 @@ class LagrangeSpace:
         return self.cell_dofs
 
 
-def lagrange_space(domain, order, tensor_size=None, dirichlet_boundary=None):
-    return LagrangeSpace(domain, order, tensor_size=tensor_size, dirichlet_boundary=dirichlet_boundary)
+def lagrange_space(domain, order, tensor_size=None, dirichlet_boundary=None, continuous=True):
+    return LagrangeSpace(domain, order, tensor_size=tensor_size, dirichlet_boundary=dirichlet_boundary,
+                         continuous=continuous)

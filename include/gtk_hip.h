@@ -59,6 +59,12 @@ typedef struct gtk_ctx gtk_ctx;
 #define GTK_NITSCHE_LOAD 4  /* int beta v u - n.grad(v) u: applied to the nodal
                                values of g (gtk_block_multiply) it is the load
                                int beta v g - n.grad(v) g of the same method */
+/* Interior-penalty terms on skeleton items (an interior face with both its
+ * cells; scalar fields; poisson.jl:99-102 over GT.skeleton, src/mesh.jl:
+ * 236-247, terms of src/compiler.jl:524-529), beta per item (gamma / h):
+ * int beta jump(v).jump(u) - jump(v).mean(grad u) - mean(grad v).jump(u),
+ * jump(u) = u1 n1 + u2 n2, mean(f) = (f1 + f2) / 2 (poisson.jl:40-41) */
+#define GTK_SKELETON 5
 
 /* arithmetic of the per-cell contraction */
 #define GTK_FP64 0   /* everything in FP64: the plain drop-in */
@@ -130,6 +136,17 @@ int gtk_set_reference(gtk_ctx *ctx, int nq, const double *w, int nshape, const d
  * from the items' DOFs.  Replaces the boundary-measure terms
  * (GT.measure(GT.boundary(mesh), degree), src/compiler.jl:513-529) of the
  * generated loops. */
+/* Skeleton items (GTK_SKELETON): gtk_set_mesh with both cells' nodes per
+ * item (side 0 then side 1, 2 x nodes_per_cell), gtk_set_space with both
+ * cells' DOFs (dofs_per_cell = 2 nshape); the nfaces tables are point sets
+ * (at most 16; the kernels take 8 for quadrilaterals, 12 for hexahedra):
+ * each the face rule's points in one cell's reference coordinates, the
+ * sets of an item's two sides listing the same physical points in the same
+ * order; grad_geo keeps its stride of nq * nv * dim values per set, the
+ * first half one cell's gradients [nq][nv / 2][dim], the rest zeros;
+ * face_jac is the set's local face's (side 0's sets give the measure), and
+ * item_face[i] = set of side 0 | set of side 1 << 16.
+ * Such a context assembles GTK_SKELETON only. */
 int gtk_set_reference_faces(gtk_ctx *ctx, int nfaces, int nq, const double *w, int nshape, const double *phi,
                             const double *grad_geo, const double *face_jac, const int32_t *item_face);
 

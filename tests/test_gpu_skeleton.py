@@ -1,0 +1,151 @@
+"""Interior-penalty skeleton terms on the GPU (GalerkinToolkitExamples/src/
+poisson.jl:99-102 over GT.skeleton) against the oracle (oracle/skeleton.py),
+through the C ABI:
+- discontinuous spaces: the DOF ids equal the oracle's (cell by cell, the
+  Dirichlet tags of the face closures);
+- the skeleton matrix alone on discontinuous spaces (quadrilaterals Q1-Q3,
+  hexahedra Q1/Q2, perturbed): the pattern of the oracle's COO assembly bit
+  for bit, FP64 values within 1e-13 and the mixed form within 1e-6
+  (relative Frobenius, and entrywise relative to the largest face-block
+  entry); continuous spaces are refused (every jump vanishes);
+- Laplace + skeleton + Nitsche as one matrix on a discontinuous space (the
+  volume and boundary blocks added into the skeleton block on the device);
+- the reference's patch tests (poisson_tests.jl:31-44): interior penalty
+  with strong and with Nitsche conditions on (0,2)^2, Neumann on y = 2,
+  Q2; the systems assembled on the device reproduce u = x + y to 1e-10
+  (solved on the host: the interior-penalty matrix with the Examples'
+  penalty is indefinite)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from nitsche import NitscheFaces
+from skeleton import DGSpace, Skeleton
+
+pytestmark = pytest.mark.gpu
+
+TOL64, TOLMX = 1e-13, 1e-6
+
+
+def setup(GT, O, cells, order, perturb, box=None, seed=11):
+    D = len(cells)
+    box = box or [0.0, 1.0] * D
+    m = GT.cartesian_mesh(tuple(box), cells)
+    if perturb:
+        m.coords = GT.perturb_interior_nodes(m, perturb, seed)
+    p = O.Problem(list(cells), box=list(box)).space(order, dirichlet=0)
+    p.set_coords(m.coords)
+    p.quadrature(2 * order)
+    return m, p
+
+
+@pytest.mark.parametrize("cells,order", [((3, 2), 2), ((2, 2, 2), 1)])
+def test_dg_numbering(GT, O, cells, order):
+    print(f"case dg numbering {cells} Q{order}", flush=True)
+    m, p = setup(GT, O, cells, order, 0.0)
+    V = GT.lagrange_space(GT.interior(m), order, continuous=False)
+    assert np.array_equal(V.cell_dofs, DGSpace(p, order).cd)
+    D = len(cells)
+    G = GT.boundary(m)
+    Vd = GT.lagrange_space(GT.interior(m), order, dirichlet_boundary=G, continuous=False)
+    on_bnd = lambda y: bool(np.any(np.abs(y) < 1e-12) or np.any(np.abs(y - 1.0) < 1e-12))
+    ref = DGSpace(p, order, dirichlet=on_bnd)
+    assert np.array_equal(Vd.cell_dofs, ref.cd) and (Vd.nfree, Vd.ndiri) == (ref.nfree, ref.ndiri)
+
+
+@pytest.mark.parametrize("cells,order,perturb", [
+    ((4, 3), 1, 0.2), ((3, 3), 2, 0.2), ((3, 2), 3, 0.15), ((2, 3, 2), 1, 0.15), ((2, 2, 2), 2, 0.1)])
+def test_skeleton_matrix(GT, O, cells, order, perturb):
+    print(f"case skeleton matrix {cells} Q{order}", flush=True)
+    m, p = setup(GT, O, cells, order, perturb)
+    V = GT.lagrange_space(GT.interior(m), order, continuous=False)
+    cd = DGSpace(p, order).cd
+    assert np.array_equal(V.cell_dofs, cd)
+    degree = 2 * order
+    gamma = 2.5
+    dL = GT.measure(GT.skeleton(m), degree)
+    sk = Skeleton(p, order, cd)
+    n = V.nfree
+    ref = sk.matrix(degree, gamma, shape=(n, n))
+    scale = max(np.abs(sk.face_blocks(f, degree, gamma)[1]).max() for f in sk.geo.interior)
+    for prec, tol in ((GT.FP64, TOL64), (GT.MIXED, TOLMX)):
+        A = GT.assemble_matrix(GT.InteriorPenalty(dL, gamma), float, V, V, precision=prec)
+        assert np.array_equal(A.colptr - 1, ref.indptr) and np.array_equal(A.rowval - 1, ref.indices)
+        err = np.abs(A.nzval - ref.data).max() / scale
+        assert err <= tol, (prec, err)
+        assert np.linalg.norm(A.nzval - ref.data) <= tol * np.linalg.norm(ref.data), prec
+
+
+def test_continuous_space_refused(GT, O):
+    m, _ = setup(GT, O, (3, 3), 1, 0.0)
+    V = GT.lagrange_space(GT.interior(m), 1)
+    dL = GT.measure(GT.skeleton(m), 2)
+    with pytest.raises(NotImplementedError):
+        GT.assemble_matrix(GT.InteriorPenalty(dL, 1.0), float, V, V)
+
+
+@pytest.mark.parametrize("cells,order", [((3, 3), 2), ((2, 2, 2), 1)])
+def test_dg_laplace_skeleton_nitsche(GT, O, cells, order):
+    print(f"case dg laplace + skeleton + nitsche {cells} Q{order}", flush=True)
+    m, p = setup(GT, O, cells, order, 0.15)
+    V = GT.lagrange_space(GT.interior(m), order, continuous=False)
+    dg = DGSpace(p, order)
+    degree = 2 * order
+    gamma = 3.0
+    dO = GT.measure(GT.interior(m), degree)
+    dL = GT.measure(GT.skeleton(m), degree)
+    dG = GT.measure(GT.boundary(m), degree)
+    n = V.nfree
+    nf = NitscheFaces(p, order)
+    nf.cd = dg.cd
+    ref = (dg.laplace(degree) + Skeleton(p, order, dg.cd).matrix(degree, gamma, shape=(n, n))
+           + nf.matrix(degree, gamma, shape=(n, n))).tocsc()
+    ref.sort_indices()
+    for prec, tol in ((GT.FP64, TOL64), (GT.MIXED, TOLMX)):
+        A = GT.assemble_matrix(GT.Laplace(dO) + GT.InteriorPenalty(dL, gamma) + GT.Nitsche(dG, gamma), float, V, V,
+                               precision=prec)
+        assert np.array_equal(A.colptr - 1, ref.indptr) and np.array_equal(A.rowval - 1, ref.indices)
+        assert np.linalg.norm(A.nzval - ref.data) <= tol * np.linalg.norm(ref.data), prec
+
+
+@pytest.mark.parametrize("method,degree", [("strong", 2), ("nitsche", 4)])
+def test_dg_patch(GT, O, method, degree):
+    """poisson_tests.jl:31-44 with discretization_method = :interior_penalty:
+    (0,2)^2 in 2x2 cells, Q2, u = x + y, Dirichlet on 1-face-1, -3, -4,
+    Neumann on 1-face-2 (y = 2, g = 1); gamma = degree (degree + 1) / 10.
+    degree 2 is the Examples' own; with Nitsche conditions that rule leaves
+    the system singular (a quadratic jump vanishing at both Gauss points of
+    every face), so that case uses 4."""
+    print(f"case dg patch {method} degree {degree}", flush=True)
+    m, p = setup(GT, O, (2, 2), 2, 0.0, box=[0.0, 2.0, 0.0, 2.0])
+    Om = GT.interior(m)
+    Gd = GT.boundary(m, ["1-face-1", "1-face-3", "1-face-4"])
+    Gn = GT.boundary(m, ["1-face-2"])
+    gamma = degree * (degree + 1) / 10.0
+    dO, dL = GT.measure(Om, degree), GT.measure(GT.skeleton(m), degree)
+    dGd, dGn = GT.measure(Gd, degree), GT.measure(Gn, degree)
+    u = lambda y: float(y[0] + y[1])
+    if method == "strong":
+        V = GT.lagrange_space(Om, 2, dirichlet_boundary=Gd, continuous=False)
+    else:
+        V = GT.lagrange_space(Om, 2, continuous=False)
+    dg = DGSpace(p, 2)
+    pos = dg.geo.cell_pos
+    vals = np.zeros(V.nfree + V.ndiri)
+    for c in range(V.cell_dofs.shape[0]):
+        for a in range(V.cell_dofs.shape[1]):
+            d = V.cell_dofs[c, a]
+            vals[d - 1 if d > 0 else V.nfree - d - 1] = u(pos[c, a])
+    n = V.nfree
+    a_form = GT.Laplace(dO) + GT.InteriorPenalty(dL, gamma)
+    l_form = GT.Load(dGn, 1.0)
+    if method == "strong":
+        uhd = GT.DiscreteField(V, np.zeros(n), vals[n:])
+        A, b = GT.linear_problem(uhd, a_form, l_form)
+    else:
+        g = GT.DiscreteField(V, vals[:n])
+        A = GT.assemble_matrix(a_form + GT.Nitsche(dGd, gamma), float, V, V)
+        b = GT.assemble_vector(l_form + GT.NitscheLoad(dGd, g, gamma), float, V)
+    x = sla.spsolve(A.to_scipy().tocsc(), b)
+    assert np.abs(x - vals[:n]).max() < 1e-10 * np.abs(vals).max()
