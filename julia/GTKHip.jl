@@ -33,6 +33,7 @@ const lib = get(ENV, "GTK_HIP_LIB", joinpath(@__DIR__, "..", "galerkintoolkit.jl
 
 const FREE, DIRICHLET = Cint(1), Cint(2)
 const LAPLACE, MASS, ELASTICITY = Cint(0), Cint(1), Cint(2)
+const NITSCHE, NITSCHE_LOAD, SKELETON = Cint(3), Cint(4), Cint(5)   # face-mode forms (include/gtk_hip.h)
 const FP64, MIXED = Cint(0), Cint(1)
 const F_CONST, F_CELL, F_NODAL = Cint(0), Cint(1), Cint(2)
 
