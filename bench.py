@@ -363,6 +363,66 @@ def elasticity_result(args, n, cells, V, A, launch, wall64, kern64, wallmx, kern
     }
 
 
+def bench_dg(GT, args, n):
+    """Interior-penalty DG (GalerkinToolkitExamples/src/poisson.jl:65-107):
+    Laplace + interior-penalty skeleton terms + Nitsche terms on the whole
+    boundary, a discontinuous Q2 space on n^2 perturbed quadrilaterals,
+    degree 4, gamma = degree (degree + 1) / 10; each part's kernel timed in
+    both arithmetic forms, and the summed matrix's wall time."""
+    from gtk_amd import _lib as L
+    from gtk_amd import assembly as AS
+    t_setup = time.perf_counter()
+    mesh = GT.cartesian_mesh((0.0, 1.0) * 2, (n, n))
+    mesh.coords = GT.perturb_interior_nodes(mesh, 0.2, args.seed)
+    Om = GT.interior(mesh)
+    V = GT.lagrange_space(Om, 2, continuous=False)
+    degree = 4
+    gamma = degree * (degree + 1) / 10.0
+    dO, dL, dG = GT.measure(Om, degree), GT.measure(GT.skeleton(mesh), degree), GT.measure(GT.boundary(mesh), degree)
+    form = GT.Laplace(dO) + GT.InteriorPenalty(dL, gamma) + GT.Nitsche(dG, gamma)
+    A64 = GT.assemble_matrix(form, float, V, V, precision=GT.FP64)
+    Amx = GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+    rel_err = float(np.linalg.norm(Amx.nzval - A64.nzval) / np.linalg.norm(A64.nzval))
+    setup_s = time.perf_counter() - t_setup
+    cells = mesh.cells.shape[0]
+    parts = {}
+    for name, f, code, kind in (("volume", form.terms[0], L.GTK_LAPLACE, None),
+                                ("skeleton", form.terms[1], L.GTK_SKELETON, None),
+                                ("nitsche", form.terms[2], L.GTK_NITSCHE, "nitsche")):
+        a = AS._assembler(V, f.dOmega, 0, kind)
+        _, k64 = time_form(a.ctx, L, code, L.GTK_FP64, args.steps, args.warmup)
+        _, kmx = time_form(a.ctx, L, code, L.GTK_MIXED, args.steps, args.warmup)
+        nnz = a.pattern(L.GTK_FREE, L.GTK_FREE)[0]
+        items = getattr(a, "items", cells)
+        per = 2 if name == "skeleton" else 1
+        nd = V.fe.nnodes * per if name != "volume" else V.fe.nnodes
+        nv = 4 * per
+        b = alg_bytes(2, mesh.coords.shape[0], items, nv, nd, 0 if name == "volume" else 1, nnz)
+        parts[name] = {"items": int(items), "nnz": int(nnz), "fp64_kernel_ms": k64, "mixed_kernel_ms": kmx,
+                       "bytes_alg_per_launch": b, "roofline_frac_mixed": b / (kmx * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        GT.assemble_matrix(form, float, V, V, precision=GT.MIXED)
+        walls.append((time.perf_counter() - t0) * 1e3)
+    wall = float(np.median(walls))
+    return {
+        "metric": "cells/s, interior-penalty DG Laplace + skeleton + Nitsche assembly (Q2 quads)",
+        "value": cells / (wall * 1e-3),
+        "unit": "cells/s",
+        "ms_per_step": wall,
+        "dtype": "fp32 contraction, fp64 geometry + accumulation (mixed); fp64",
+        "config": {"workload": f"2D interior-penalty DG, {n}^2 Q2 quadrilaterals (discontinuous), interior "
+                               f"vertices perturbed U(-0.2h,0.2h) (seed {args.seed}), degree {degree}, gamma {gamma}",
+                   "name": "dgip", "cells": cells, "dofs": int(V.nfree), "nnz": int(A64.nnz)},
+        "parts": parts,
+        "mixed": {"rel_fro_err_vs_fp64": rel_err},
+        "note": "value: wall time of the mixed summed assembly (three kernels, two block additions on the device, "
+                "the values copied to host memory)",
+        "setup_s": setup_s,
+    }
+
+
 def batch_problems(GT, count, seed, nmin=16, nmax=256):
     """`count` unit-square Poisson problems: n per side uniform in
     [nmin, nmax], Lagrange order uniform in {1, 2, 3}, Gauss with order+1
@@ -497,11 +557,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["all", "q2hex", "q1hex", "p2tet", "batch"], default="all",
-                    help="all (default): the Q2 hex headline with the other three as sub-results of the same line; "
+    ap.add_argument("--workload", choices=["all", "q2hex", "q1hex", "p2tet", "batch", "dgip"], default="all",
+                    help="all (default): the Q2 hex headline with the others as sub-results of the same line; "
                          "q2hex: Q2 hex 64^3 perturbed Laplace (the headline) alone; q1hex: Q1 hex 128^3 with "
                          "lognormal kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve; batch: 256 "
-                         "uneven 2D Poisson problems in one graph launch")
+                         "uneven 2D Poisson problems in one graph launch; dgip: interior-penalty DG on 512^2 "
+                         "discontinuous Q2 quadrilaterals (volume, skeleton and Nitsche kernels)")
     ap.add_argument("--problems", type=int, default=256, help="problems of the batch workload")
     ap.add_argument("--batch-cpu-stride", type=int, default=2, help="CPU baseline: every k-th problem of the batch")
     ap.add_argument("--tune-cache", default="", help="autotuner cache file (default: the user cache)")
@@ -543,10 +604,14 @@ def main():
             out["p2tet"] = sub_result(bench_elasticity(GT, args, 48))
             step("batch")
             out["batch"] = sub_result(bench_batch(GT, args, args.problems))
+            step("dgip")
+            out["dgip"] = sub_result(bench_dg(GT, args, 512))
     elif wl == "q1hex":
         out = bench_laplace(GT, args, args.cells or 128, 1, 2, 0.0, True, "q1hex")
     elif wl == "p2tet":
         out = bench_elasticity(GT, args, args.cells or 48)
+    elif wl == "dgip":
+        out = bench_dg(GT, args, args.cells or 512)
     else:
         out = bench_batch(GT, args, args.problems)
     print(json.dumps(out))

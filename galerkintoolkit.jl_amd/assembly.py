@@ -365,14 +365,16 @@ class _SkeletonAssembler(_Assembler):
         if not np.all(np.take_along_axis(V1, perm, axis=1) == V0):
             raise ValueError("the two sides of a face do not share its vertices")
         nfv = fv.shape[1]
-        ident = tuple(range(nfv))
-        keys, item_sets = {}, np.zeros((len(faces), 2), np.int64)
-        for i in range(len(faces)):
-            for side, pk in ((0, ident), (1, tuple(int(v) for v in perm[i]))):
-                item_sets[i, side] = keys.setdefault((int(lfaces[i, side]), pk), len(keys))
+        nit = len(faces)
+        # point sets: distinct (local face, vertex order) keys of the sides
+        side_keys = np.concatenate([
+            np.concatenate([lfaces[:, :1], np.broadcast_to(np.arange(nfv), (nit, nfv))], axis=1),
+            np.concatenate([lfaces[:, 1:], perm], axis=1)])
+        uniq, inv = np.unique(side_keys, axis=0, return_inverse=True)
+        item_sets = inv.reshape(2, nit).T
         nsets = 8 if D == 2 else 12
-        if len(keys) > nsets:
-            raise NotImplementedError(f"skeleton terms: {len(keys)} point sets (at most {nsets})")
+        if len(uniq) > nsets:
+            raise NotImplementedError(f"skeleton terms: {len(uniq)} point sets (at most {nsets})")
         ft = g.ftype[D - 1]
         N = R.linear_shape(ft, dL.x)
         _, dNf = R.Lagrange(ft, 1).tabulate(dL.x[:1])
@@ -381,8 +383,9 @@ class _SkeletonAssembler(_Assembler):
         normals = R.face_normals(mesh.cellkind)
         nq, nv = len(dL.w), len(g.verts)
         phi, gphi, gN, A, nrm = [], [], [], [], []
-        for (l, pk) in sorted(keys, key=keys.get):
-            X = np.asarray([g.verts[v] for v in fv[l]], np.float64)[list(pk)]
+        for key in uniq:
+            l, pk = int(key[0]), key[1:]
+            X = np.asarray([g.verts[v] for v in fv[l]], np.float64)[pk]
             xc = N @ X
             val, grad = fe.tabulate(xc)
             phi.append(val)
@@ -397,15 +400,15 @@ class _SkeletonAssembler(_Assembler):
         Xf = mesh.coords[V0]
         diff = Xf[:, :, None, :] - Xf[:, None, :, :]
         self.h = np.sqrt((diff ** 2).sum(axis=3)).max(axis=(1, 2))
-        item_nodes = mesh.cell_nodes_1b()[cells].reshape(len(faces), -1)
-        item_dofs = space.cell_dofs[cells].reshape(len(faces), -1)
+        item_nodes = mesh.cell_nodes_1b()[cells].reshape(nit, -1)
+        item_dofs = space.cell_dofs[cells].reshape(nit, -1)
         item_face = (item_sets[:, 0] | (item_sets[:, 1] << 16)).astype(np.int32)
         self.ctx = L.Context(device)
         self.ctx.set_mesh(D, mesh.coords, item_nodes)
         self.ctx.set_space(1, item_dofs, space.nfree, space.ndiri)
         self.ctx.set_reference_faces(dL.w, np.stack(phi), np.stack(gN), np.stack(A), item_face)
         self.ctx.set_reference_face_gradients(np.stack(gphi), np.stack(nrm))
-        self.items = len(faces)
+        self.items = nit
         self.patterns = {}
         self.values_owner = {}
 

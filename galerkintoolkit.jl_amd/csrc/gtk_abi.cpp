@@ -843,9 +843,28 @@ static int build_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, i
     }
     const size_t acc_bytes = acc_of(ep->k);
     int max_cells = tile_cells_for(ep->k, acc_bytes, kLdsBudget);
-    if (max_cells_cap > 0) max_cells = std::max(8, std::min(max_cells, max_cells_cap));
-    e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
-    if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    bool built = false;
+    if (max_cells_cap > 0) {
+      max_cells = std::max(8, std::min(max_cells, max_cells_cap));
+    } else {
+      // one workgroup per tile: a small problem is first cut into at least
+      // two tiles per CU (tiles of at least 8 cells) so that it is not run
+      // by a few workgroups; where a DOF touches more cells than that, the
+      // LDS-bound tiles (512^2 Q2 quadrilaterals, mixed: the 2,048 Nitsche
+      // items 0.233 -> 0.007 ms, the 523,264 skeleton items 1.04 -> 0.73 ms)
+      int cus = 0;
+      HIPCHK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+      const int64_t spread = (ctx->n_cells + 2 * (int64_t)std::max(cus, 1) - 1) / (2 * (int64_t)std::max(cus, 1));
+      const int small = (int)std::max<int64_t>(8, std::min<int64_t>(max_cells, spread));
+      if (small < max_cells) {
+        e = gtk::build_tiles(m, cols, small, 1 << 20, 1 << 20, tile_chunk(ctx, small), ep->tiles, nt);
+        built = e.empty();
+      }
+    }
+    if (!built) {
+      e = gtk::build_tiles(m, cols, max_cells, 1 << 20, 1 << 20, tile_chunk(ctx, max_cells), ep->tiles, nt);
+      if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
+    }
     e = gtk::build_offsets(m, plan.pat, ep->tiles, ep->offs, nt);
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     ep->acc_offset = (int)align16(ep->k.table_bytes + (size_t)ep->tiles.max_tile_cells * ep->k.geo_bytes_per_cell);
