@@ -10,6 +10,8 @@ through the C ABI:
   entry); continuous spaces are refused (every jump vanishes);
 - Laplace + skeleton + Nitsche as one matrix on a discontinuous space (the
   volume and boundary blocks added into the skeleton block on the device);
+- the volume load on a discontinuous space, and the device solve of a
+  positive definite interior-penalty system against a direct solve;
 - the reference's patch tests (poisson_tests.jl:31-44): interior penalty
   with strong and with Nitsche conditions on (0,2)^2, Neumann on y = 2,
   Q2; the systems assembled on the device reproduce u = x + y to 1e-10
@@ -149,3 +151,35 @@ def test_dg_patch(GT, O, method, degree):
         b = GT.assemble_vector(l_form + GT.NitscheLoad(dGd, g, gamma), float, V)
     x = sla.spsolve(A.to_scipy().tocsc(), b)
     assert np.abs(x - vals[:n]).max() < 1e-10 * np.abs(vals).max()
+
+
+@pytest.mark.parametrize("cells,order", [((4, 3), 2), ((2, 3, 2), 1)])
+def test_dg_load_and_spd_solve(GT, O, cells, order):
+    """The volume load on a discontinuous space against the oracle, and the
+    device solve (Jacobi-PCG inside FP64 refinement) of a symmetric
+    interior-penalty system with a large penalty (positive definite)
+    against a direct solve of the same matrix."""
+    print(f"case dg load + solve {cells} Q{order}", flush=True)
+    m, p = setup(GT, O, cells, order, 0.15)
+    V = GT.lagrange_space(GT.interior(m), order, continuous=False)
+    dg = DGSpace(p, order)
+    degree = 2 * order
+    dO = GT.measure(GT.interior(m), degree)
+    f = lambda y: 1.0 + y[0] - 2.0 * y[1]
+    # a nodal f of the space: exact for this linear f
+    pos = dg.geo.cell_pos
+    fv = np.zeros(V.nfree)
+    for c in range(V.cell_dofs.shape[0]):
+        for a in range(V.cell_dofs.shape[1]):
+            fv[V.cell_dofs[c, a] - 1] = f(pos[c, a])
+    b = GT.assemble_vector(GT.Load(dO, GT.DiscreteField(V, fv)), float, V)
+    bref = dg.load(degree, f)
+    assert np.linalg.norm(b - bref) <= TOL64 * np.linalg.norm(bref)
+    gamma = 20.0 * order * (order + 1)
+    dL = GT.measure(GT.skeleton(m), degree)
+    dG = GT.measure(GT.boundary(m), degree)
+    A = GT.assemble_matrix(GT.Laplace(dO) + GT.InteriorPenalty(dL, gamma) + GT.Nitsche(dG, gamma), float, V, V)
+    xd = sla.spsolve(A.to_scipy().tocsc(), b)
+    for prec in (GT.FP64, GT.MIXED):
+        x, st = GT.solve(A, b, precision=prec, rtol=1e-11)
+        assert st.converged and np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd), prec
