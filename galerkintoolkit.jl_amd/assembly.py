@@ -179,8 +179,21 @@ class SparseMatrixCSC:
         return sp.csc_matrix((self.nzval, self.rowval - 1, self.colptr - 1), shape=(self.m, self.n))
 
 
+def _diameters(xyz, fnodes):
+    """Largest distance between the nodes of each face (fnodes: items x
+    face nodes, 0-based): GT.face_diameter_field (src/field.jl:488-492,
+    src/accessors.jl:907-921)."""
+    X = xyz[fnodes]
+    diff = X[:, :, None, :] - X[:, None, :, :]
+    return np.sqrt((diff ** 2).sum(axis=3)).max(axis=(1, 2))
+
+
 class _Assembler:
     """The device context of one (space, measure) pair."""
+
+    def set_coordinates(self, xyz):
+        """New node coordinates (same topology)."""
+        self.ctx.update_coordinates(xyz)
 
     def __init__(self, space, dOmega, device=0):
         mesh = space.mesh
@@ -301,9 +314,8 @@ class _NitscheAssembler(_Assembler):
             A.append(X.T @ dNf[0])
         # face diameters: largest distance between the face's nodes
         fn = np.asarray(g.faces[D - 1])[lface] - 1  # (items, face vertices) local vertices
-        Xf = mesh.coords[np.take_along_axis(mesh.cells[cells], fn, axis=1)]
-        diff = Xf[:, :, None, :] - Xf[:, None, :, :]
-        self.h = np.sqrt((diff ** 2).sum(axis=3)).max(axis=(1, 2))
+        self.fnodes = np.take_along_axis(mesh.cells[cells], fn, axis=1)
+        self.h = _diameters(mesh.coords, self.fnodes)
         self.ctx = L.Context(device)
         self.ctx.set_mesh(D, mesh.coords, mesh.cell_nodes_1b()[cells])
         self.ctx.set_space(1, space.cell_dofs[cells], space.nfree, space.ndiri)
@@ -316,6 +328,11 @@ class _NitscheAssembler(_Assembler):
     def beta(self, gamma):
         """gamma / h per item."""
         return np.ascontiguousarray(float(gamma) / self.h)
+
+    def set_coordinates(self, xyz):
+        """New node coordinates: the context's, and the face diameters."""
+        self.ctx.update_coordinates(xyz)
+        self.h = _diameters(xyz, self.fnodes)
 
 
 class _SkeletonAssembler(_Assembler):
@@ -397,9 +414,8 @@ class _SkeletonAssembler(_Assembler):
             nrm.append(normals[l])
         while len(phi) < nsets:  # unused sets: copies of the first
             phi.append(phi[0]), gphi.append(gphi[0]), gN.append(gN[0]), A.append(A[0]), nrm.append(nrm[0])
-        Xf = mesh.coords[V0]
-        diff = Xf[:, :, None, :] - Xf[:, None, :, :]
-        self.h = np.sqrt((diff ** 2).sum(axis=3)).max(axis=(1, 2))
+        self.fnodes = V0
+        self.h = _diameters(mesh.coords, self.fnodes)
         item_nodes = mesh.cell_nodes_1b()[cells].reshape(nit, -1)
         item_dofs = space.cell_dofs[cells].reshape(nit, -1)
         item_face = (item_sets[:, 0] | (item_sets[:, 1] << 16)).astype(np.int32)
@@ -415,6 +431,11 @@ class _SkeletonAssembler(_Assembler):
     def beta(self, gamma):
         """gamma / h per item."""
         return np.ascontiguousarray(float(gamma) / self.h)
+
+    def set_coordinates(self, xyz):
+        """New node coordinates: the context's, and the face diameters."""
+        self.ctx.update_coordinates(xyz)
+        self.h = _diameters(xyz, self.fnodes)
 
 
 def _assembler(space, dOmega, device, kind=None):
@@ -578,8 +599,13 @@ def _set_coordinates(form, space, device, coords):
     xyz = np.ascontiguousarray(coords, np.float64)
     if xyz.shape != space.mesh.coords.shape:
         raise ValueError("coordinates must keep the mesh's shape")
-    for t in terms:
-        _assembler(space, t.dOmega, device).ctx.update_coordinates(xyz)
+    # every context of the space on this device: the terms' and those of the
+    # Nitsche matrices and loads over the same measures
+    cache = space.__dict__.get("_assemblers", {})
+    measures = {id(t.dOmega) for t in terms}
+    for (mid, dev, _), a in cache.items():
+        if dev == device and mid in measures:
+            a.set_coordinates(xyz)
 
 
 def update_matrix(A, cache, parameters=None, coords=None, form=None):

@@ -10,6 +10,8 @@ through the C ABI:
   entry); continuous spaces are refused (every jump vanishes);
 - Laplace + skeleton + Nitsche as one matrix on a discontinuous space (the
   volume and boundary blocks added into the skeleton block on the device);
+- update_matrix with moved nodes (the face diameters follow) equals a fresh
+  assembly;
 - the volume load on a discontinuous space, and the device solve of a
   positive definite interior-penalty system against a direct solve;
 - the reference's patch tests (poisson_tests.jl:31-44): interior penalty
@@ -183,3 +185,26 @@ def test_dg_load_and_spd_solve(GT, O, cells, order):
     for prec in (GT.FP64, GT.MIXED):
         x, st = GT.solve(A, b, precision=prec, rtol=1e-11)
         assert st.converged and np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd), prec
+
+
+def test_dg_update_coordinates(GT, O):
+    """update_matrix with new node coordinates refills Laplace + skeleton +
+    Nitsche with the moved geometry (the face diameters too): equal to a
+    fresh assembly on a mesh with those coordinates."""
+    print("case dg update coordinates", flush=True)
+    cells, order = (4, 3), 2
+    m, _ = setup(GT, O, cells, order, 0.1, seed=3)
+    m2, _ = setup(GT, O, cells, order, 0.2, seed=5)
+    forms = []
+    for mesh in (m, m2):
+        V = GT.lagrange_space(GT.interior(mesh), order, continuous=False)
+        dO = GT.measure(GT.interior(mesh), 4)
+        dL = GT.measure(GT.skeleton(mesh), 4)
+        dG = GT.measure(GT.boundary(mesh), 4)
+        forms.append((V, GT.Laplace(dO) + GT.InteriorPenalty(dL, 3.0) + GT.Nitsche(dG, 3.0)))
+    (V1, f1), (V2, f2) = forms
+    A, cache = GT.assemble_matrix(f1, float, V1, V1, reuse=True)
+    GT.update_matrix(A, cache, coords=m2.coords)
+    B = GT.assemble_matrix(f2, float, V2, V2)
+    assert np.array_equal(A.colptr, B.colptr) and np.array_equal(A.rowval, B.rowval)
+    assert np.linalg.norm(A.nzval - B.nzval) <= 1e-14 * np.linalg.norm(B.nzval)
