@@ -18,10 +18,11 @@ from . import refcell as R
 from .assembly import (FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _assembler, _coefficients,
                        _form_code, _load_data, _ncoef_items)
 
-# element types of the batched kernel: scalar quadrilaterals of order 1, 2, 3
-# with (order+1)^2 Gauss points
-ELEM_TYPES = {1: 0, 2: 1, 3: 2}
-ELEM_NAMES = {0: "quad-q1", 1: "quad-q2", 2: "quad-q3"}
+# element types of the batched kernels: scalar quadrilaterals of order 1, 2,
+# 3 with (order+1)^2 Gauss points, scalar hexahedra of order 1, 2 with
+# (order+1)^3 (a batch holds one cell kind)
+ELEM_TYPES = {(R.QUAD, 1): 0, (R.QUAD, 2): 1, (R.QUAD, 3): 2, (R.HEX, 1): 3, (R.HEX, 2): 4}
+ELEM_NAMES = {0: "quad-q1", 1: "quad-q2", 2: "quad-q3", 3: "hex-q1", 4: "hex-q2"}
 
 # tile sizes the autotuner tries (cells per tile; 0 = as many as fit in LDS)
 TILE_CANDIDATES = (0, 256, 160, 96, 64, 40)
@@ -30,12 +31,14 @@ WAVE_CANDIDATES = (8, 12)
 
 
 def elem_type(space, dOmega=None):
-    """Element type of the batched kernel for a space, or raise."""
-    if space.mesh.cellkind != R.QUAD or space.ncomp != 1 or space.order not in ELEM_TYPES:
-        raise NotImplementedError("batched problems are scalar Q1/Q2/Q3 Lagrange spaces on quadrilaterals")
-    if dOmega is not None and len(dOmega.w) != (space.order + 1) ** 2:
-        raise NotImplementedError("batched problems use (order+1)^2 Gauss points: GT.measure(Omega, 2*order)")
-    return ELEM_TYPES[space.order]
+    """Element type of the batched kernels for a space, or raise."""
+    key = (space.mesh.cellkind, space.order)
+    if space.ncomp != 1 or key not in ELEM_TYPES:
+        raise NotImplementedError("batched problems are scalar Q1/Q2/Q3 Lagrange spaces on quadrilaterals or "
+                                  "Q1/Q2 on hexahedra")
+    if dOmega is not None and len(dOmega.w) != (space.order + 1) ** space.mesh.D:
+        raise NotImplementedError("batched problems use (order+1)^D Gauss points: GT.measure(Omega, 2*order)")
+    return ELEM_TYPES[key]
 
 
 def default_cache_path():
@@ -215,6 +218,9 @@ class MatrixBatch:
         self.device = device
         self.forms = list(forms)
         self.spaces = list(spaces)
+        if len({V.mesh.cellkind for V in spaces}) != 1:
+            raise NotImplementedError("a batch holds quadrilaterals or hexahedra, not both")
+        self.cellkind = spaces[0].mesh.cellkind
         self.problems = []
         for form, V in zip(forms, spaces):
             et = elem_type(V, form.dOmega)
@@ -244,7 +250,8 @@ class MatrixBatch:
 
     def _tune(self, tuning, cache):
         """tuning: "auto", None (8 waves, tiles as large as LDS allows) or a
-        dict {order: max_cells, "waves": w}."""
+        dict {order: max_cells, "waves": w} (orders of the batch's cell
+        kind)."""
         self.tuning_report = None
         if tuning == "auto":
             self.waves, self.tiles, self.tuning_report = autotune(self.problems, self.form_code, self.precision,
@@ -254,7 +261,8 @@ class MatrixBatch:
         else:
             t = dict(tuning)
             self.waves = int(t.pop("waves", 8))
-            self.tiles = {ELEM_TYPES[o]: int(c) for o, c in t.items()}
+            self.tiles = {ELEM_TYPES[(self.cellkind, o)]: int(c) for o, c in t.items()
+                          if (self.cellkind, o) in ELEM_TYPES}
 
     def with_precision(self, precision, tuning=None):
         """A batch over the same problems and device contexts (patterns,

@@ -1615,7 +1615,8 @@ struct gtk_batch {
   std::vector<gtk_ctx *> ctxs;
   std::vector<std::unique_ptr<ExecPlan>> plans;  // one per problem
   std::vector<uint64_t> gens;                    // each context's generation when the plans were built
-  int tile_cells[3] = {0, 0, 0};                 // cap on the cells of a tile per element type (0: LDS bound)
+  int tile_cells[gtk::kBatchedTypes] = {};     // cap on the cells of a tile per element type (0: LDS bound)
+  int family = -1;                               // of the problems' element types (type / 3)
   int waves = 8;                                 // waves per workgroup
   std::vector<gtk::MatrixArgs> args;  // host copy of d_problems
   struct Load {
@@ -1716,10 +1717,15 @@ int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx) {
   if (ctx->device != b->device) BFAIL(b, GTK_EINVAL, "the problem's context is on another device");
   if (!find_plan(ctx, GTK_FREE, GTK_FREE)) BFAIL(b, GTK_ESTATE, "gtk_build_pattern(GTK_FREE, GTK_FREE) first");
   if (!ctx->has_ref) BFAIL(b, GTK_ESTATE, "gtk_set_reference first");
-  if (ctx->ncomp != 1 || gtk::batched_type(elem_key(ctx)) < 0)
-    BFAIL(b, GTK_EUNSUP, "batched problems are scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2 points");
+  const int type = ctx->ncomp == 1 ? gtk::batched_type(elem_key(ctx)) : -1;
+  if (type < 0)
+    BFAIL(b, GTK_EUNSUP, "batched problems are scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2 points or Q1/Q2 "
+                         "hexahedra with (k+1)^3");
+  if (b->family >= 0 && type / 3 != b->family)
+    BFAIL(b, GTK_EUNSUP, "a batch holds quadrilaterals or hexahedra, not both");
   if (ctx->ncoef > 1) BFAIL(b, GTK_EINVAL, "scalar forms take one coefficient per cell");
   batch_release_graph(b);
+  b->family = type / 3;
   b->ctxs.push_back(ctx);
   b->loads.push_back(std::make_unique<gtk_batch::Load>());
   return GTK_OK;
@@ -1760,7 +1766,8 @@ int gtk_batch_get_vector(gtk_batch *b, int problem, double *out) {
 
 int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells) {
   if (!b) return GTK_EINVAL;
-  if (elem_type < 0 || elem_type > 2 || max_cells < 0) BFAIL(b, GTK_EINVAL, "elem_type 0..2, max_cells >= 0");
+  if (elem_type < 0 || elem_type >= gtk::kBatchedTypes || max_cells < 0)
+    BFAIL(b, GTK_EINVAL, "elem_type 0..4, max_cells >= 0");
   b->tile_cells[elem_type] = max_cells;
   batch_release_graph(b);
   return GTK_OK;
@@ -1779,7 +1786,7 @@ int gtk_batch_build(gtk_batch *b) {
   if (b->ctxs.empty()) BFAIL(b, GTK_ESTATE, "gtk_batch_add problems first");
   BHIPCHK(b, hipSetDevice(b->device));
   batch_release_graph(b);
-  if (!gtk::find_batched_kernel(b->form, b->precision, b->waves, b->k))
+  if (!gtk::find_batched_kernel(b->form, b->precision, b->waves, b->family, b->k))
     BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
   b->plans.clear();
   b->gens.clear();
@@ -1803,7 +1810,7 @@ int gtk_batch_build(gtk_batch *b) {
       BFAIL(b, GTK_EUNSUP, "a problem does not fit the matrix-core batches with byte offsets");
     b->lds = std::max(b->lds, ep->lds);
     args.push_back(matrix_args(ctx, *P, *ep));
-    types.push_back(type);
+    types.push_back(type % 3);  // slot in the family's kernel
     const int64_t ns = elem_key(ctx).ns;
     for (int t = 0; t < ep->tiles.ntiles; t++)
       mitems.push_back({ns * (ep->tiles.tile_cell_ptr[t + 1] - ep->tiles.tile_cell_ptr[t]), make_int2((int)p, t)});

@@ -106,10 +106,13 @@ struct KernelInfo {
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out);
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);
-// Batched kernel over 2D Q1/Q2/Q3 quadrilaterals (type index of a problem's
-// element: batched_type; -1 if the element is not one of them), 8 or 12
-// waves per workgroup.
+// Batched kernels: family 0 over scalar Q1/Q2/Q3 quadrilaterals with
+// (k+1)^2 Gauss points (types 0, 1, 2), family 1 over scalar Q1/Q2
+// hexahedra with (k+1)^3 (types 3, 4); batched_type gives a problem's type
+// (-1: none), type / 3 its family and type % 3 its slot in the family's
+// kernel; 8 or 12 waves per workgroup.
+constexpr int kBatchedTypes = 5;
 int batched_type(const ElemKey &key);
-bool find_batched_kernel(int form, int precision, int waves, KernelInfo &out);
+bool find_batched_kernel(int form, int precision, int waves, int family, KernelInfo &out);
 
 }  // namespace gtk

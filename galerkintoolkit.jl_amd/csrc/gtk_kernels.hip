@@ -1888,29 +1888,38 @@ int batched_type(const ElemKey &key) {
   if (match<QuadQ1>(key)) return 0;
   if (match<QuadQ2>(key)) return 1;
   if (match<QuadQ3>(key)) return 2;
+  if (match<HexQ1>(key)) return 3;
+  if (match<HexQ2>(key)) return 4;
   return -1;
 }
 
-template <int FORM, typename CT, int NW>
+template <int FORM, typename CT, int NW, int FAMILY>
 static KernelInfo batched_info() {
   KernelInfo k;
-  k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, QuadQ1, QuadQ2, QuadQ3>);
+  if constexpr (FAMILY == 0)
+    k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, QuadQ1, QuadQ2, QuadQ3>);
+  else  // the third slot is unused
+    k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, HexQ1, HexQ2, HexQ2>);
   k.mfma = true;
   k.block = NW * 64;
   return k;
 }
 
-template <int FORM>
+template <int FORM, int FAMILY>
 static KernelInfo batched_info(int precision, int waves) {
-  if (waves == 12) return precision == 0 ? batched_info<FORM, double, 12>() : batched_info<FORM, float, 12>();
-  return precision == 0 ? batched_info<FORM, double, 8>() : batched_info<FORM, float, 8>();
+  if (waves == 12)
+    return precision == 0 ? batched_info<FORM, double, 12, FAMILY>() : batched_info<FORM, float, 12, FAMILY>();
+  return precision == 0 ? batched_info<FORM, double, 8, FAMILY>() : batched_info<FORM, float, 8, FAMILY>();
 }
 
-bool find_batched_kernel(int form, int precision, int waves, KernelInfo &out) {
-  if (waves != 8 && waves != 12) return false;
-  if (form == kLaplace) out = batched_info<kLaplace>(precision, waves);
-  else if (form == kMass) out = batched_info<kMass>(precision, waves);
-  else return false;
+bool find_batched_kernel(int form, int precision, int waves, int family, KernelInfo &out) {
+  if ((waves != 8 && waves != 12) || family < 0 || family > 1) return false;
+  if (form == kLaplace)
+    out = family == 0 ? batched_info<kLaplace, 0>(precision, waves) : batched_info<kLaplace, 1>(precision, waves);
+  else if (form == kMass)
+    out = family == 0 ? batched_info<kMass, 0>(precision, waves) : batched_info<kMass, 1>(precision, waves);
+  else
+    return false;
   return true;
 }
 

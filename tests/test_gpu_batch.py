@@ -2,7 +2,9 @@
 distortion, coefficient fields) assembled by one persistent launch captured
 in a HIP graph, each problem's matrix checked against the oracle (pattern
 bit-exact, FP64 values within relative Frobenius error 1e-13, mixed within
-1e-6) and against its own single-problem assembly."""
+1e-6) and against its own single-problem assembly; the same for batches of
+uneven 3D problems (Q1/Q2 hexahedra), whose kernel is a second
+instantiation of the batched kernel."""
 import numpy as np
 import pytest
 
@@ -160,7 +162,7 @@ def test_batch_autotune_cache(GT, O, tmp_path):
     b = GT.MatrixBatch(forms, spaces, precision=GT.MIXED, tuning="auto", cache=cache)
     rep = b.tuning_report
     orders = {q["order"] for q in probs}
-    types = [GT.batch.ELEM_NAMES[GT.batch.ELEM_TYPES[o]] for o in orders]
+    types = [GT.batch.ELEM_NAMES[GT.batch.ELEM_TYPES[(GT.refcell.QUAD, o)]] for o in orders]
     assert all(not rep[t]["cached"] for t in types)
     for t in types:
         assert rep[t]["max_cells"] in GT.batch.TILE_CANDIDATES
@@ -178,7 +180,7 @@ def test_batch_autotune_cache(GT, O, tmp_path):
 
 
 def test_batch_rejects(GT, O):
-    m = GT.cartesian_mesh((0.0, 1.0) * 3, (2, 2, 2))
+    m = GT.cartesian_mesh((0.0, 1.0) * 3, (2, 2, 2), simplexify=True)  # tetrahedra: no batched kernel
     V = GT.lagrange_space(GT.interior(m), 1)
     with pytest.raises(NotImplementedError):
         GT.MatrixBatch([GT.Laplace(GT.measure(GT.interior(m), 2))], [V], tuning=None)
@@ -189,3 +191,66 @@ def test_batch_rejects(GT, O):
         GT.MatrixBatch([GT.Laplace(dO), GT.Mass(dO)], [V2, V2], tuning=None)
     with pytest.raises(NotImplementedError):  # not (order+1)^2 points
         GT.MatrixBatch([GT.Laplace(GT.measure(GT.interior(m2), 6))], [V2], tuning=None)
+
+
+def make_problems_3d(GT, O, count, seed, nmin=2, nmax=6):
+    u = GT.inputs.uniform(seed, 5 * count)
+    out = []
+    for i in range(count):
+        cells = tuple(nmin + int(u[5 * i + d] * (nmax - nmin + 1)) for d in range(3))
+        order = 1 + int(u[5 * i + 3] * 2)
+        m = GT.cartesian_mesh((0.0, 1.0) * 3, cells)
+        if u[5 * i + 4] < 0.5:
+            m.coords = GT.perturb_interior_nodes(m, 0.15, seed + i)
+        Om = GT.interior(m)
+        V = GT.lagrange_space(Om, order, dirichlet_boundary=GT.boundary(m))
+        dO = GT.measure(Om, 2 * order)
+        kappa = GT.inputs.lognormal(seed + 1000 + i, m.cells.shape[0])
+        p = O.Problem(list(cells)).space(order, dirichlet=1)
+        p.set_coords(m.coords)
+        p.quadrature(2 * order)
+        out.append(dict(m=m, Om=Om, V=V, dO=dO, kappa=kappa, p=p, order=order))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["laplace", "mass"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_3d_matches_oracle(GT, O, kind, precision):
+    print(f"case batch 3d {kind} {precision} 8 problems", flush=True)
+    probs = make_problems_3d(GT, O, 8, seed=61)
+    assert {q["order"] for q in probs} == {1, 2}
+    prec = GT.FP64 if precision == "fp64" else GT.MIXED
+    forms = forms_of(GT, probs, kind)
+    loads, lref = loads_of(GT, probs, 700)
+    tol = TOL_FP64 if precision == "fp64" else TOL_MIXED
+    code = O.LAPLACE if kind == "laplace" else O.MASS
+    for tuning in (None, {1: 16, 2: 16, "waves": 8}):
+        b = GT.MatrixBatch(forms, [q["V"] for q in probs], precision=prec, tuning=tuning, loads=loads)
+        b.run()
+        for q, A in zip(probs, b.matrices()):
+            cp, rv, nz = q["p"].assemble_matrix(code, coef=q["kappa"])
+            assert np.array_equal(A.colptr, cp) and np.array_equal(A.rowval, rv)
+            assert relfro(A.nzval, nz) <= tol, (q["order"], q["m"].cells.shape, tuning)
+        for q, v, (fk, fd) in zip(probs, b.vectors(), lref):
+            assert relfro(v, q["p"].assemble_vector(fk, fd)) <= TOL_FP64, (q["order"], fk)
+        b.close()
+
+
+def test_batch_3d_waves_and_kinds(GT, O):
+    """A 3D batch at 12 waves gives the 8-wave values bit for bit, and a
+    batch mixing quadrilaterals and hexahedra is refused."""
+    print("case batch 3d waves / mixed kinds", flush=True)
+    probs = make_problems_3d(GT, O, 5, seed=19)
+    spaces = [q["V"] for q in probs]
+    forms = forms_of(GT, probs, "laplace")
+    vals = []
+    for w in (8, 12):
+        b = GT.MatrixBatch(forms, spaces, precision=GT.MIXED, tuning={"waves": w})
+        b.run()
+        vals.append([A.nzval.copy() for A in b.matrices()])
+        b.close()
+    for a, c in zip(*vals):
+        assert np.array_equal(a, c)
+    q2 = make_problems(GT, O, 1, seed=3, with_oracle=False)[0]
+    with pytest.raises(NotImplementedError):
+        GT.MatrixBatch(forms[:1] + forms_of(GT, [q2], "laplace"), spaces[:1] + [q2["V"]], tuning=None)
