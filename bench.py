@@ -423,17 +423,22 @@ def bench_dg(GT, args, n):
     }
 
 
-def batch_problems(GT, count, seed, nmin=16, nmax=256):
-    """`count` unit-square Poisson problems: n per side uniform in
-    [nmin, nmax], Lagrange order uniform in {1, 2, 3}, Gauss with order+1
-    points per direction, Dirichlet on the whole boundary, lognormal kappa
-    per cell and Gaussian nodal f (seeded per problem)."""
+BATCH_SHAPES = {2: (16, 256, 3), 3: (8, 40, 2)}  # n per side in [nmin, nmax], orders 1..kmax
+
+
+def batch_problems(GT, count, seed, dim=2):
+    """`count` unit-square (dim 2) or unit-cube (dim 3) Poisson problems: n
+    per side uniform in [16, 256] (3D: [8, 40]), Lagrange order uniform in
+    {1, 2, 3} (3D: {1, 2}), Gauss with order+1 points per direction,
+    Dirichlet on the whole boundary, lognormal kappa per cell and Gaussian
+    nodal f (seeded per problem)."""
+    nmin, nmax, kmax = BATCH_SHAPES[dim]
     u = GT.inputs.uniform(seed, 2 * count)
     probs = []
     for i in range(count):
         n = nmin + int(u[2 * i] * (nmax - nmin + 1))
-        order = 1 + int(u[2 * i + 1] * 3)
-        m = GT.cartesian_mesh((0.0, 1.0, 0.0, 1.0), (n, n))
+        order = 1 + int(u[2 * i + 1] * kmax)
+        m = GT.cartesian_mesh((0.0, 1.0) * dim, (n,) * dim)
         Om = GT.interior(m)
         V = GT.lagrange_space(Om, order, dirichlet_boundary=GT.boundary(m))
         dO = GT.measure(Om, 2 * order)
@@ -451,7 +456,7 @@ def cpu_baseline_batch(probs, stride):
     sample = probs[::stride]
     cells, dt = 0, 0.0
     for q in sample:
-        p = O.Problem((q["n"], q["n"])).space(q["order"], dirichlet=1)
+        p = O.Problem((q["n"],) * q["m"].D).space(q["order"], dirichlet=1)
         p.quadrature(2 * q["order"])
         t0 = time.perf_counter()
         p.assemble_matrix(O.LAPLACE, coef=q["kappa"])
@@ -463,11 +468,11 @@ def cpu_baseline_batch(probs, stride):
                       f"+ nodal load vector + COO->CSC in the reference loop order, single thread, {dt:.2f} s"}
 
 
-def bench_batch(GT, args, count):
-    """Batched uneven 2D Poisson: every problem's matrix and load vector in
-    one persistent launch replayed from a HIP graph."""
+def bench_batch(GT, args, count, dim=2):
+    """Batched uneven 2D (or 3D) Poisson: every problem's matrix and load
+    vector in one persistent launch replayed from a HIP graph."""
     t_setup = time.perf_counter()
-    probs = batch_problems(GT, count, args.seed)
+    probs = batch_problems(GT, count, args.seed, dim)
     forms = [GT.Laplace(q["dO"], GT.face_constant_field(q["kappa"], q["Om"])) for q in probs]
     loads = [GT.Load(q["dO"], GT.DiscreteField(q["V"], q["f"][:q["V"].nfree], q["f"][q["V"].nfree:]))
              for q in probs]
@@ -485,7 +490,7 @@ def bench_batch(GT, args, count):
         V = q["V"]
         nd = V.cell_dofs.shape[1]
         nnz += pnnz
-        nbytes += alg_bytes(2, q["m"].coords.shape[0], nc, 4, nd, 1, pnnz)
+        nbytes += alg_bytes(dim, q["m"].coords.shape[0], nc, 1 << dim, nd, 1, pnnz)
         nbytes += 8 * (V.nfree + V.ndiri) + 8 * V.nfree  # nodal f in, load vector out
 
     def timed(b):
@@ -505,9 +510,12 @@ def bench_batch(GT, args, count):
     verrs = np.array([np.linalg.norm(v - r) / np.linalg.norm(r) for v, r in zip(bm.vectors(), vref)])
     orders = [q["order"] for q in probs]
     achieved = nbytes / (kernmx * 1e-3) / 1e9
-    traffic = measured_traffic("batch256", problems=count, seed=args.seed)
+    traffic = measured_traffic("batch256", problems=count, seed=args.seed) if dim == 2 else None
+    nmin, nmax, kmax = BATCH_SHAPES[dim]
+    shape = "unit-square" if dim == 2 else "unit-cube"
     out = {
-        "metric": "aggregate cells/s, batched uneven 2D Poisson (matrix + load vector per problem), one graph launch",
+        "metric": f"aggregate cells/s, batched uneven {dim}D Poisson (matrix + load vector per problem), one graph "
+                  f"launch",
         "value": cells / (wallmx * 1e-3),
         "unit": "cells/s",
         "n_gpus": 1,
@@ -519,11 +527,12 @@ def bench_batch(GT, args, count):
         "vs_baseline": None,
         "dtype": "fp32 contraction, fp64 geometry + accumulation",
         "data": "synthetic",
-        "config": {"workload": f"{count} unit-square problems, n per side U[16,256], Lagrange order U{{1,2,3}}, "
-                               f"(order+1)^2 Gauss, lognormal kappa per cell, Gaussian nodal f, Dirichlet boundary "
-                               f"(seed {args.seed})",
-                   "name": "batch", "problems": count, "cells": int(cells), "nnz": int(nnz),
-                   "orders": {str(o): orders.count(o) for o in (1, 2, 3)}, "parallelism": "single GPU",
+        "config": {"workload": f"{count} {shape} problems, n per side U[{nmin},{nmax}], Lagrange order "
+                               f"U{{1..{kmax}}}, (order+1)^{dim} Gauss, lognormal kappa per cell, Gaussian nodal f, "
+                               f"Dirichlet boundary (seed {args.seed})",
+                   "name": "batch" if dim == 2 else "batch3d", "problems": count, "cells": int(cells),
+                   "nnz": int(nnz), "orders": {str(o): orders.count(o) for o in range(1, kmax + 1)},
+                   "parallelism": "single GPU",
                    "launch": {name: {"waves": b.waves,
                                      "tiles": {GT.batch.ELEM_NAMES[k]: v for k, v in b.tiles.items()}}
                               for name, b in (("mixed", bm), ("fp64", b64))}},
@@ -539,7 +548,7 @@ def bench_batch(GT, args, count):
         "setup_s": setup_s,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_batch(probs, args.batch_cpu_stride)
+        out["cpu_baseline"] = cpu_baseline_batch(probs, args.batch_cpu_stride if dim == 2 else 8)
     return out
 
 
@@ -557,11 +566,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["all", "q2hex", "q1hex", "p2tet", "batch", "dgip"], default="all",
+    ap.add_argument("--workload", choices=["all", "q2hex", "q1hex", "p2tet", "batch", "batch3d", "dgip"],
+                    default="all",
                     help="all (default): the Q2 hex headline with the others as sub-results of the same line; "
                          "q2hex: Q2 hex 64^3 perturbed Laplace (the headline) alone; q1hex: Q1 hex 128^3 with "
                          "lognormal kappa; p2tet: P2 tet elasticity 48^3 x 6 with the refined solve; batch: 256 "
-                         "uneven 2D Poisson problems in one graph launch; dgip: interior-penalty DG on 512^2 "
+                         "uneven 2D Poisson problems in one graph launch; batch3d: 64 uneven 3D ones (Q1/Q2 hexahedra); "
+                         "dgip: interior-penalty DG on 512^2 "
                          "discontinuous Q2 quadrilaterals (volume, skeleton and Nitsche kernels)")
     ap.add_argument("--problems", type=int, default=256, help="problems of the batch workload")
     ap.add_argument("--batch-cpu-stride", type=int, default=2, help="CPU baseline: every k-th problem of the batch")
@@ -612,6 +623,8 @@ def main():
         out = bench_elasticity(GT, args, args.cells or 48)
     elif wl == "dgip":
         out = bench_dg(GT, args, args.cells or 512)
+    elif wl == "batch3d":
+        out = bench_batch(GT, args, 64, dim=3)
     else:
         out = bench_batch(GT, args, args.problems)
     print(json.dumps(out))
