@@ -258,6 +258,14 @@ static gtk::ElemKey elem_key(const gtk_ctx *ctx) {
 
 // LDS available to one workgroup (gfx950: 160 KiB per CU), less headroom.
 static constexpr size_t kLdsBudget = 156 * 1024;
+// LDS of a load-vector tile of its own launch: several workgroups share a
+// CU and hide each other's gathers (Q2 hex 64^3 load vector 0.537 -> 0.417
+// ms, Q1 quadrilaterals 512^2 0.031 -> 0.020 ms against tiles as large as
+// LDS allows).  In a batch, where each workgroup holds a CU's LDS for the
+// matrix tiles, the vector tiles stay that large (40 KB tiles there: the
+// 256-problem batch 4.49 -> 5.73 ms).
+static constexpr size_t kVecLdsBudget = 40 * 1024;
+static constexpr int kBatchedPlan = 0x100;  // vplans key bit: the tiles of a batch
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
@@ -289,13 +297,13 @@ static int tile_cells_for(const gtk::KernelInfo &k, size_t acc_bytes, size_t bud
 
 // Tile limits of the vector kernel: per cell f . dV at the points, the node
 // coordinates and the nodal values; tables phi and the geometry gradients.
-static void vector_tile_limits(const gtk_ctx *ctx, int &max_cells, int &chunk) {
+static void vector_tile_limits(const gtk_ctx *ctx, size_t budget, int &max_cells, int &chunk) {
   const size_t fs = (size_t)ctx->nq * ctx->ncomp | 1;  // odd per-cell stride of f . dV
   const size_t geo =
       (fs + (size_t)ctx->nv * ctx->dim + (size_t)ctx->nshape * ctx->ncomp + (ctx->nfaces ? 1 : 0)) * 8;
   const size_t tables = (size_t)std::max(1, ctx->nfaces) * ctx->nq *
                         (((size_t)ctx->nshape | 1) + ((size_t)ctx->nv * ctx->dim | 1)) * 8;
-  const size_t avail = kLdsBudget > tables + 1024 ? kLdsBudget - tables - 1024 : 8 * geo;
+  const size_t avail = budget > tables + 1024 ? budget - tables - 1024 : 8 * geo;
   max_cells = (int)std::max<size_t>(8, std::min<size_t>(4096, avail / geo));
   chunk = tile_chunk(ctx, max_cells);
 }
@@ -937,12 +945,12 @@ static int get_exec(gtk_ctx *ctx, MatrixPlan &plan, int form, int precision, Exe
 
 // Kernel arguments of a work plan.
 // Tile plan and output of the load vector over a rows set, built on first use.
-static int get_vector_plan(gtk_ctx *ctx, int rows, VectorPlan *&out) {
-  auto &slot = ctx->vplans[rows];
+static int get_vector_plan(gtk_ctx *ctx, int rows, VectorPlan *&out, bool batched = false) {
+  auto &slot = ctx->vplans[rows | (batched ? kBatchedPlan : 0)];
   if (!slot) {
     auto vp = std::make_unique<VectorPlan>();
     int max_cells, chunk;
-    vector_tile_limits(ctx, max_cells, chunk);
+    vector_tile_limits(ctx, batched ? kLdsBudget : kVecLdsBudget, max_cells, chunk);
     std::string e = gtk::build_tiles(view(ctx), rows, max_cells, 1 << 20, 1 << 20, chunk, vp->tiles, nthreads_host());
     if (!e.empty()) FAIL(ctx, GTK_EINVAL, e);
     int rc = upload_tiles(ctx, vp->tiles, vp->tile_cell_ptr, vp->tile_cells, vp->tile_col_ptr, vp->col_id,
@@ -1756,7 +1764,7 @@ int gtk_batch_get_vector(gtk_batch *b, int problem, double *out) {
   if (problem < 0 || problem >= (int)b->ctxs.size()) BFAIL(b, GTK_EINVAL, "problem index out of range");
   if (b->loads[problem]->fkind < 0) BFAIL(b, GTK_ESTATE, "the problem has no load (gtk_batch_set_load)");
   gtk_ctx *ctx = b->ctxs[problem];
-  auto it = ctx->vplans.find(GTK_FREE);
+  auto it = ctx->vplans.find(GTK_FREE | kBatchedPlan);
   if (it == ctx->vplans.end() || !it->second) BFAIL(b, GTK_ESTATE, "gtk_batch_build first");
   BHIPCHK(b, hipSetDevice(b->device));
   if (it->second->nrows)
@@ -1827,7 +1835,8 @@ int gtk_batch_build(gtk_batch *b) {
     gtk::KernelInfo vk;
     if (!gtk::find_vector_kernel(elem_key(ctx), l.fkind, vk)) BFAIL(b, GTK_EUNSUP, "no vector kernel for a problem");
     VectorPlan *vp = nullptr;
-    if (get_vector_plan(ctx, GTK_FREE, vp)) BFAIL(b, GTK_EINVAL, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
+    if (get_vector_plan(ctx, GTK_FREE, vp, true))
+      BFAIL(b, GTK_EINVAL, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
     if (vp->tiles.uncovered) BFAIL(b, GTK_EUNSUP, "a problem's load vector has rows no cell touches");
     b->lds = std::max(b->lds, vk.table_bytes + (size_t)vp->tiles.max_tile_cells * vk.geo_bytes_per_cell);
     vargs[p] = vector_args(ctx, *vp, l.f.as<double>());
@@ -1916,7 +1925,7 @@ int gtk_batch_run(gtk_batch *b, gtk_stats *stats) {
   for (size_t p = 0; p < b->ctxs.size(); p++) {
     const gtk_batch::Load &l = *b->loads[p];
     if (l.fkind < 0) continue;
-    auto it = b->ctxs[p]->vplans.find(GTK_FREE);
+    auto it = b->ctxs[p]->vplans.find(GTK_FREE | kBatchedPlan);
     if (it == b->ctxs[p]->vplans.end() || !it->second)
       BFAIL(b, GTK_ESTATE, "a problem's load-vector plan is gone: gtk_batch_build again");
     const gtk::VectorArgs v = vector_args(b->ctxs[p], *it->second, l.f.as<double>());
