@@ -237,11 +237,19 @@ __global__ void k_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, i
 }
 
 // one thread per column: every dst entry gets at most one src entry
+// kAddLanes lanes per column: lane l takes the column's source entries
+// l, l + kAddLanes, ...; the binary searches of a column's lanes read the
+// same destination segment (21.2 M volume entries of a 512^2 Q2 DG problem
+// into its skeleton pattern: 3.09 -> 0.47 ms against a thread per column)
+constexpr int kAddLanes = 16;
+
 __global__ void k_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,
                             const int32_t *dcolptr, const int32_t *drowval, double *dval, int32_t *missing) {
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(t0 % kAddLanes);
+  for (int64_t j = t0 / kAddLanes; j < ncols; j += (int64_t)gridDim.x * blockDim.x / kAddLanes) {
     const int d0 = dcolptr[j] - 1, d1 = dcolptr[j + 1] - 1;
-    for (int k = scolptr[j] - 1; k < scolptr[j + 1] - 1; k++) {
+    for (int k = scolptr[j] - 1 + lane; k < scolptr[j + 1] - 1; k += kAddLanes) {
       const int32_t r = srowval[k];
       int lo = d0, hi = d1;
       while (lo < hi) {
@@ -318,7 +326,8 @@ void launch_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, int row
 void launch_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,
                       const int32_t *dcolptr, const int32_t *drowval, double *dval, int32_t *missing, hipStream_t st) {
   if (ncols == 0) return;
-  k_add_block<<<grid_for(ncols, 256), 256, 0, st>>>(ncols, scolptr, srowval, sval, dcolptr, drowval, dval, missing);
+  k_add_block<<<grid_for(ncols * kAddLanes, 256), 256, 0, st>>>(ncols, scolptr, srowval, sval, dcolptr, drowval, dval,
+                                                                 missing);
 }
 
 }  // namespace gtk
