@@ -335,14 +335,56 @@ class _NitscheAssembler(_Assembler):
         self.h = _diameters(xyz, self.fnodes)
 
 
+def skeleton_items(space, dL):
+    """Host side of the skeleton items of a space over a skeleton measure:
+    one item per interior face with both its cells (the lower cell id
+    first, side 0), and the point sets of the face rule: for each side, the
+    rule's points in that cell's reference coordinates, side 1's listed in
+    the physical order of side 0's through the vertices the two local faces
+    share.  Returns a dict: cells and lfaces (items x 2), item_sets (items x
+    2, the set of each side), fnodes (side 0's face vertices, mesh nodes),
+    and per set: the local face and vertex order (keys) and the points in
+    the cell's reference coordinates (points, sets x nq x D)."""
+    mesh = space.mesh
+    D = mesh.D
+    faces = dL.domain.faces
+    cf = mesh.cell_faces[D - 1]
+    nlf = cf.shape[1]
+    flat = cf.ravel()
+    order = np.argsort(flat, kind="stable")  # (cell, local face) slots grouped by face, cells ascending
+    sf = flat[order]
+    start = np.searchsorted(sf, faces)
+    if np.any(np.searchsorted(sf, faces, side="right") - start != 2):
+        raise ValueError("a face of the skeleton does not have two cells")
+    p0, p1 = order[start], order[start + 1]
+    cells = np.stack([p0 // nlf, p1 // nlf], axis=1)
+    lfaces = np.stack([p0 % nlf, p1 % nlf], axis=1)
+    g = R.GEO[mesh.cellkind]
+    fv = np.asarray(g.faces[D - 1]) - 1  # (nlf, face vertices)
+    V0 = np.take_along_axis(mesh.cells[cells[:, 0]], fv[lfaces[:, 0]], axis=1)  # side 0's face, mesh nodes
+    V1 = np.take_along_axis(mesh.cells[cells[:, 1]], fv[lfaces[:, 1]], axis=1)
+    # perm[k]: position in side 1's local face of side 0's k-th vertex
+    perm = np.argmax(V1[:, None, :] == V0[:, :, None], axis=2)
+    if not np.all(np.take_along_axis(V1, perm, axis=1) == V0):
+        raise ValueError("the two sides of a face do not share its vertices")
+    nfv = fv.shape[1]
+    nit = len(faces)
+    # point sets: distinct (local face, vertex order) keys of the sides
+    side_keys = np.concatenate([
+        np.concatenate([lfaces[:, :1], np.broadcast_to(np.arange(nfv), (nit, nfv))], axis=1),
+        np.concatenate([lfaces[:, 1:], perm], axis=1)])
+    keys, inv = np.unique(side_keys, axis=0, return_inverse=True)
+    N = R.linear_shape(g.ftype[D - 1], dL.x)
+    points = np.stack([N @ np.asarray([g.verts[v] for v in fv[int(k[0])]], np.float64)[k[1:]] for k in keys])
+    return dict(cells=cells, lfaces=lfaces, item_sets=inv.reshape(2, nit).T, fnodes=V0, keys=keys, points=points)
+
+
 class _SkeletonAssembler(_Assembler):
-    """The device context of interior-penalty terms over a skeleton measure:
-    one item per interior face with both its cells (their nodes and DOFs,
-    the lower cell id first), and point sets of the face rule: for each
-    side, the rule's points in that cell's reference coordinates, side 1's
-    listed in the physical order of side 0's through the vertices the two
-    local faces share; the reference gradients and normals of each set's
-    local face; the face diameters."""
+    """The device context of interior-penalty terms over a skeleton measure
+    (the items and point sets of skeleton_items): the cells' shape
+    functions, reference gradients and order-1 geometry gradients at each
+    set's points, the set's local face map and reference outward normal,
+    and the face diameters."""
 
     def __init__(self, space, dL, device=0):
         mesh = space.mesh
@@ -357,53 +399,27 @@ class _SkeletonAssembler(_Assembler):
             raise NotImplementedError("skeleton terms: quadrilaterals and hexahedra")
         self.space = space
         self.dOmega = dL
-        faces = dL.domain.faces
-        cf = mesh.cell_faces[D - 1]
-        nlf = cf.shape[1]
-        flat = cf.ravel()
-        order = np.argsort(flat, kind="stable")  # (cell, local face) slots grouped by face, cells ascending
-        sf = flat[order]
-        start = np.searchsorted(sf, faces)
-        if np.any(np.searchsorted(sf, faces, side="right") - start != 2):
-            raise ValueError("a face of the skeleton does not have two cells")
-        p0, p1 = order[start], order[start + 1]
-        cells = np.stack([p0 // nlf, p1 // nlf], axis=1)  # (items, 2)
-        lfaces = np.stack([p0 % nlf, p1 % nlf], axis=1)
+        it = skeleton_items(space, dL)
+        cells = it["cells"]
+        nit = len(cells)
         # every cell must lie on an interior face: the skeleton pattern then
         # holds the volume and boundary blocks of a sum
         if not np.all(np.bincount(cells.ravel(), minlength=mesh.cells.shape[0]) > 0):
             raise NotImplementedError("skeleton terms: every cell needs an interior face")
-        g = R.GEO[mesh.cellkind]
-        fv = np.asarray(g.faces[D - 1]) - 1  # (nlf, face vertices)
-        V0 = np.take_along_axis(mesh.cells[cells[:, 0]], fv[lfaces[:, 0]], axis=1)  # side 0's face, mesh nodes
-        V1 = np.take_along_axis(mesh.cells[cells[:, 1]], fv[lfaces[:, 1]], axis=1)
-        # perm[k]: position in side 1's local face of side 0's k-th vertex
-        perm = np.argmax(V1[:, None, :] == V0[:, :, None], axis=2)
-        if not np.all(np.take_along_axis(V1, perm, axis=1) == V0):
-            raise ValueError("the two sides of a face do not share its vertices")
-        nfv = fv.shape[1]
-        nit = len(faces)
-        # point sets: distinct (local face, vertex order) keys of the sides
-        side_keys = np.concatenate([
-            np.concatenate([lfaces[:, :1], np.broadcast_to(np.arange(nfv), (nit, nfv))], axis=1),
-            np.concatenate([lfaces[:, 1:], perm], axis=1)])
-        uniq, inv = np.unique(side_keys, axis=0, return_inverse=True)
-        item_sets = inv.reshape(2, nit).T
         nsets = 8 if D == 2 else 12
-        if len(uniq) > nsets:
-            raise NotImplementedError(f"skeleton terms: {len(uniq)} point sets (at most {nsets})")
-        ft = g.ftype[D - 1]
-        N = R.linear_shape(ft, dL.x)
-        _, dNf = R.Lagrange(ft, 1).tabulate(dL.x[:1])
+        if len(it["keys"]) > nsets:
+            raise NotImplementedError(f"skeleton terms: {len(it['keys'])} point sets (at most {nsets})")
+        g = R.GEO[mesh.cellkind]
+        fv = np.asarray(g.faces[D - 1]) - 1
+        _, dNf = R.Lagrange(g.ftype[D - 1], 1).tabulate(dL.x[:1])
         geo = R.Lagrange(mesh.cellkind, 1)
         fe = space.fe
         normals = R.face_normals(mesh.cellkind)
         nq, nv = len(dL.w), len(g.verts)
         phi, gphi, gN, A, nrm = [], [], [], [], []
-        for key in uniq:
-            l, pk = int(key[0]), key[1:]
-            X = np.asarray([g.verts[v] for v in fv[l]], np.float64)[pk]
-            xc = N @ X
+        for key, xc in zip(it["keys"], it["points"]):
+            l = int(key[0])
+            X = np.asarray([g.verts[v] for v in fv[l]], np.float64)[key[1:]]
             val, grad = fe.tabulate(xc)
             phi.append(val)
             gphi.append(grad)
@@ -414,8 +430,9 @@ class _SkeletonAssembler(_Assembler):
             nrm.append(normals[l])
         while len(phi) < nsets:  # unused sets: copies of the first
             phi.append(phi[0]), gphi.append(gphi[0]), gN.append(gN[0]), A.append(A[0]), nrm.append(nrm[0])
-        self.fnodes = V0
+        self.fnodes = it["fnodes"]
         self.h = _diameters(mesh.coords, self.fnodes)
+        item_sets = it["item_sets"]
         item_nodes = mesh.cell_nodes_1b()[cells].reshape(nit, -1)
         item_dofs = space.cell_dofs[cells].reshape(nit, -1)
         item_face = (item_sets[:, 0] | (item_sets[:, 1] << 16)).astype(np.int32)

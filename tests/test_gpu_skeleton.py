@@ -12,6 +12,8 @@ through the C ABI:
   volume and boundary blocks added into the skeleton block on the device);
 - update_matrix with moved nodes (the face diameters follow) equals a fresh
   assembly;
+- cells listing their vertices rotated (non-identity point sets) give the
+  matrix of the standard vertex order, up to the renumbering of their DOFs;
 - the volume load on a discontinuous space, and the device solve of a
   positive definite interior-penalty system against a direct solve;
 - the reference's patch tests (poisson_tests.jl:31-44): interior penalty
@@ -208,3 +210,44 @@ def test_dg_update_coordinates(GT, O):
     B = GT.assemble_matrix(f2, float, V2, V2)
     assert np.array_equal(A.colptr, B.colptr) and np.array_equal(A.rowval, B.rowval)
     assert np.linalg.norm(A.nzval - B.nzval) <= 1e-14 * np.linalg.norm(B.nzval)
+
+
+def test_rotated_cells_match_standard(GT, O):
+    """Skeleton items whose two local faces list the shared vertices in
+    different orders (the second cell's vertices rotated, a non-identity
+    point set): the matrix equals that of the same geometry with standard
+    vertex order, up to the renumbering of that cell's DOFs (matched by
+    position), FP64 within 1e-13."""
+    from gtk_amd.mesh import Mesh
+    R = GT.refcell
+    print("case skeleton rotated cells", flush=True)
+    nid = lambda i, j, k: i + 3 * (j + 2 * k)
+    bits2 = [(n & 1, (n >> 1) & 1) for n in range(4)]
+    bits3 = [(n & 1, (n >> 1) & 1, (n >> 2) & 1) for n in range(8)]
+    c2 = np.array([[i, j] for j in range(2) for i in range(3)], float)
+    c3 = np.array([[i, j, k] for k in range(2) for j in range(2) for i in range(3)], float)
+    cases = [
+        (2, c2, [[a + 3 * b for a, b in bits2], [1 + a + 3 * b for a, b in bits2]],
+         [[a + 3 * b for a, b in bits2], [(2 - a) + 3 * (1 - b) for a, b in bits2]], R.QUAD),
+        (3, c3, [[nid(a, b, c) for a, b, c in bits3], [nid(1 + a, b, c) for a, b, c in bits3]],
+         [[nid(a, b, c) for a, b, c in bits3], [nid(1 + a, 1 - c, b) for a, b, c in bits3]], R.HEX)]
+    for D, coords, std, rot, kind in cases:
+        mats, pos = [], []
+        for cells in (std, rot):
+            m = Mesh(D, coords, np.array(cells), kind, {}, {})
+            V = GT.lagrange_space(GT.interior(m), 2, continuous=False)
+            dL = GT.measure(GT.skeleton(m), 4)
+            A = GT.assemble_matrix(GT.InteriorPenalty(dL, 2.0), float, V, V).to_scipy().toarray()
+            e = np.indices([3] * D).reshape(D, -1)[::-1].T / 2.0  # Q2 nodes, x fastest
+            P = np.zeros((V.nfree, D))
+            for c in range(2):
+                P[V.cell_dofs[c] - 1] = R.linear_shape(kind, e) @ m.coords[m.cells[c]]
+            mats.append(A)
+            pos.append(P)
+        # DOF k of the standard numbering sits where DOF perm[k] of the rotated one does
+        n = len(pos[0])
+        perm = np.array([int(np.argmin(np.linalg.norm(pos[1] - pos[0][k], axis=1)
+                                       + 10.0 * (np.arange(n) // (n // 2) != k // (n // 2)))) for k in range(n)])
+        assert sorted(perm) == list(range(n))
+        B = mats[1][np.ix_(perm, perm)]
+        assert np.abs(B - mats[0]).max() <= 1e-13 * np.abs(mats[0]).max(), D
