@@ -352,15 +352,20 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
       std::memset(off, 0xFF, (size_t)nd * off_row);
       for (int n = 0; n < kBatchSlots; n++) at[n] = 0xFFFFu << 16;
       const int32_t *cells = t.tile_cells.data() + t.tile_cell_ptr[batch_tile[b]];
-      int ns = 0, tmax = 0, abase = 0;
+      int ns = 0, tmax = 0, abase = 0, cd = 0;
       const int ncols = t.batch_col[b + 1] - t.batch_col[b];
       for (int ci = 0; ci < ncols; ci++) {
         const int32_t pc = t.batch_col[b] + ci;
         const int32_t j = t.col_id[pc];
         const int32_t *seg = p.rowval.data() + (p.colptr[j] - 1);
         const int32_t len = p.colptr[j + 1] - p.colptr[j];
-        delta[ci] = p.colptr[j] - 1 - abase;
-        if (len > 0) endmask[(abase + len - 1) / 64] |= 1ull << ((abase + len - 1) % 64);
+        // a column with no row of the row set (a Dirichlet vertex whose
+        // cells have only Dirichlet DOFs, in a FREE x DIRICHLET block) has
+        // nothing to add or write: no slots, no end bit, no delta (the
+        // write-out finds column c's delta at the c-th end bit)
+        if (len == 0) continue;
+        delta[cd++] = p.colptr[j] - 1 - abase;
+        endmask[(abase + len - 1) / 64] |= 1ull << ((abase + len - 1) % 64);
         const int cnt = t.col_ctr_ptr[pc + 1] - t.col_ctr_ptr[pc];
         tmax = std::max(tmax, cnt);
         ns = batch_slot_start(ns, cnt);  // slots skipped here stay empty
@@ -383,10 +388,10 @@ std::string build_batches(const MeshView &m, const Pattern &p, const TilePlan &t
         abase += len;
       }
       if (abase < 64 * out.rounds) {  // padding column up to the last round
-        delta[ncols] = (int32_t)(p.nnz - abase);
+        delta[cd] = (int32_t)(p.nnz - abase);
         endmask[out.rounds - 1] |= 1ull << 63;
       }
-      reinterpret_cast<uint32_t *>(r)[0] = (uint32_t)ns | (uint32_t)tmax << 8 | (uint32_t)ncols << 16;
+      reinterpret_cast<uint32_t *>(r)[0] = (uint32_t)ns | (uint32_t)tmax << 8 | (uint32_t)cd << 16;
       reinterpret_cast<uint32_t *>(r)[1] = (uint32_t)abase;
     }
   });

@@ -44,7 +44,8 @@ static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const g
       const int32_t *delta = reinterpret_cast<const int32_t *>(R + gtk::BatchPlan::kDelta);
       const uint64_t *endmask = reinterpret_cast<const uint64_t *>(R + gtk::BatchPlan::kEndmask);
       const uint8_t *off = R + bp.off_offset;
-      if (nslots > gtk::kBatchSlots || ncols > 16 || ncols < 1 || acc_total > bp.max_batch_acc)
+      if (nslots > gtk::kBatchSlots || ncols > 16 || ncols < (acc_total > 0 ? 1 : 0) ||
+          acc_total > bp.max_batch_acc)
         return fail("batch header out of range");
       std::vector<double> acc(acc_total, 0.0);
       std::vector<int> stamp(acc_total, -1);
@@ -89,7 +90,7 @@ static int replay_batches(const gtk::MeshView &m, const gtk::Pattern &p, const g
       if (seen != used) return fail("slots without a rank step");
       int ends = 0;
       for (int k = 0; k < acc_total; k++) ends += (endmask[k / 64] >> (k % 64)) & 1;
-      if (ends != ncols || !((endmask[(acc_total - 1) / 64] >> ((acc_total - 1) % 64)) & 1))
+      if (ends != ncols || (acc_total > 0 && !((endmask[(acc_total - 1) / 64] >> ((acc_total - 1) % 64)) & 1)))
         return fail("end mask does not close every column");
       if (ncols > gtk::kBatchCols || bp.rounds != (nd + 3) / 4) return fail("batch shape");
       // the kernel's write-out: 64*rounds entries, padding entries go past nnz

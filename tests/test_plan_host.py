@@ -95,6 +95,10 @@ def test_plan_replay(harness, GT, cells, order, simplex, diri, max_cells, chunk,
 @pytest.mark.parametrize("mode", [0, 1], ids=["offsets", "batches"])
 def test_plan_dirichlet_columns(harness, GT, mode):
     run(harness, GT, (5, 6), 2, diri=1, rows=1, cols=2, mode=mode)
+    # P1 triangles and tetrahedra: the corner vertices' columns hold no free
+    # row (every vertex of their cells is a Dirichlet vertex)
+    run(harness, GT, (6, 5), 1, simplex=True, diri=1, rows=1, cols=2, mode=mode)
+    run(harness, GT, (3, 3, 2), 1, simplex=True, diri=1, rows=1, cols=2, mode=mode)
 
 
 def test_batches_cover_hex_q2(harness, GT):
