@@ -395,8 +395,7 @@ class _SkeletonAssembler(_Assembler):
             # an item would list each DOF of the face twice, and a continuous
             # basis function has no jump, so every term is rounding
             raise NotImplementedError("skeleton terms on discontinuous spaces (lagrange_space(...; continuous=False))")
-        if mesh.cellkind not in (R.QUAD, R.HEX):
-            raise NotImplementedError("skeleton terms: quadrilaterals and hexahedra")
+
         self.space = space
         self.dOmega = dL
         it = skeleton_items(space, dL)

@@ -1829,6 +1829,12 @@ using TetP2v = Elem<3, 4, 10, 14, 3>;
 // Q2 quadrilaterals with integration degree 2 (2x2 Gauss points): the rule
 // GalerkinToolkitExamples/src/poisson.jl keeps by default
 using QuadQ2d2 = Elem<2, 4, 9, 4, 1>;
+// scalar P1/P2 triangles and tetrahedra with integration degree 2k (the
+// collapsed Gauss-Jacobi rules; the 11-point Strang rule for P2 tetrahedra)
+using TriP1 = Elem<2, 3, 3, 4, 1>;
+using TriP2 = Elem<2, 3, 6, 9, 1>;
+using TetP1 = Elem<3, 4, 4, 8, 1>;
+using TetP2 = Elem<3, 4, 10, 11, 1>;
 // boundary items: edges of Q1..Q3 quadrilaterals (k+1 Gauss points), faces
 // of Q1/Q2 hexahedra ((k+1)^2), faces of P2 vector tetrahedra (9-point rule)
 using QuadEdge1 = Elem<2, 4, 2, 2, 1, 4>;
@@ -1854,6 +1860,10 @@ using QuadSkeleton2 = Elem<2, 8, 9, 3, 1, 8>;
 using QuadSkeleton3 = Elem<2, 8, 16, 4, 1, 8>;
 using HexSkeleton1 = Elem<3, 16, 8, 4, 1, 12>;
 using HexSkeleton2 = Elem<3, 16, 27, 9, 1, 12>;
+using TriSkeleton1 = Elem<2, 6, 3, 2, 1, 8>;
+using TriSkeleton2 = Elem<2, 6, 6, 3, 1, 8>;
+using TetSkeleton1 = Elem<3, 8, 4, 4, 1, 12>;
+using TetSkeleton2 = Elem<3, 8, 10, 9, 1, 12>;
 
 bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_bytes, bool allow_mfma, int waves,
                         KernelInfo &out) {
@@ -1881,7 +1891,15 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<QuadSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<QuadSkeleton3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
-         try_matrix<HexSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out);
+         try_matrix<HexSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriP1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriP2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetP1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetP2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out);
 }
 
 int batched_type(const ElemKey &key) {
@@ -1930,7 +1948,9 @@ bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {
          try_vector<QuadEdge1>(key, fkind, out) || try_vector<QuadEdge2>(key, fkind, out) ||
          try_vector<QuadEdge3>(key, fkind, out) || try_vector<HexFace1>(key, fkind, out) ||
          try_vector<HexFace2>(key, fkind, out) || try_vector<TetFace2v>(key, fkind, out) ||
-         try_vector<QuadQ2d2>(key, fkind, out) || try_vector<QuadEdge2d2>(key, fkind, out);
+         try_vector<QuadQ2d2>(key, fkind, out) || try_vector<QuadEdge2d2>(key, fkind, out) ||
+         try_vector<TriP1>(key, fkind, out) || try_vector<TriP2>(key, fkind, out) ||
+         try_vector<TetP1>(key, fkind, out) || try_vector<TetP2>(key, fkind, out);
 }
 
 }  // namespace gtk
