@@ -34,13 +34,13 @@ pytestmark = pytest.mark.gpu
 TOL64, TOLMX = 1e-13, 1e-6
 
 
-def setup(GT, O, cells, order, perturb, box=None, seed=11):
+def setup(GT, O, cells, order, perturb, box=None, seed=11, simplex=False):
     D = len(cells)
     box = box or [0.0, 1.0] * D
-    m = GT.cartesian_mesh(tuple(box), cells)
+    m = GT.cartesian_mesh(tuple(box), cells, simplexify=simplex)
     if perturb:
         m.coords = GT.perturb_interior_nodes(m, perturb, seed)
-    p = O.Problem(list(cells), box=list(box)).space(order, dirichlet=0)
+    p = O.Problem(list(cells), box=list(box), simplexify=simplex).space(order, dirichlet=0)
     p.set_coords(m.coords)
     p.quadrature(2 * order)
     return m, p
@@ -60,11 +60,13 @@ def test_dg_numbering(GT, O, cells, order):
     assert np.array_equal(Vd.cell_dofs, ref.cd) and (Vd.nfree, Vd.ndiri) == (ref.nfree, ref.ndiri)
 
 
-@pytest.mark.parametrize("cells,order,perturb", [
-    ((4, 3), 1, 0.2), ((3, 3), 2, 0.2), ((3, 2), 3, 0.15), ((2, 3, 2), 1, 0.15), ((2, 2, 2), 2, 0.1)])
-def test_skeleton_matrix(GT, O, cells, order, perturb):
-    print(f"case skeleton matrix {cells} Q{order}", flush=True)
-    m, p = setup(GT, O, cells, order, perturb)
+@pytest.mark.parametrize("cells,order,perturb,simplex", [
+    ((4, 3), 1, 0.2, False), ((3, 3), 2, 0.2, False), ((3, 2), 3, 0.15, False), ((2, 3, 2), 1, 0.15, False),
+    ((2, 2, 2), 2, 0.1, False), ((4, 3), 1, 0.2, True), ((3, 3), 2, 0.2, True), ((2, 3, 2), 1, 0.1, True),
+    ((2, 2, 2), 2, 0.1, True)])
+def test_skeleton_matrix(GT, O, cells, order, perturb, simplex):
+    print(f"case skeleton matrix {cells} order {order} simplex={simplex}", flush=True)
+    m, p = setup(GT, O, cells, order, perturb, simplex=simplex)
     V = GT.lagrange_space(GT.interior(m), order, continuous=False)
     cd = DGSpace(p, order).cd
     assert np.array_equal(V.cell_dofs, cd)
@@ -251,3 +253,32 @@ def test_rotated_cells_match_standard(GT, O):
         assert sorted(perm) == list(range(n))
         B = mats[1][np.ix_(perm, perm)]
         assert np.abs(B - mats[0]).max() <= 1e-13 * np.abs(mats[0]).max(), D
+
+
+@pytest.mark.parametrize("cells,order", [((4, 3), 1), ((3, 3), 2), ((2, 2, 2), 1)])
+def test_dg_simplex_patch_strong(GT, O, cells, order):
+    """Interior penalty on triangles and tetrahedra, strong Dirichlet
+    conditions on the whole boundary, assembled and lifted on the device
+    (linear_problem): a linear u is reproduced to 1e-10."""
+    print(f"case dg simplex patch {cells} P{order}", flush=True)
+    D = len(cells)
+    m, p = setup(GT, O, cells, order, 0.15, simplex=True)
+    Om = GT.interior(m)
+    V = GT.lagrange_space(Om, order, dirichlet_boundary=GT.boundary(m), continuous=False)
+    u = lambda y: 1.0 + 2.0 * y[0] - y[1] + (0.5 * y[2] if D == 3 else 0.0)
+    on_bnd = lambda y: bool(np.any(np.abs(y) < 1e-12) or np.any(np.abs(y - 1.0) < 1e-12))
+    dg = DGSpace(p, order, dirichlet=on_bnd)
+    assert np.array_equal(V.cell_dofs, dg.cd)
+    pos = dg.geo.cell_pos
+    vals = np.zeros(V.nfree + V.ndiri)
+    for c in range(V.cell_dofs.shape[0]):
+        for a in range(V.cell_dofs.shape[1]):
+            d = V.cell_dofs[c, a]
+            vals[d - 1 if d > 0 else V.nfree - d - 1] = u(pos[c, a])
+    n = V.nfree
+    degree = 2 * order
+    dO, dL = GT.measure(Om, degree), GT.measure(GT.skeleton(m), degree)
+    uhd = GT.DiscreteField(V, np.zeros(n), vals[n:])
+    A, b = GT.linear_problem(uhd, GT.Laplace(dO) + GT.InteriorPenalty(dL, 10.0), GT.Load(dO, 0.0))
+    x = sla.spsolve(A.to_scipy().tocsc(), b)
+    assert np.abs(x - vals[:n]).max() < 1e-10 * np.abs(vals).max()

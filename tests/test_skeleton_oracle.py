@@ -30,10 +30,10 @@ from nitsche import NitscheFaces
 from skeleton import DGSpace, Skeleton
 
 
-def problem(GT, O, cells, order, box, perturb, seed=7):
-    m = GT.cartesian_mesh(tuple(box), cells)
+def problem(GT, O, cells, order, box, perturb, seed=7, simplex=False):
+    m = GT.cartesian_mesh(tuple(box), cells, simplexify=simplex)
     x = GT.perturb_interior_nodes(m, perturb, seed) if perturb else m.coords
-    p = O.Problem(list(cells), box=list(box)).space(order, dirichlet=0)
+    p = O.Problem(list(cells), box=list(box), simplexify=simplex).space(order, dirichlet=0)
     p.set_coords(x)
     p.quadrature(2 * order)
     return m, x, p
@@ -69,10 +69,12 @@ def test_dg_numbering_rules(GT, O, cells, order):
     assert np.array_equal(-dg.cd[diri], np.arange(1, dg.ndiri + 1))
 
 
-@pytest.mark.parametrize("cells,order,perturb", [((3, 3), 2, 0.2), ((3, 2), 1, 0.2), ((2, 2, 2), 1, 0.15)])
-def test_dg_volume_gathers_to_continuous(GT, O, cells, order, perturb):
+@pytest.mark.parametrize("cells,order,perturb,simplex", [
+    ((3, 3), 2, 0.2, False), ((3, 2), 1, 0.2, False), ((2, 2, 2), 1, 0.15, False),
+    ((3, 3), 2, 0.2, True), ((2, 2, 2), 1, 0.1, True)])
+def test_dg_volume_gathers_to_continuous(GT, O, cells, order, perturb, simplex):
     D = len(cells)
-    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb, simplex=simplex)
     cp, rv, nz = p.assemble_matrix(O.LAPLACE)
     n = len(cp) - 1
     K = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(n, n)).toarray()
@@ -112,10 +114,12 @@ def test_penalty_closed_form(GT, O):
     assert np.abs((S3 - S1) / 2.0 - ref).max() < 1e-14
 
 
-@pytest.mark.parametrize("cells,order,perturb", [((3, 3), 2, 0.2), ((3, 2), 1, 0.2), ((2, 2, 2), 1, 0.1)])
-def test_continuous_jumps_vanish(GT, O, cells, order, perturb):
+@pytest.mark.parametrize("cells,order,perturb,simplex", [
+    ((3, 3), 2, 0.2, False), ((3, 2), 1, 0.2, False), ((2, 2, 2), 1, 0.1, False),
+    ((3, 3), 2, 0.2, True), ((2, 2, 2), 2, 0.1, True)])
+def test_continuous_jumps_vanish(GT, O, cells, order, perturb, simplex):
     D = len(cells)
-    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb, simplex=simplex)
     sk = Skeleton(p, order, p.cell_dofs())
     n = O.Problem.q(p, O.Q_NFREE)
     S = sk.matrix(2 * order, 4.0, shape=(n, n)).toarray()
@@ -124,10 +128,11 @@ def test_continuous_jumps_vanish(GT, O, cells, order, perturb):
     assert np.abs(S).max() <= 1e-13 * block
 
 
-@pytest.mark.parametrize("cells,order,perturb", [((3, 3), 2, 0.2), ((2, 2, 2), 1, 0.1)])
-def test_symmetry(GT, O, cells, order, perturb):
+@pytest.mark.parametrize("cells,order,perturb,simplex", [
+    ((3, 3), 2, 0.2, False), ((2, 2, 2), 1, 0.1, False), ((3, 3), 2, 0.2, True)])
+def test_symmetry(GT, O, cells, order, perturb, simplex):
     D = len(cells)
-    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb, simplex=simplex)
     dg = DGSpace(p, order)
     S = Skeleton(p, order, dg.cd).matrix(2 * order, 2.0, shape=(dg.nfree, dg.nfree)).toarray()
     assert np.abs(S - S.T).max() <= 1e-13 * np.abs(S).max()
@@ -207,3 +212,27 @@ def dg_patch(GT, O, cells, order, perturb, kind, dirichlet_method, degree):
 def test_dg_patch_exact(GT, O, cells, order, perturb, kind, method, degree):
     x, ue = dg_patch(GT, O, cells, order, perturb, kind, method, degree)
     assert np.abs(x - ue).max() < 1e-10 * max(1.0, np.abs(ue).max())
+
+
+@pytest.mark.parametrize("cells,order,perturb", [((4, 3), 1, 0.2), ((3, 3), 2, 0.2), ((2, 2, 2), 1, 0.1)])
+def test_dg_patch_simplex_strong(GT, O, cells, order, perturb):
+    """Interior penalty on triangles and tetrahedra with strong Dirichlet
+    conditions on the whole boundary: a linear u is reproduced exactly."""
+    D = len(cells)
+    _, _, p = problem(GT, O, cells, order, [0.0, 1.0] * D, perturb, simplex=True)
+    u = lambda y: 1.0 + 2.0 * y[0] - y[1] + (0.5 * y[2] if D == 3 else 0.0)
+    on_bnd = lambda y: bool(np.any(np.abs(y) < 1e-12) or np.any(np.abs(y - 1.0) < 1e-12))
+    degree = 2 * order
+    dg = DGSpace(p, order, dirichlet=on_bnd)
+    n = dg.nfree
+    sk = Skeleton(p, order, dg.cd)
+    A = dg.laplace(degree) + sk.matrix(degree, 10.0, shape=(n, n))
+    Ad = dg.laplace(degree, cols=2) + sk.matrix(degree, 10.0, cols=2, shape=(n, dg.ndiri))
+    ue = np.zeros(dg.nfree + dg.ndiri)
+    pos = dg.geo.cell_pos
+    for c in range(dg.cd.shape[0]):
+        for a in range(dg.nd):
+            d = dg.cd[c, a]
+            ue[d - 1 if d > 0 else n - d - 1] = u(pos[c, a])
+    x = sla.spsolve(A.tocsc(), -(Ad @ ue[n:]))
+    assert np.abs(x - ue[:n]).max() < 1e-10 * np.abs(ue).max()
