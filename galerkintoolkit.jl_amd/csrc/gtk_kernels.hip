@@ -1844,6 +1844,12 @@ using HexFace1 = Elem<3, 8, 4, 4, 1, 6>;
 using HexFace2 = Elem<3, 8, 9, 9, 1, 6>;
 using TetFace2v = Elem<3, 4, 6, 9, 3, 4>;
 using QuadEdge2d2 = Elem<2, 4, 3, 2, 1, 4>;
+// scalar boundary items of simplices: edges of P1/P2 triangles, faces of
+// P1/P2 tetrahedra (integration degree 2k)
+using TriEdge1 = Elem<2, 3, 2, 2, 1, 3>;
+using TriEdge2 = Elem<2, 3, 3, 3, 1, 3>;
+using TetFace1 = Elem<3, 4, 3, 4, 1, 4>;
+using TetFace2 = Elem<3, 4, 6, 9, 1, 4>;
 // Nitsche items: boundary faces with all DOFs of their cell, Gauss rules of
 // k+1 points per direction on the face
 using QuadNitsche1 = Elem<2, 4, 4, 2, 1, 4>;
@@ -1851,6 +1857,10 @@ using QuadNitsche2 = Elem<2, 4, 9, 3, 1, 4>;
 using QuadNitsche3 = Elem<2, 4, 16, 4, 1, 4>;
 using HexNitsche1 = Elem<3, 8, 8, 4, 1, 6>;
 using HexNitsche2 = Elem<3, 8, 27, 9, 1, 6>;
+using TriNitsche1 = Elem<2, 3, 3, 2, 1, 3>;
+using TriNitsche2 = Elem<2, 3, 6, 3, 1, 3>;
+using TetNitsche1 = Elem<3, 4, 4, 4, 1, 4>;
+using TetNitsche2 = Elem<3, 4, 10, 9, 1, 4>;
 // skeleton items: interior faces with both cells (their nodes and DOFs),
 // Gauss rules of k+1 points per direction on the face, and for Q2
 // quadrilaterals also 2 (integration degree 2)
@@ -1899,7 +1909,15 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<TriSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TriSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TetSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
-         try_matrix<TetSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out);
+         try_matrix<TetSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriEdge1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriEdge2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetFace1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetFace2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriNitsche1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetNitsche1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetNitsche2>(key, form, precision, off_bytes, allow_mfma, waves, out);
 }
 
 int batched_type(const ElemKey &key) {
@@ -1950,7 +1968,9 @@ bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {
          try_vector<HexFace2>(key, fkind, out) || try_vector<TetFace2v>(key, fkind, out) ||
          try_vector<QuadQ2d2>(key, fkind, out) || try_vector<QuadEdge2d2>(key, fkind, out) ||
          try_vector<TriP1>(key, fkind, out) || try_vector<TriP2>(key, fkind, out) ||
-         try_vector<TetP1>(key, fkind, out) || try_vector<TetP2>(key, fkind, out);
+         try_vector<TetP1>(key, fkind, out) || try_vector<TetP2>(key, fkind, out) ||
+         try_vector<TriEdge1>(key, fkind, out) || try_vector<TriEdge2>(key, fkind, out) ||
+         try_vector<TetFace1>(key, fkind, out) || try_vector<TetFace2>(key, fkind, out);
 }
 
 }  // namespace gtk

@@ -28,6 +28,11 @@ CASES = [
     ((3, 2, 3), 1, False, 1, 0, None, 0.2),
     ((2, 3, 2), 2, False, 1, 1, "2-face-5", 0.2),
     ((2, 2, 2), 2, True, 3, 0, "2-face-2", 0.0),
+    # scalar P1/P2 triangles and tetrahedra
+    ((5, 4), 1, True, 1, 1, None, 0.2),
+    ((4, 3), 2, True, 1, 0, "1-face-2", 0.1),
+    ((2, 3, 2), 1, True, 1, 1, None, 0.1),
+    ((2, 2, 2), 2, True, 1, 0, "2-face-5", 0.0),
 ]
 SELECT = {None: None, "1-face-2": lambda x: abs(x[1] - 1.0) < 1e-12, "2-face-5": lambda x: abs(x[0]) < 1e-12,
           "2-face-2": lambda x: abs(x[2] - 1.0) < 1e-12}

@@ -114,3 +114,27 @@ def test_patch_solve(GT, O, precision):
     b = GT.assemble_vector(GT.NitscheLoad(dG, GT.DiscreteField(V, ue), gamma), float, V)
     x, st = GT.solve(A, b, precision=prec, rtol=1e-12)
     assert st.converged and np.abs(x - ue).max() < 1e-9
+
+
+@pytest.mark.parametrize("cells,order,perturb", [((5, 4), 1, 0.2), ((4, 3), 2, 0.15), ((2, 3, 2), 1, 0.1),
+                                                 ((2, 2, 2), 2, 0.0)])
+def test_nitsche_matrix_simplex(GT, O, cells, order, perturb):
+    """The Nitsche terms on P1/P2 triangles and tetrahedra over the whole
+    boundary, FP64 and mixed, against the oracle."""
+    print(f"case nitsche simplex {cells} P{order}", flush=True)
+    D = len(cells)
+    m = GT.cartesian_mesh(tuple([0.0, 1.0] * D), cells, simplexify=True)
+    if perturb:
+        m.coords = GT.perturb_interior_nodes(m, perturb, 13)
+    V = GT.lagrange_space(GT.interior(m), order)
+    p = O.Problem(list(cells), simplexify=True).space(order, dirichlet=0)
+    p.set_coords(m.coords)
+    p.quadrature(2 * order)
+    nf = NitscheFaces(p, order)
+    degree = 2 * order
+    dG = GT.measure(GT.boundary(m), degree)
+    ref = nf.matrix(degree, 3.5, shape=(V.nfree, V.nfree))
+    for prec, tol in ((GT.FP64, TOL64), (GT.MIXED, TOLMX)):
+        A = GT.assemble_matrix(GT.Nitsche(dG, 3.5), float, V, V, precision=prec)
+        assert np.array_equal(A.colptr - 1, ref.indptr) and np.array_equal(A.rowval - 1, ref.indices)
+        assert relfro(A.nzval, ref.data) <= tol, prec
