@@ -20,9 +20,11 @@ from .assembly import (FP64, FREE, Laplace, Mass, SparseMatrixCSC, _Assembler, _
 
 # element types of the batched kernels: scalar quadrilaterals of order 1, 2,
 # 3 with (order+1)^2 Gauss points, scalar hexahedra of order 1, 2 with
-# (order+1)^3 (a batch holds one cell kind)
-ELEM_TYPES = {(R.QUAD, 1): 0, (R.QUAD, 2): 1, (R.QUAD, 3): 2, (R.HEX, 1): 3, (R.HEX, 2): 4}
-ELEM_NAMES = {0: "quad-q1", 1: "quad-q2", 2: "quad-q3", 3: "hex-q1", 4: "hex-q2"}
+# (order+1)^3, scalar triangles of order 1, 2 with degree-2k rules
+# ((order+1)^2 points as well; a batch holds one cell kind)
+ELEM_TYPES = {(R.QUAD, 1): 0, (R.QUAD, 2): 1, (R.QUAD, 3): 2, (R.HEX, 1): 3, (R.HEX, 2): 4, (R.TRI, 1): 6,
+              (R.TRI, 2): 7}
+ELEM_NAMES = {0: "quad-q1", 1: "quad-q2", 2: "quad-q3", 3: "hex-q1", 4: "hex-q2", 6: "tri-p1", 7: "tri-p2"}
 
 # tile sizes the autotuner tries (cells per tile; 0 = as many as fit in LDS)
 TILE_CANDIDATES = (0, 256, 160, 96, 64, 40)
@@ -34,8 +36,8 @@ def elem_type(space, dOmega=None):
     """Element type of the batched kernels for a space, or raise."""
     key = (space.mesh.cellkind, space.order)
     if space.ncomp != 1 or key not in ELEM_TYPES:
-        raise NotImplementedError("batched problems are scalar Q1/Q2/Q3 Lagrange spaces on quadrilaterals or "
-                                  "Q1/Q2 on hexahedra")
+        raise NotImplementedError("batched problems are scalar Q1/Q2/Q3 Lagrange spaces on quadrilaterals, "
+                                  "Q1/Q2 on hexahedra or P1/P2 on triangles")
     if dOmega is not None and len(dOmega.w) != (space.order + 1) ** space.mesh.D:
         raise NotImplementedError("batched problems use (order+1)^D Gauss points: GT.measure(Omega, 2*order)")
     return ELEM_TYPES[key]
@@ -219,7 +221,7 @@ class MatrixBatch:
         self.forms = list(forms)
         self.spaces = list(spaces)
         if len({V.mesh.cellkind for V in spaces}) != 1:
-            raise NotImplementedError("a batch holds quadrilaterals or hexahedra, not both")
+            raise NotImplementedError("a batch holds one cell kind")
         self.cellkind = spaces[0].mesh.cellkind
         self.problems = []
         for form, V in zip(forms, spaces):

@@ -1727,10 +1727,10 @@ int gtk_batch_add(gtk_batch *b, gtk_ctx *ctx) {
   if (!ctx->has_ref) BFAIL(b, GTK_ESTATE, "gtk_set_reference first");
   const int type = ctx->ncomp == 1 ? gtk::batched_type(elem_key(ctx)) : -1;
   if (type < 0)
-    BFAIL(b, GTK_EUNSUP, "batched problems are scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2 points or Q1/Q2 "
-                         "hexahedra with (k+1)^3");
+    BFAIL(b, GTK_EUNSUP, "batched problems are scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2 points, Q1/Q2 "
+                         "hexahedra with (k+1)^3 or P1/P2 triangles with degree-2k rules");
   if (b->family >= 0 && type / 3 != b->family)
-    BFAIL(b, GTK_EUNSUP, "a batch holds quadrilaterals or hexahedra, not both");
+    BFAIL(b, GTK_EUNSUP, "a batch holds one cell kind");
   if (ctx->ncoef > 1) BFAIL(b, GTK_EINVAL, "scalar forms take one coefficient per cell");
   batch_release_graph(b);
   b->family = type / 3;
@@ -1775,7 +1775,7 @@ int gtk_batch_get_vector(gtk_batch *b, int problem, double *out) {
 int gtk_batch_set_tile_cells(gtk_batch *b, int elem_type, int max_cells) {
   if (!b) return GTK_EINVAL;
   if (elem_type < 0 || elem_type >= gtk::kBatchedTypes || max_cells < 0)
-    BFAIL(b, GTK_EINVAL, "elem_type 0..4, max_cells >= 0");
+    BFAIL(b, GTK_EINVAL, "elem_type 0..7, max_cells >= 0");
   b->tile_cells[elem_type] = max_cells;
   batch_release_graph(b);
   return GTK_OK;

@@ -108,10 +108,11 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
 bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);
 // Batched kernels: family 0 over scalar Q1/Q2/Q3 quadrilaterals with
 // (k+1)^2 Gauss points (types 0, 1, 2), family 1 over scalar Q1/Q2
-// hexahedra with (k+1)^3 (types 3, 4); batched_type gives a problem's type
-// (-1: none), type / 3 its family and type % 3 its slot in the family's
-// kernel; 8 or 12 waves per workgroup.
-constexpr int kBatchedTypes = 5;
+// hexahedra with (k+1)^3 (types 3, 4), family 2 over scalar P1/P2
+// triangles with degree-2k rules (types 6, 7); batched_type gives a
+// problem's type (-1: none), type / 3 its family and type % 3 its slot in
+// the family's kernel; 8 or 12 waves per workgroup.
+constexpr int kBatchedTypes = 8;
 int batched_type(const ElemKey &key);
 bool find_batched_kernel(int form, int precision, int waves, int family, KernelInfo &out);
 

@@ -1926,6 +1926,8 @@ int batched_type(const ElemKey &key) {
   if (match<QuadQ3>(key)) return 2;
   if (match<HexQ1>(key)) return 3;
   if (match<HexQ2>(key)) return 4;
+  if (match<TriP1>(key)) return 6;
+  if (match<TriP2>(key)) return 7;
   return -1;
 }
 
@@ -1934,8 +1936,10 @@ static KernelInfo batched_info() {
   KernelInfo k;
   if constexpr (FAMILY == 0)
     k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, QuadQ1, QuadQ2, QuadQ3>);
-  else  // the third slot is unused
+  else if constexpr (FAMILY == 1)  // the third slot is unused
     k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, HexQ1, HexQ2, HexQ2>);
+  else
+    k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, TriP1, TriP2, TriP2>);
   k.mfma = true;
   k.block = NW * 64;
   return k;
@@ -1948,14 +1952,18 @@ static KernelInfo batched_info(int precision, int waves) {
   return precision == 0 ? batched_info<FORM, double, 8, FAMILY>() : batched_info<FORM, float, 8, FAMILY>();
 }
 
+template <int FORM>
+static KernelInfo batched_family_info(int precision, int waves, int family) {
+  if (family == 0) return batched_info<FORM, 0>(precision, waves);
+  if (family == 1) return batched_info<FORM, 1>(precision, waves);
+  return batched_info<FORM, 2>(precision, waves);
+}
+
 bool find_batched_kernel(int form, int precision, int waves, int family, KernelInfo &out) {
-  if ((waves != 8 && waves != 12) || family < 0 || family > 1) return false;
-  if (form == kLaplace)
-    out = family == 0 ? batched_info<kLaplace, 0>(precision, waves) : batched_info<kLaplace, 1>(precision, waves);
-  else if (form == kMass)
-    out = family == 0 ? batched_info<kMass, 0>(precision, waves) : batched_info<kMass, 1>(precision, waves);
-  else
-    return false;
+  if ((waves != 8 && waves != 12) || family < 0 || family > 2) return false;
+  if (form == kLaplace) out = batched_family_info<kLaplace>(precision, waves, family);
+  else if (form == kMass) out = batched_family_info<kMass>(precision, waves, family);
+  else return false;
   return true;
 }
 

@@ -324,8 +324,9 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
  * captured once in a HIP graph.  A problem is a context on the batch's
  * device with a (GTK_FREE, GTK_FREE) pattern built and its reference tables
  * and coefficients set, on scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2
- * Gauss points or (another batch) scalar Q1/Q2 hexahedra with (k+1)^3;
- * form GTK_LAPLACE or GTK_MASS.  Each problem's values land
+ * Gauss points, or (another batch) scalar Q1/Q2 hexahedra with (k+1)^3,
+ * or scalar P1/P2 triangles with degree-2k rules; form GTK_LAPLACE or
+ * GTK_MASS.  Each problem's values land
  * in its own context (gtk_get_values, gtk_values_device_ptr).  Contexts
  * must outlive the batch and have no *_async work pending at a run; new
  * coefficients or coordinates set between runs are picked up, a new mesh,
@@ -333,7 +334,8 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
  * No counterpart in the reference, which assembles one problem per
  * GT.assemble_matrix / GT.assemble_vector call (src/problems.jl:244-274,
  * :319-350); a batch replaces a loop of such calls.  elem_type of
- * gtk_batch_set_tile_cells: 0, 1, 2 quadrilateral Q1, Q2, Q3; 3, 4 hexahedral Q1, Q2;
+ * gtk_batch_set_tile_cells: 0, 1, 2 quadrilateral Q1, Q2, Q3; 3, 4 hexahedral Q1, Q2; 6, 7
+ * triangular P1, P2;
  * max_cells caps the cells of a tile for that type (0: as many as fit in
  * LDS), the knob the autotuner sets. */
 typedef struct gtk_batch gtk_batch;

@@ -3,8 +3,8 @@ distortion, coefficient fields) assembled by one persistent launch captured
 in a HIP graph, each problem's matrix checked against the oracle (pattern
 bit-exact, FP64 values within relative Frobenius error 1e-13, mixed within
 1e-6) and against its own single-problem assembly; the same for batches of
-uneven 3D problems (Q1/Q2 hexahedra), whose kernel is a second
-instantiation of the batched kernel."""
+uneven 3D problems (Q1/Q2 hexahedra) and of P1/P2 triangles, each cell kind
+an instantiation of the batched kernel of its own."""
 import numpy as np
 import pytest
 
@@ -254,3 +254,45 @@ def test_batch_3d_waves_and_kinds(GT, O):
     q2 = make_problems(GT, O, 1, seed=3, with_oracle=False)[0]
     with pytest.raises(NotImplementedError):
         GT.MatrixBatch(forms[:1] + forms_of(GT, [q2], "laplace"), spaces[:1] + [q2["V"]], tuning=None)
+
+
+def make_problems_tri(GT, O, count, seed, nmin=2, nmax=12):
+    u = GT.inputs.uniform(seed, 4 * count)
+    out = []
+    for i in range(count):
+        cells = (nmin + int(u[4 * i] * (nmax - nmin + 1)), nmin + int(u[4 * i + 1] * (nmax - nmin + 1)))
+        order = 1 + int(u[4 * i + 2] * 2)
+        m = GT.cartesian_mesh((0.0, 1.0) * 2, cells, simplexify=True)
+        if u[4 * i + 3] < 0.5:
+            m.coords = GT.perturb_interior_nodes(m, 0.15, seed + i)
+        Om = GT.interior(m)
+        V = GT.lagrange_space(Om, order, dirichlet_boundary=GT.boundary(m))
+        dO = GT.measure(Om, 2 * order)
+        kappa = GT.inputs.lognormal(seed + 1000 + i, m.cells.shape[0])
+        p = O.Problem(list(cells), simplexify=True).space(order, dirichlet=1)
+        p.set_coords(m.coords)
+        p.quadrature(2 * order)
+        out.append(dict(m=m, Om=Om, V=V, dO=dO, kappa=kappa, p=p, order=order))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_triangles_match_oracle(GT, O, precision):
+    """A batch of uneven P1/P2 triangle problems (the batched kernel's
+    triangle instantiation), Laplace with lognormal kappa and loads."""
+    print(f"case batch triangles {precision}", flush=True)
+    probs = make_problems_tri(GT, O, 8, seed=29)
+    assert {q["order"] for q in probs} == {1, 2}
+    prec = GT.FP64 if precision == "fp64" else GT.MIXED
+    tol = TOL_FP64 if precision == "fp64" else TOL_MIXED
+    forms = forms_of(GT, probs, "laplace")
+    loads, lref = loads_of(GT, probs, 500)
+    b = GT.MatrixBatch(forms, [q["V"] for q in probs], precision=prec, tuning=None, loads=loads)
+    b.run()
+    for q, A in zip(probs, b.matrices()):
+        cp, rv, nz = q["p"].assemble_matrix(O.LAPLACE, coef=q["kappa"])
+        assert np.array_equal(A.colptr, cp) and np.array_equal(A.rowval, rv)
+        assert relfro(A.nzval, nz) <= tol, (q["order"], q["m"].cells.shape)
+    for q, v, (fk, fd) in zip(probs, b.vectors(), lref):
+        assert relfro(v, q["p"].assemble_vector(fk, fd)) <= TOL_FP64, (q["order"], fk)
+    b.close()
