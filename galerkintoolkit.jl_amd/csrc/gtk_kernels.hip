@@ -1835,6 +1835,14 @@ using TriP1 = Elem<2, 3, 3, 4, 1>;
 using TriP2 = Elem<2, 3, 6, 9, 1>;
 using TetP1 = Elem<3, 4, 4, 8, 1>;
 using TetP2 = Elem<3, 4, 10, 11, 1>;
+// vector (elasticity) fields: Q1/Q2 quadrilaterals, Q1 hexahedra, P1/P2
+// triangles, P1 tetrahedra (at most 64 DOFs per cell), degree-2k rules
+using QuadQ1v = Elem<2, 4, 4, 4, 2>;
+using QuadQ2v = Elem<2, 4, 9, 9, 2>;
+using HexQ1v = Elem<3, 8, 8, 8, 3>;
+using TriP1v = Elem<2, 3, 3, 4, 2>;
+using TriP2v = Elem<2, 3, 6, 9, 2>;
+using TetP1v = Elem<3, 4, 4, 8, 3>;
 // boundary items: edges of Q1..Q3 quadrilaterals (k+1 Gauss points), faces
 // of Q1/Q2 hexahedra ((k+1)^2), faces of P2 vector tetrahedra (9-point rule)
 using QuadEdge1 = Elem<2, 4, 2, 2, 1, 4>;
@@ -1902,6 +1910,12 @@ bool find_matrix_kernel(const ElemKey &key, int form, int precision, int off_byt
          try_matrix<QuadSkeleton3>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexSkeleton1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<HexSkeleton2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadQ1v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<QuadQ2v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<HexQ1v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriP1v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TriP2v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
+         try_matrix<TetP1v>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TriP1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TriP2>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
          try_matrix<TetP1>(key, form, precision, off_bytes, allow_mfma, waves, out) ||
@@ -1976,6 +1990,9 @@ bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out) {
          try_vector<HexFace2>(key, fkind, out) || try_vector<TetFace2v>(key, fkind, out) ||
          try_vector<QuadQ2d2>(key, fkind, out) || try_vector<QuadEdge2d2>(key, fkind, out) ||
          try_vector<TriP1>(key, fkind, out) || try_vector<TriP2>(key, fkind, out) ||
+         try_vector<QuadQ1v>(key, fkind, out) || try_vector<QuadQ2v>(key, fkind, out) ||
+         try_vector<HexQ1v>(key, fkind, out) || try_vector<TriP1v>(key, fkind, out) ||
+         try_vector<TriP2v>(key, fkind, out) || try_vector<TetP1v>(key, fkind, out) ||
          try_vector<TetP1>(key, fkind, out) || try_vector<TetP2>(key, fkind, out) ||
          try_vector<TriEdge1>(key, fkind, out) || try_vector<TriEdge2>(key, fkind, out) ||
          try_vector<TetFace1>(key, fkind, out) || try_vector<TetFace2>(key, fkind, out);
