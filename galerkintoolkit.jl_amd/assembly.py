@@ -13,6 +13,7 @@ form object.  The result is the reference's SparseMatrixCSC{Float64,Int32}
 layout (1-based colptr/rowval) and a Float64 vector.
 """
 import dataclasses
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -553,8 +554,20 @@ class VectorCache:
         self.form, self.space, self.rows, self.device = form, space, rows, device
 
 
-def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic="host"):
-    """(assembler of the block's device values, nzval, stats, pattern)."""
+def _launch(a, code, prec, rows, cols, nnz, device_only):
+    """The block's values assembled on the device: (nzval, stats), or
+    (None, None) with device_only (no copy to host memory)."""
+    if device_only:
+        a.ctx.assemble_matrix_async(code, prec, rows, cols)
+        a.ctx.sync()
+        a.values_owner.pop((rows, cols), None)
+        return None, None
+    return a.ctx.assemble_matrix(code, prec, rows, cols, nnz)
+
+
+def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic="host", device_only=False):
+    """(assembler of the block's device values, nzval, stats, pattern);
+    device_only: the values stay on the device (nzval and stats None)."""
     if isinstance(form, FormSum):
         return _assemble_matrix_sum(form, space, rows, cols, precision, device, symbolic)
     if form.dOmega.on_boundary and not isinstance(form, (Mass, Nitsche)):
@@ -566,7 +579,7 @@ def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic
         pat = a.pattern(rows, cols, symbolic)
         a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
         prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
-        nzval, stats = a.ctx.assemble_matrix(L.GTK_SKELETON, prec, rows, cols, pat[0])
+        nzval, stats = _launch(a, L.GTK_SKELETON, prec, rows, cols, pat[0], device_only)
         return a, nzval, stats, pat
     if isinstance(form, Nitsche):
         if not form.dOmega.on_boundary:
@@ -575,14 +588,14 @@ def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic
         pat = a.pattern(rows, cols, symbolic)
         a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
         prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
-        nzval, stats = a.ctx.assemble_matrix(L.GTK_NITSCHE, prec, rows, cols, pat[0])
+        nzval, stats = _launch(a, L.GTK_NITSCHE, prec, rows, cols, pat[0], device_only)
         return a, nzval, stats, pat
     a = _assembler(space, form.dOmega, device)
     pat = a.pattern(rows, cols, symbolic)
     coef, ncoef = _coefficients(form, _ncoef_items(form, space))
     a.ctx.set_cell_coefficients(coef, ncoef)
     prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
-    nzval, stats = a.ctx.assemble_matrix(_form_code(form), prec, rows, cols, pat[0])
+    nzval, stats = _launch(a, _form_code(form), prec, rows, cols, pat[0], device_only)
     return a, nzval, stats, pat
 
 
@@ -669,13 +682,19 @@ def _assemble_matrix_sum(fs, space, rows, cols, precision, device, symbolic):
         raise NotImplementedError("a sum of matrix integrals: one over the cells, boundary integrals and at most "
                                   "one skeleton integral")
     base = skel[0] if skel else vol[0]
-    a_base, _, stats, pat = _assemble_matrix_values(base, space, rows, cols, precision, device, symbolic)
+    # every block stays on the device until the sum is copied out once
+    a_base, _, _, pat = _assemble_matrix_values(base, space, rows, cols, precision, device, symbolic,
+                                                device_only=True)
     for t in fs.terms:
         if t is base:
             continue
-        a_t, _, _, _ = _assemble_matrix_values(t, space, rows, cols, precision, device)
+        a_t, _, _, _ = _assemble_matrix_values(t, space, rows, cols, precision, device, device_only=True)
         a_base.ctx.add_block(rows, cols, a_t.ctx)
+    t0 = time.perf_counter()
     nzval = a_base.ctx.get_values(rows, cols, pat[0])
+    stats = L.GtkStats()
+    stats.nnz = pat[0]
+    stats.total_ms = (time.perf_counter() - t0) * 1e3
     return a_base, nzval, stats, pat
 
 
