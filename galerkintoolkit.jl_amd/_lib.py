@@ -21,7 +21,7 @@ GTK_NOALLOC_MAX_DOUBLES = 8 * 1024 * 1024  # scratch of gtk_assemble_vector_noal
 EXPORTS = [
     "gtk_create", "gtk_destroy", "gtk_last_error", "gtk_version", "gtk_set_mesh", "gtk_update_coordinates",
     "gtk_set_space", "gtk_set_reference", "gtk_set_reference_faces", "gtk_build_pattern", "gtk_build_pattern_device",
-    "gtk_local_nnz_map", "gtk_add_block",
+    "gtk_local_nnz_map", "gtk_add_block", "gtk_gather_block",
     "gtk_get_pattern", "gtk_set_cell_coefficients",
     "gtk_assemble_matrix", "gtk_get_values", "gtk_set_values", "gtk_values_device_ptr", "gtk_assemble_vector",
     "gtk_assemble_vector_noalloc",
@@ -88,6 +88,8 @@ def load():
     L.gtk_local_nnz_map.argtypes = [vp, i, i, i32]
     L.gtk_set_reference_faces.argtypes = [vp, i, i, f64, i, f64, f64, f64, i32]
     L.gtk_add_block.argtypes = [vp, i, i, vp]
+    L.gtk_gather_block.argtypes = [vp, i, i, vp, i, i, np.ctypeslib.ndpointer(np.int64, flags="C"),
+                                   np.ctypeslib.ndpointer(np.int64, flags="C")]
     L.gtk_set_cell_coefficients.argtypes = [vp, i, C.c_void_p]
     L.gtk_assemble_matrix.argtypes = [vp, i, i, i, i, C.c_void_p, C.POINTER(GtkStats)]
     L.gtk_get_values.argtypes = [vp, i, i, f64]
@@ -186,6 +188,15 @@ class Context:
 
     def add_block(self, rows, cols, src):
         self.check(self.L.gtk_add_block(self.h, rows, cols, src.h), "gtk_add_block")
+
+    def gather_block(self, rows, cols, src, src_rows, src_cols, ptr, idx):
+        """Values of block (rows, cols) := sums of src's values grouped by
+        ptr/idx (gtk_gather_block)."""
+        idx = np.ascontiguousarray(idx, np.int64)
+        if idx.size == 0:
+            idx = np.zeros(1, np.int64)
+        self.check(self.L.gtk_gather_block(self.h, rows, cols, src.h, src_rows, src_cols,
+                                           np.ascontiguousarray(ptr, np.int64), idx), "gtk_gather_block")
 
     def build_pattern(self, rows, cols):
         nnz, ncols = C.c_int64(), C.c_int64()

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib as L
 from . import refcell as R
-from .space import DIRICHLET, FREE
+from .space import DIRICHLET, FREE, LagrangeSpace
 
 FP64, MIXED = "fp64", "mixed"
 
@@ -393,9 +393,7 @@ class _SkeletonAssembler(_Assembler):
         if space.ncomp != 1:
             raise NotImplementedError("skeleton terms: scalar fields")
         if space.continuous:
-            # an item would list each DOF of the face twice, and a continuous
-            # basis function has no jump, so every term is rounding
-            raise NotImplementedError("skeleton terms on discontinuous spaces (lagrange_space(...; continuous=False))")
+            raise ValueError("a continuous space's skeleton terms go through _ContinuousSkeletonAssembler")
 
         self.space = space
         self.dOmega = dL
@@ -455,6 +453,88 @@ class _SkeletonAssembler(_Assembler):
         self.h = _diameters(xyz, self.fnodes)
 
 
+class _ContinuousSkeletonAssembler(_Assembler):
+    """Interior-penalty terms of a continuous space: the face items would
+    list each DOF of the face twice (the column-owner plan keeps one row per
+    lane), so the terms are assembled on the cells' own copies of their DOFs
+    (the discontinuous space of the same order, every DOF free) and each
+    nonzero of the continuous block is the sum, in order, of the copies'
+    entries that land on it (gtk_gather_block), as the reference's COO
+    compression sums its duplicates.  The pattern is that of the face items
+    over the continuous DOFs (both cells' DOFs coupled)."""
+
+    def __init__(self, space, dL, device=0):
+        mesh = space.mesh
+        if space.ncomp != 1:
+            raise NotImplementedError("skeleton terms: scalar fields")
+        self.space = space
+        self.dOmega = dL
+        self.dg_space = LagrangeSpace(space.domain, space.order, continuous=False)
+        self.dg = _SkeletonAssembler(self.dg_space, dL, device)
+        it = skeleton_items(space, dL)
+        cells = it["cells"]
+        nit = len(cells)
+        self.ctx = L.Context(device)
+        self.ctx.set_mesh(mesh.D, mesh.coords, mesh.cell_nodes_1b()[cells].reshape(nit, -1))
+        self.ctx.set_space(1, space.cell_dofs[cells].reshape(nit, -1), space.nfree, space.ndiri)
+        self.cont = space.cell_dofs.reshape(-1).astype(np.int64)  # continuous id of each copy (copy id - 1)
+        self.items = nit
+        self.patterns = {}
+        self.values_owner = {}
+        self.maps = {}
+
+    @property
+    def h(self):
+        return self.dg.h
+
+    def beta(self, gamma):
+        return self.dg.beta(gamma)
+
+    def set_coordinates(self, xyz):
+        self.ctx.update_coordinates(xyz)
+        self.dg.set_coordinates(xyz)
+
+    def gather_map(self, rows, cols):
+        """(ptr, idx): the copies' entries summed into each nonzero of the
+        (rows, cols) block, in ascending order of their position."""
+        key = (rows, cols)
+        if key not in self.maps:
+            nnz_d, ncols_d, _, cp_d, rv_d = self.dg.pattern(FREE, FREE)
+            nnz, _, nrows, cp, rv = self.pattern(rows, cols)
+            col_d = np.repeat(np.arange(ncols_d, dtype=np.int64), np.diff(cp_d))
+            I = self.cont[rv_d.astype(np.int64) - 1]
+            J = self.cont[col_d]
+            keep = (I > 0 if rows == FREE else I < 0) & (J > 0 if cols == FREE else J < 0)
+            src = np.nonzero(keep)[0]
+            r, c = np.abs(I[src]), np.abs(J[src])
+            tcol = np.repeat(np.arange(len(cp) - 1, dtype=np.int64), np.diff(cp))
+            tkeys = tcol * (nrows + 1) + rv.astype(np.int64)
+            skeys = (c - 1) * (nrows + 1) + r
+            pos = np.searchsorted(tkeys, skeys)
+            if np.any(pos >= nnz) or np.any(tkeys[np.minimum(pos, nnz - 1)] != skeys):
+                raise RuntimeError("a copy's entry falls outside the continuous skeleton pattern")
+            order = np.argsort(pos, kind="stable")
+            ptr = np.zeros(nnz + 1, np.int64)
+            ptr[1:] = np.cumsum(np.bincount(pos, minlength=nnz))
+            self.maps[key] = (ptr, src[order].astype(np.int64))
+        return self.maps[key]
+
+    def assemble(self, gamma, prec, rows, cols, nnz, device_only):
+        self.dg.pattern(FREE, FREE)
+        self.dg.ctx.set_cell_coefficients(self.dg.beta(gamma), 1)
+        self.dg.ctx.assemble_matrix_async(L.GTK_SKELETON, prec, FREE, FREE)
+        self.dg.ctx.sync()
+        self.dg.values_owner.pop((FREE, FREE), None)
+        ptr, idx = self.gather_map(rows, cols)
+        self.ctx.gather_block(rows, cols, self.dg.ctx, FREE, FREE, ptr, idx)
+        self.values_owner.pop((rows, cols), None)
+        if device_only:
+            return None, None
+        stats = L.GtkStats()
+        stats.nnz = nnz
+        return self.ctx.get_values(rows, cols, nnz), stats
+
+
 def _assembler(space, dOmega, device, kind=None):
     """The context of (space, measure); kind "nitsche" (or "nitsche_load",
     a second one so the load's matrix never replaces the Nitsche matrix's
@@ -464,7 +544,7 @@ def _assembler(space, dOmega, device, kind=None):
     a = cache.get(key)
     if a is None or a.dOmega is not dOmega:
         if dOmega.on_skeleton:
-            a = _SkeletonAssembler(space, dOmega, device)
+            a = (_ContinuousSkeletonAssembler if space.continuous else _SkeletonAssembler)(space, dOmega, device)
         elif kind is not None:
             a = _NitscheAssembler(space, dOmega, device)
         else:
@@ -577,8 +657,11 @@ def _assemble_matrix_values(form, space, rows, cols, precision, device, symbolic
     if isinstance(form, InteriorPenalty):
         a = _assembler(space, form.dOmega, device)
         pat = a.pattern(rows, cols, symbolic)
-        a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
         prec = L.GTK_FP64 if precision == FP64 else L.GTK_MIXED
+        if isinstance(a, _ContinuousSkeletonAssembler):
+            nzval, stats = a.assemble(form.gamma, prec, rows, cols, pat[0], device_only)
+            return a, nzval, stats, pat
+        a.ctx.set_cell_coefficients(a.beta(form.gamma), 1)
         nzval, stats = _launch(a, L.GTK_SKELETON, prec, rows, cols, pat[0], device_only)
         return a, nzval, stats, pat
     if isinstance(form, Nitsche):

@@ -747,6 +747,32 @@ int gtk_add_block(gtk_ctx *dst, int rows, int cols, gtk_ctx *src) {
   return GTK_OK;
 }
 
+int gtk_gather_block(gtk_ctx *dst, int rows, int cols, gtk_ctx *src, int src_rows, int src_cols, const int64_t *ptr,
+                     const int64_t *idx) {
+  if (!dst || !src || !ptr) return GTK_EINVAL;
+  if (dst->device != src->device) FAIL(dst, GTK_EINVAL, "the contexts are on different devices");
+  auto id = dst->mplans.find(key_of(rows, cols));
+  auto is = src->mplans.find(key_of(src_rows, src_cols));
+  if (id == dst->mplans.end() || is == src->mplans.end()) FAIL(dst, GTK_ESTATE, "both blocks must be built");
+  MatrixPlan *D = id->second.get(), *S = is->second.get();
+  const int64_t n = D->pat.nnz, m = ptr[n];
+  if (ptr[0] != 0 || m < 0 || (m > 0 && !idx)) FAIL(dst, GTK_EINVAL, "invalid grouping");
+  for (int64_t t = 0; t < n; t++)
+    if (ptr[t + 1] < ptr[t]) FAIL(dst, GTK_EINVAL, "grouping pointers must not decrease");
+  for (int64_t k = 0; k < m; k++)
+    if (idx[k] < 0 || idx[k] >= S->pat.nnz) FAIL(dst, GTK_EINVAL, "source index out of range");
+  HIPCHK(dst, hipSetDevice(dst->device));
+  HIPCHK(dst, hipStreamSynchronize(src->stream));  // the source values are complete
+  DevBuf dptr, didx;
+  int rc;
+  if ((rc = upload(dst, dptr, ptr, (size_t)n + 1))) return rc;
+  if (m && (rc = upload(dst, didx, idx, (size_t)m))) return rc;
+  gtk::launch_gather_block(n, dptr.as<int64_t>(), didx.as<int64_t>(), S->nzval.as<double>(), D->nzval.as<double>(),
+                           dst->stream);
+  HIPCHK(dst, hipStreamSynchronize(dst->stream));
+  return GTK_OK;
+}
+
 int gtk_local_nnz_map(gtk_ctx *ctx, int rows, int cols, int32_t *map_out) {
   if (!ctx || !map_out) return GTK_EINVAL;
   auto it = ctx->mplans.find(key_of(rows, cols));

@@ -46,6 +46,9 @@ void launch_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, int row
 // dst values += src values over the src pattern (each src column's rows are
 // found in the same dst column by binary search); *missing = 1 if a src
 // entry has no dst entry.
+// dst[t] = the sum, in order, of src[idx[ptr[t]]] .. src[idx[ptr[t+1]-1]]
+void launch_gather_block(int64_t n, const int64_t *ptr, const int64_t *idx, const double *src, double *dst,
+                         hipStream_t st);
 void launch_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,
                       const int32_t *dcolptr, const int32_t *drowval, double *dval, int32_t *missing, hipStream_t st);
 

@@ -263,6 +263,15 @@ __global__ void k_add_block(int64_t ncols, const int32_t *scolptr, const int32_t
   }
 }
 
+// dst[t] = sum over k in [ptr[t], ptr[t+1]) of src[idx[k]], in k order
+__global__ void k_gather_block(int64_t n, const int64_t *ptr, const int64_t *idx, const double *src, double *dst) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t k = ptr[t]; k < ptr[t + 1]; k++) s += src[idx[k]];
+    dst[t] = s;
+  }
+}
+
 unsigned grid_for(int64_t n, int block) {
   const int64_t g = (n + block - 1) / block;
   return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
@@ -321,6 +330,12 @@ void launch_local_map(const int32_t *cell_dofs, int64_t n_cells, int nd, int row
   const int64_t total = n_cells * nd * nd;
   if (total == 0) return;
   k_local_map<<<grid_for(total, 256), 256, 0, st>>>(cell_dofs, n_cells, nd, rows, cols, colptr, rowval, map);
+}
+
+void launch_gather_block(int64_t n, const int64_t *ptr, const int64_t *idx, const double *src, double *dst,
+                         hipStream_t st) {
+  if (n == 0) return;
+  k_gather_block<<<grid_for(n, 256), 256, 0, st>>>(n, ptr, idx, src, dst);
 }
 
 void launch_add_block(int64_t ncols, const int32_t *scolptr, const int32_t *srowval, const double *sval,

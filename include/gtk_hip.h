@@ -190,6 +190,15 @@ int gtk_local_nnz_map(gtk_ctx *ctx, int rows, int cols, int32_t *map_out);
  * the reference compresses into one matrix (src/assembly.jl:545-575).
  * GTK_EINVAL if a src entry is outside dst's pattern. */
 int gtk_add_block(gtk_ctx *dst, int rows, int cols, gtk_ctx *src);
+/* dst's (rows, cols) values set from src's (src_rows, src_cols) values
+ * through a host-given grouping: value t of dst becomes the sum, in order, of
+ * src values idx[ptr[t]] .. idx[ptr[t+1] - 1] (ptr has nnz(dst) + 1
+ * entries, from 0).  The skeleton terms of a continuous space are assembled
+ * on the cells' own copies of their DOFs (a discontinuous space) and
+ * gathered onto the continuous DOFs this way: the sum over the copies that
+ * the reference's COO compression performs (src/assembly.jl:571-575). */
+int gtk_gather_block(gtk_ctx *dst, int rows, int cols, gtk_ctx *src, int src_rows, int src_cols, const int64_t *ptr,
+                     const int64_t *idx);
 
 /* Per-cell coefficients resident on the device: kappa (ncoef=1) or
  * (lambda, mu) (ncoef=2), n_cells*ncoef values, cell-major

@@ -7,7 +7,9 @@ through the C ABI:
   hexahedra Q1/Q2, perturbed): the pattern of the oracle's COO assembly bit
   for bit, FP64 values within 1e-13 and the mixed form within 1e-6
   (relative Frobenius, and entrywise relative to the largest face-block
-  entry); continuous spaces are refused (every jump vanishes);
+  entry); on continuous spaces (every jump vanishes) the pattern of the
+  oracle's COO assembly and values equal to rounding, and the reference's
+  continuous Poisson check with skeleton terms (assembly_tests.jl:318-345);
 - Laplace + skeleton + Nitsche as one matrix on a discontinuous space (the
   volume and boundary blocks added into the skeleton block on the device);
 - update_matrix with moved nodes (the face diameters follow) equals a fresh
@@ -85,12 +87,48 @@ def test_skeleton_matrix(GT, O, cells, order, perturb, simplex):
         assert np.linalg.norm(A.nzval - ref.data) <= tol * np.linalg.norm(ref.data), prec
 
 
-def test_continuous_space_refused(GT, O):
-    m, _ = setup(GT, O, (3, 3), 1, 0.0)
-    V = GT.lagrange_space(GT.interior(m), 1)
-    dL = GT.measure(GT.skeleton(m), 2)
-    with pytest.raises(NotImplementedError):
-        GT.assemble_matrix(GT.InteriorPenalty(dL, 1.0), float, V, V)
+@pytest.mark.parametrize("cells,order,simplex", [((3, 3), 1, False), ((3, 2), 2, False), ((2, 2, 2), 1, False),
+                                                  ((3, 3), 1, True)])
+def test_continuous_space_skeleton(GT, O, cells, order, simplex):
+    """Skeleton terms on a continuous space (the reference's own check,
+    test/assembly_tests.jl:318-345): the pattern couples both cells of every
+    interior face (bit-exact against the oracle's COO assembly) and the
+    values, sums of the cells' copies whose jumps cancel, agree with the
+    oracle's to rounding of the face blocks."""
+    print(f"case continuous skeleton {cells} order {order} simplex={simplex}", flush=True)
+    m, p = setup(GT, O, cells, order, 0.15, simplex=simplex)
+    V = GT.lagrange_space(GT.interior(m), order)
+    assert np.array_equal(V.cell_dofs, p.cell_dofs())
+    dL = GT.measure(GT.skeleton(m), 2 * order)
+    sk = Skeleton(p, order, p.cell_dofs())
+    ref = sk.matrix(2 * order, 2.0, shape=(V.nfree, V.nfree))
+    scale = max(np.abs(sk.face_blocks(f, 2 * order, 2.0)[1]).max() for f in sk.geo.interior)
+    for prec, tol in ((GT.FP64, TOL64), (GT.MIXED, TOLMX)):
+        A = GT.assemble_matrix(GT.InteriorPenalty(dL, 2.0), float, V, V, precision=prec)
+        assert np.array_equal(A.colptr - 1, ref.indptr) and np.array_equal(A.rowval - 1, ref.indices)
+        assert np.abs(A.nzval - ref.data).max() <= tol * scale, prec
+
+
+def test_reference_skeleton_poisson(GT, O):
+    """test/assembly_tests.jl:318-345: Q1 on (0,1)^2 in 4x4 cells, Laplace
+    plus the interior-penalty terms (gamma 1, degree 2), u = x + y on the
+    whole boundary, no source: the device-assembled system reproduces u."""
+    print("case reference skeleton poisson", flush=True)
+    m = GT.cartesian_mesh((0.0, 1.0, 0.0, 1.0), (4, 4))
+    Om = GT.interior(m)
+    V = GT.lagrange_space(Om, 1, dirichlet_boundary=GT.boundary(m))
+    dO, dL = GT.measure(Om, 2), GT.measure(GT.skeleton(m), 2)
+    e = np.indices([2, 2]).reshape(2, -1)[::-1].T.astype(float)
+    vals = np.zeros(V.nfree + V.ndiri)
+    for c in range(m.cells.shape[0]):
+        X = GT.refcell.linear_shape(m.cellkind, e) @ m.coords[m.cells[c]]
+        for a, d in enumerate(V.cell_dofs[c]):
+            vals[d - 1 if d > 0 else V.nfree - d - 1] = X[a].sum()
+    n = V.nfree
+    uhd = GT.DiscreteField(V, np.zeros(n), vals[n:])
+    A, b = GT.linear_problem(uhd, GT.Laplace(dO) + GT.InteriorPenalty(dL, 1.0), GT.Load(dO, 0.0))
+    x = sla.spsolve(A.to_scipy().tocsc(), b)
+    assert np.abs(x - vals[:n]).max() < 1e-10
 
 
 @pytest.mark.parametrize("cells,order", [((3, 3), 2), ((2, 2, 2), 1)])
