@@ -170,6 +170,121 @@ function lift_dirichlet!(c::Ctx, b::Vector{Float64}, xd::Vector{Float64}; rows=F
     b
 end
 
+"New node coordinates, same topology (node-major, D per node); the next assembly uses them."
+function update_coordinates!(c::Ctx, mesh)
+    x = collect(reinterpret(Float64, GT.node_coordinates(mesh)))
+    check(c, ccall((:gtk_update_coordinates, lib), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.h, x), "gtk_update_coordinates")
+end
+
+"Copy the device values of a built (rows, cols) block into `nzval`."
+function get_values!(c::Ctx, nzval::Vector{Float64}; rows=FREE, cols=FREE)
+    check(c, ccall((:gtk_get_values, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}), c.h, rows, cols, nzval),
+          "gtk_get_values")
+    nzval
+end
+
+"y += alpha A_(rows, cols) x with the values the device holds (mul!(y, A, x, alpha, 1))."
+function block_multiply!(c::Ctx, y::Vector{Float64}, x::Vector{Float64}; alpha=1.0, rows=FREE, cols=DIRICHLET)
+    check(c, ccall((:gtk_block_multiply, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Float64, Ptr{Float64}, Ptr{Float64}),
+                   c.h, rows, cols, alpha, x, y), "gtk_block_multiply")
+    y
+end
+
+# ---------------------------------------------------------------------------
+# face mode: boundary, Nitsche and skeleton items (src/compiler.jl:513-529)
+#
+# A face-mode context's "cells" are items: a boundary item is a cell and one
+# of its local faces (set_mesh!/set_space! with the item's cell nodes and
+# the DOFs of its cell, or of the face's closure for loads and mass), a
+# skeleton item is an interior face with both its cells (nodes and DOFs of
+# side 0, then of side 1).  The tables are the face rule's points mapped
+# into the reference cell, one set per local face (skeleton: per local face
+# and vertex order), laid out as include/gtk_hip.h states; the library's
+# Python package builds them the same way (assembly.py, the boundary,
+# Nitsche and skeleton assemblers).
+# ---------------------------------------------------------------------------
+
+"""
+    set_items!(c, D, x, item_nodes, item_dofs, nfree, ndiri; ncomp=1)
+
+Mesh and space of a face-mode context: `item_nodes` (nodes per item x
+items, 1-based) and `item_dofs` (DOFs per item x items, GT's signed
+encoding), `x` the node coordinates (D x nodes).
+"""
+function set_items!(c::Ctx, D, x::AbstractMatrix{Float64}, item_nodes::AbstractMatrix{Int32},
+                    item_dofs::AbstractMatrix{Int32}, nfree, ndiri; ncomp=1)
+    check(c, ccall((:gtk_set_mesh, lib), Cint, (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Int64, Cint, Ptr{Int32}),
+                   c.h, D, size(x, 2), collect(x), size(item_nodes, 2), size(item_nodes, 1), collect(item_nodes)),
+          "gtk_set_mesh")
+    check(c, ccall((:gtk_set_space, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Int32}),
+                   c.h, ncomp, size(item_dofs, 1), nfree, ndiri, collect(item_dofs)), "gtk_set_space")
+end
+
+"""
+    set_reference_faces!(c, w, phi, grad_geo, face_jac, item_face)
+
+Per point set f: `phi[:, :, f]` (nshape x nq), `grad_geo[:, :, :, f]`
+(D x nv x nq), `face_jac[:, :, f]` (D x D-1), with `w` the face rule's
+weights and `item_face` each item's 0-based set (skeleton items: set of
+side 0 | set of side 1 << 16).
+"""
+function set_reference_faces!(c::Ctx, w::Vector{Float64}, phi::Array{Float64,3}, grad_geo::Array{Float64,4},
+                              face_jac::Array{Float64,3}, item_face::Vector{Int32})
+    nf = size(phi, 3)
+    check(c, ccall((:gtk_set_reference_faces, lib), Cint,
+                   (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}),
+                   c.h, nf, length(w), w, size(phi, 1), vec(phi), vec(grad_geo), vec(face_jac), item_face),
+          "gtk_set_reference_faces")
+end
+
+"""
+    set_reference_face_gradients!(c, grad_phi, normals)
+
+Nitsche and skeleton terms: the reference gradients at each set's points,
+`grad_phi[:, :, :, f]` (D x nshape x nq), and the reference cell's outward
+unit normal of each set's local face, `normals[:, f]` (src/domain.jl:226,
+:258).
+"""
+function set_reference_face_gradients!(c::Ctx, grad_phi::Array{Float64,4}, normals::Matrix{Float64})
+    check(c, ccall((:gtk_set_reference_face_gradients, lib), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                   c.h, vec(grad_phi), vec(normals)), "gtk_set_reference_face_gradients")
+end
+
+"""
+    assemble_penalty!(c, nzval, code, beta; precision=FP64, rows=FREE, cols=FREE)
+
+Nitsche (`NITSCHE`: -(∂n u) v - u (∂n v) + β u v over boundary items) or
+interior-penalty (`SKELETON`: -{∂n u}[v] - [u]{∂n v} + β [u][v] over
+skeleton items) values of a built face-mode block, with `beta[i] = γ / h`
+per item (h the face's diameter).
+"""
+function assemble_penalty!(c::Ctx, nzval::Vector{Float64}, code::Cint, beta::Vector{Float64};
+                           precision=FP64, rows=FREE, cols=FREE)
+    set_cell_coefficients!(c, beta, 1)
+    assemble_values!(c, nzval, code, Cint(precision), Cint(rows), Cint(cols))
+end
+
+"dst's (rows, cols) values += src's: a boundary or skeleton block summed into the volume block on the device."
+function add_block!(dst::Ctx, src::Ctx; rows=FREE, cols=FREE)
+    check(dst, ccall((:gtk_add_block, lib), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}), dst.h, rows, cols, src.h),
+          "gtk_add_block")
+end
+
+"""
+    gather_block!(dst, src, ptr, idx; rows=FREE, cols=FREE, src_rows=FREE, src_cols=FREE)
+
+dst's values set, in order, from src's: value t of dst is the sum of src
+values `idx[ptr[t]+1 : ptr[t+1]]` (0-based `ptr` of length nnz(dst) + 1 and
+`idx` into src's values): skeleton terms assembled on the cells' own DOF
+copies, summed onto a continuous space's pattern.
+"""
+function gather_block!(dst::Ctx, src::Ctx, ptr::Vector{Int64}, idx::Vector{Int64}; rows=FREE, cols=FREE,
+                       src_rows=FREE, src_cols=FREE)
+    check(dst, ccall((:gtk_gather_block, lib), Cint,
+                     (Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Cint, Cint, Ptr{Int64}, Ptr{Int64}),
+                     dst.h, rows, cols, src.h, src_rows, src_cols, ptr, idx), "gtk_gather_block")
+end
+
 # ---------------------------------------------------------------------------
 # forms the library recognises (the integrands of src/compiler.jl:1544-1606
 # the supported weak forms lower to)
