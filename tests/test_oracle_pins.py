@@ -265,7 +265,8 @@ def test_laplace_row_sums_and_symmetry(GT, O, cells, order, perturb):
 @pytest.mark.parametrize("cells,order,perturb,simplex", [((4, 5), 1, 0.2, False), ((3, 4), 2, 0.2, False),
                                                           ((3, 3), 3, 0.15, False), ((3, 2, 3), 1, 0.2, False),
                                                           ((2, 3, 2), 2, 0.2, False), ((2, 2, 2), 2, 0.0, True),
-                                                          ((3, 3), 2, 0.1, True)])
+                                                          ((3, 3), 2, 0.1, True), ((4, 3), 1, 0.2, True),
+                                                          ((2, 3, 2), 1, 0.15, True), ((2, 2, 2), 2, 0.15, True)])
 def test_patch_linear(GT, O, cells, order, perturb, simplex):
     """u = sum(x) is reproduced exactly (assembly_tests.jl:75-118)."""
     tol = GOLD["patch_test_tolerance"]["tol"]
@@ -285,13 +286,15 @@ def test_patch_linear(GT, O, cells, order, perturb, simplex):
     assert np.max(np.abs(u - xf.sum(axis=1))) < tol
 
 
-@pytest.mark.parametrize("cells,order", [((3, 4), 2), ((3, 3), 3), ((2, 2, 3), 2)])
-def test_patch_quadratic(GT, O, cells, order):
-    """u = |x|^2 with f = -Lap u = -2D is exact for Q2, Q3 on affine meshes."""
+@pytest.mark.parametrize("cells,order,simplex", [((3, 4), 2, False), ((3, 3), 3, False), ((2, 2, 3), 2, False),
+                                                  ((3, 4), 2, True), ((2, 2, 2), 2, True)])
+def test_patch_quadratic(GT, O, cells, order, simplex):
+    """u = |x|^2 with f = -Lap u = -2D is exact for Q2, Q3 on affine meshes
+    and for P2 on triangles and tetrahedra (affine cells)."""
     D = len(cells)
-    m = GT.cartesian_mesh(tuple([0.0, 1.0] * D), cells)
+    m = GT.cartesian_mesh(tuple([0.0, 1.0] * D), cells, simplexify=simplex)
     V = GT.lagrange_space(GT.interior(m), order, dirichlet_boundary=GT.boundary(m))
-    p = O.Problem(list(cells)).space(order, dirichlet=1)
+    p = O.Problem(list(cells), simplexify=simplex).space(order, dirichlet=1)
     p.quadrature(2 * order)
     nf = p.q(O.Q_NFREE)
     A = csc(*p.assemble_matrix(O.LAPLACE), nf)
@@ -300,6 +303,34 @@ def test_patch_quadratic(GT, O, cells, order):
     xf, xd = dof_coordinates(GT, m, V)
     u = spla.spsolve(A.tocsc(), b - Ad @ (xd ** 2).sum(axis=1))
     assert np.max(np.abs(u - (xf ** 2).sum(axis=1))) < 1e-10
+
+
+@pytest.mark.parametrize("cells,order", [((4, 3), 1), ((3, 2), 2), ((2, 3, 2), 1), ((2, 2, 2), 2)])
+def test_simplex_mass_and_load(GT, O, cells, order):
+    """On perturbed simplex meshes (each cell affine): sum(M) = |Omega| and
+    sum(b(f=1)) = |Omega| without Dirichlet rows (problems_tests.jl:55-79),
+    b(f nodal) = M f, and each cell's mass total is its volume."""
+    D = len(cells)
+    m = GT.cartesian_mesh(tuple([0.0, 1.0] * D), cells, simplexify=True)
+    x = GT.perturb_interior_nodes(m, 0.2, 5)
+    p = O.Problem(list(cells), simplexify=True).space(order)
+    p.set_coords(x)
+    p.quadrature(2 * order)
+    nf = p.q(O.Q_NFREE)
+    M = csc(*p.assemble_matrix(O.MASS), nf)
+    assert abs(M.sum() - 1.0) < 1e-13
+    assert abs(p.assemble_vector(0, [1.0]).sum() - 1.0) < 1e-13
+    f = np.random.default_rng(3).standard_normal(nf)
+    assert np.allclose(p.assemble_vector(2, f), M @ f, atol=1e-14, rtol=1e-12)
+    # a per-cell coefficient that is 1 on one cell only: its volume
+    nc = p.q(O.Q_NCELLS)
+    xm = x[m.cells]
+    e = xm[:, 1:] - xm[:, :1]
+    vol = np.abs(np.linalg.det(e)) / (2.0 if D == 2 else 6.0)
+    for c in (0, nc // 2, nc - 1):
+        k = np.zeros(nc)
+        k[c] = 1.0
+        assert abs(csc(*p.assemble_matrix(O.MASS, coef=k), nf).sum() - vol[c]) < 1e-14
 
 
 def test_mass_and_load_consistency(GT, O):
