@@ -385,41 +385,53 @@ def test_elasticity_rigid_modes(GT, O, cells, simplex):
 
 
 def _vector_dofs(GT, m, V):
-    """Coordinates and component of each free DOF of a three-component
-    space (component-major inside each node)."""
+    """Coordinates and component of each free and Dirichlet DOF of a
+    D-component space (component-major inside each node)."""
+    D = m.coords.shape[1]
     xf, xd = dof_coordinates(GT, m, V)
-    return xf, np.tile(np.arange(3), V.nfree // 3), xd, np.tile(np.arange(3), V.ndiri // 3)
+    return xf, np.tile(np.arange(D), V.nfree // D), xd, np.tile(np.arange(D), V.ndiri // D)
 
 
-def _tet_volumes(m):
+def _simplex_volumes(m):
     X = m.coords[m.cells]
-    return np.abs(np.einsum("cij,cij->c", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])[:, None, :],
-                            (X[:, 3] - X[:, 0])[:, None, :])) / 6.0
+    return np.abs(np.linalg.det(X[:, 1:] - X[:, :1])) / math.factorial(X.shape[2])
 
 
-@pytest.mark.parametrize("cells,simplex", [((2, 2, 2), True), ((2, 3, 2), False)])
-def test_elasticity_uniform_strain_energy(GT, O, cells, simplex):
+# (cells, order, simplex): P2 tetrahedra and Q2 hexahedra (the benchmark's
+# kinds) and the other element kinds of the elasticity kernels
+ELASTIC_KINDS = [((2, 2, 2), 2, True), ((2, 3, 2), 2, False), ((2, 2, 2), 1, True), ((2, 3, 2), 1, False),
+                 ((3, 2), 1, True), ((3, 2), 2, True), ((3, 4), 1, False), ((2, 3), 2, False)]
+
+
+def _elastic_degree(D, order, simplex):
+    return 5 if (simplex and D == 3 and order == 2) else 2 * order
+
+
+@pytest.mark.parametrize("cells,order,simplex", ELASTIC_KINDS)
+def test_elasticity_uniform_strain_energy(GT, O, cells, order, simplex):
     """u = E x (E not symmetric) has the uniform strain eps = (E + E^T)/2 and
     the energy u^T K u = sum_c |Omega_c| (lambda_c (tr eps)^2 + 2 mu_c eps:eps)
     (sigma = lambda tr(eps) I + 2 mu eps, docs example_linear_elasticity.jl:
-    86-95).  lambda != mu per cell on tets (cell volumes exact), constant
-    lambda = 0.6, mu = 1.7 on hexes (total volume 1): a swap of lambda and
-    mu, a factor on mu or a missing skew cancellation fails."""
-    m = GT.cartesian_mesh((0.0, 1.0) * 3, cells, simplexify=simplex)
+    86-95; in 2D the same law, plane strain).  lambda != mu per cell on
+    simplices (cell volumes exact), constant lambda = 0.6, mu = 1.7 on
+    quadrilaterals and hexahedra (total volume 1): a swap of lambda and mu, a
+    factor on mu or a missing skew cancellation fails."""
+    D = len(cells)
+    m = GT.cartesian_mesh((0.0, 1.0) * D, cells, simplexify=simplex)
     m.coords = GT.perturb_interior_nodes(m, 0.15, 8)
-    V = GT.lagrange_space(GT.interior(m), 2, tensor_size=(3,))
-    p = O.Problem(list(cells), simplexify=simplex).space(2, ncomp=3)
+    V = GT.lagrange_space(GT.interior(m), order, tensor_size=(D,))
+    p = O.Problem(list(cells), simplexify=simplex).space(order, ncomp=D)
     p.set_coords(m.coords)
-    p.quadrature(5 if simplex else 4)
+    p.quadrature(_elastic_degree(D, order, simplex))
     nc = p.q(O.Q_NCELLS)
     rng = np.random.default_rng(9)
     if simplex:
-        lam, mu, vol = rng.uniform(0.5, 2.0, nc), rng.uniform(0.5, 2.0, nc), _tet_volumes(m)
+        lam, mu, vol = rng.uniform(0.5, 2.0, nc), rng.uniform(0.5, 2.0, nc), _simplex_volumes(m)
     else:
         lam, mu, vol = np.full(nc, 0.6), np.full(nc, 1.7), None
     K = csc(*p.assemble_matrix(O.ELASTICITY, coef=np.stack([lam, mu], axis=1).ravel()), p.q(O.Q_NFREE))
     xf, comp, _, _ = _vector_dofs(GT, m, V)
-    E = np.array([[0.3, -1.1, 0.4], [0.7, -0.2, 0.9], [-0.5, 0.25, 1.3]])
+    E = np.array([[0.3, -1.1, 0.4], [0.7, -0.2, 0.9], [-0.5, 0.25, 1.3]])[:D, :D]
     u = np.einsum("kj,kj->k", E[comp], xf)
     eps = 0.5 * (E + E.T)
     per_volume = lambda l, k: l * np.trace(eps) ** 2 + 2.0 * k * np.sum(eps * eps)
@@ -430,26 +442,29 @@ def test_elasticity_uniform_strain_energy(GT, O, cells, simplex):
     assert abs(swapped - want) > 1e-3 * want
 
 
-@pytest.mark.parametrize("cells,simplex", [((2, 2, 2), True), ((2, 2, 3), False)])
-def test_elasticity_linear_patch_solve(GT, O, cells, simplex):
+@pytest.mark.parametrize("cells,order,simplex", ELASTIC_KINDS)
+def test_elasticity_linear_patch_solve(GT, O, cells, order, simplex):
     """Constant lambda != mu, no body force, u = E x + c prescribed on the
     whole boundary: the discrete solution at the free DOFs is E x + c
-    exactly (P2 and Q2 reproduce linear fields; tolerance of the
+    exactly (every kind reproduces linear fields; tolerance of the
     reference's patch tests, test/assembly_tests.jl:112-118)."""
-    m = GT.cartesian_mesh((0.0, 1.0) * 3, cells, simplexify=simplex)
+    D = len(cells)
+    cells = tuple(max(c, 2) for c in cells)  # at least one free node
+    m = GT.cartesian_mesh((0.0, 1.0) * D, cells, simplexify=simplex)
     m.coords = GT.perturb_interior_nodes(m, 0.15, 12)
-    V = GT.lagrange_space(GT.interior(m), 2, tensor_size=(3,), dirichlet_boundary=GT.boundary(m))
-    p = O.Problem(list(cells), simplexify=simplex).space(2, ncomp=3, dirichlet=1)
+    V = GT.lagrange_space(GT.interior(m), order, tensor_size=(D,), dirichlet_boundary=GT.boundary(m))
+    p = O.Problem(list(cells), simplexify=simplex).space(order, ncomp=D, dirichlet=1)
     p.set_coords(m.coords)
-    p.quadrature(5 if simplex else 4)
+    p.quadrature(_elastic_degree(D, order, simplex))
     nc = p.q(O.Q_NCELLS)
     lm = np.stack([np.full(nc, 0.8), np.full(nc, 1.9)], axis=1).ravel()
     nf = p.q(O.Q_NFREE)
+    assert nf > 0
     K = csc(*p.assemble_matrix(O.ELASTICITY, coef=lm), nf)
     Kd = csc(*p.assemble_matrix(O.ELASTICITY, coef=lm, rows=O.FREE, cols=O.DIRICHLET), nf)
     xf, cf, xd, cd = _vector_dofs(GT, m, V)
-    E = np.array([[0.4, 0.1, -0.3], [-0.6, 0.2, 0.5], [0.9, -0.7, 0.1]])
-    c = np.array([0.2, -0.1, 0.05])
+    E = np.array([[0.4, 0.1, -0.3], [-0.6, 0.2, 0.5], [0.9, -0.7, 0.1]])[:D, :D]
+    c = np.array([0.2, -0.1, 0.05])[:D]
     g = lambda x, k: np.einsum("kj,kj->k", E[k], x) + c[k]
     u = spla.spsolve(K.tocsc(), -(Kd @ g(xd, cd)))
     assert np.max(np.abs(u - g(xf, cf))) < 1e-10
