@@ -170,9 +170,9 @@ static int resolve_scratch(int device) {
 
 // Zero fill for the no-alloc path (a kernel, not a memset node, so callers
 // may capture the call in a graph).
-// clears a batched launch's item counter: the first node of its graph
-__global__ void claim_reset_kernel(int *counter) {
-  if (threadIdx.x == 0) counter[threadIdx.x] = 0;
+// clears the item counters of a batch's launches: the first node of its graph
+__global__ void claim_reset_kernel(int *counter, int n) {
+  if ((int)threadIdx.x < n) counter[threadIdx.x] = 0;
 }
 
 __global__ void zero_fill_kernel(double *p, int64_t n) {
@@ -1661,9 +1661,15 @@ struct gtk_batch {
   std::vector<std::unique_ptr<Load>> loads;  // per problem
   std::vector<gtk::VectorArgs> vargs;       // host copy of d_loads
   DevBuf d_problems, d_loads, d_types, d_fkinds, d_items, d_counter;
-  gtk::KernelInfo k;
-  size_t lds = 0;
-  int nitems = 0, grid = 0;
+  // the graph's persistent launches, in order: one per element type of a
+  // hexahedral batch (Q1 tiles, then Q2 tiles), else one; each claims items
+  // from its own counter
+  struct Launch {
+    gtk::KernelInfo k;
+    size_t lds = 0;
+    int first = 0, nitems = 0, grid = 0;
+  };
+  std::vector<Launch> launches;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int64_t cells = 0, nnz = 0;
@@ -1820,40 +1826,49 @@ int gtk_batch_build(gtk_batch *b) {
   if (b->ctxs.empty()) BFAIL(b, GTK_ESTATE, "gtk_batch_add problems first");
   BHIPCHK(b, hipSetDevice(b->device));
   batch_release_graph(b);
-  if (!gtk::find_batched_kernel(b->form, b->precision, b->waves, b->family, b->k))
-    BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
+  // a hexahedral batch runs one launch per element type: a kernel holding
+  // both tile bodies keeps them as calls, whose saved registers go to
+  // scratch at every tile (mixed 4.22 -> 3.80 ms, FP64 6.02 -> 5.70 ms on
+  // 64 uneven Q1/Q2 problems; bytes written per launch 5.0 -> 2.6 GB)
+  const bool split = b->family == 1;
+  const int ngroups = split ? 2 : 1;
+  std::vector<gtk_batch::Launch> groups(ngroups);
+  for (int g = 0; g < ngroups; g++)
+    if (!gtk::find_batched_kernel(b->form, b->precision, b->waves, split ? gtk::kHexQ1Launch + g : b->family,
+                                  groups[g].k))
+      BFAIL(b, GTK_EUNSUP, "no batched kernel for the form");
   b->plans.clear();
   b->gens.clear();
   std::vector<gtk::MatrixArgs> args;
   std::vector<gtk::VectorArgs> vargs(b->ctxs.size());
-  std::vector<int32_t> types, fkinds(b->ctxs.size(), -1);
+  std::vector<int32_t> types, fkinds(b->ctxs.size(), -1), group(b->ctxs.size(), 0);
   // work items with their estimated cost (cells x shape functions per
-  // cell): matrix tiles, then load-vector tiles, each by descending cost so
-  // that the claims of the last workgroups are short items
-  std::vector<std::pair<int64_t, int2>> mitems, vitems;
-  b->lds = 0;
+  // cell), per launch: matrix tiles, then load-vector tiles, each by
+  // descending cost so that the claims of the last workgroups are short items
+  std::vector<std::vector<std::pair<int64_t, int2>>> mitems(ngroups), vitems(ngroups);
   b->cells = b->nnz = 0;
   for (size_t p = 0; p < b->ctxs.size(); p++) {
     gtk_ctx *ctx = b->ctxs[p];
     MatrixPlan *P = find_plan(ctx, GTK_FREE, GTK_FREE);
     const int type = gtk::batched_type(elem_key(ctx));
+    const int g = split ? type % 3 : 0;
+    group[p] = g;
     std::unique_ptr<ExecPlan> ep;
     int rc = build_exec(ctx, *P, b->form, b->precision, b->waves, b->tile_cells[type], ep);
     if (rc) BFAIL(b, rc, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
     if (!ep->k.mfma || ep->bat.off_bytes != 1)
       BFAIL(b, GTK_EUNSUP, "a problem does not fit the matrix-core batches with byte offsets");
-    b->lds = std::max(b->lds, ep->lds);
+    groups[g].lds = std::max(groups[g].lds, ep->lds);
     args.push_back(matrix_args(ctx, *P, *ep));
-    types.push_back(type % 3);  // slot in the family's kernel
+    types.push_back(type % 3);
     const int64_t ns = elem_key(ctx).ns;
     for (int t = 0; t < ep->tiles.ntiles; t++)
-      mitems.push_back({ns * (ep->tiles.tile_cell_ptr[t + 1] - ep->tiles.tile_cell_ptr[t]), make_int2((int)p, t)});
+      mitems[g].push_back({ns * (ep->tiles.tile_cell_ptr[t + 1] - ep->tiles.tile_cell_ptr[t]), make_int2((int)p, t)});
     b->cells += ctx->n_cells;
     b->nnz += P->pat.nnz;
     b->plans.push_back(std::move(ep));
     b->gens.push_back(ctx->generation);
   }
-  // load-vector tiles after every matrix tile: short items fill the tail
   for (size_t p = 0; p < b->ctxs.size(); p++) {
     const gtk_batch::Load &l = *b->loads[p];
     if (l.fkind < 0) continue;
@@ -1864,24 +1879,25 @@ int gtk_batch_build(gtk_batch *b) {
     if (get_vector_plan(ctx, GTK_FREE, vp, true))
       BFAIL(b, GTK_EINVAL, std::string("problem ") + std::to_string(p) + ": " + ctx->err);
     if (vp->tiles.uncovered) BFAIL(b, GTK_EUNSUP, "a problem's load vector has rows no cell touches");
-    b->lds = std::max(b->lds, vk.table_bytes + (size_t)vp->tiles.max_tile_cells * vk.geo_bytes_per_cell);
+    const int g = group[p];
+    groups[g].lds = std::max(groups[g].lds, vk.table_bytes + (size_t)vp->tiles.max_tile_cells * vk.geo_bytes_per_cell);
     vargs[p] = vector_args(ctx, *vp, l.f.as<double>());
     fkinds[p] = l.fkind;
     const int64_t ns = elem_key(ctx).ns;
     for (int t = 0; t < vp->tiles.ntiles; t++)
-      vitems.push_back({ns * (vp->tiles.tile_cell_ptr[t + 1] - vp->tiles.tile_cell_ptr[t]), make_int2((int)p, ~t)});
+      vitems[g].push_back({ns * (vp->tiles.tile_cell_ptr[t + 1] - vp->tiles.tile_cell_ptr[t]), make_int2((int)p, ~t)});
   }
   auto by_cost = [](const std::pair<int64_t, int2> &x, const std::pair<int64_t, int2> &y) { return x.first > y.first; };
-  std::stable_sort(mitems.begin(), mitems.end(), by_cost);
-  std::stable_sort(vitems.begin(), vitems.end(), by_cost);
   std::vector<int2> items;
-  items.reserve(mitems.size() + vitems.size());
-  for (const auto &m : mitems) items.push_back(m.second);
-  for (const auto &v : vitems) items.push_back(v.second);
-  // the claim slots after the largest tile's LDS
-  const int claim_offset = (int)((b->lds + 15) & ~(size_t)15);
-  b->lds = (size_t)claim_offset + 16;
-  const int zero = 0;
+  for (int g = 0; g < ngroups; g++) {
+    std::stable_sort(mitems[g].begin(), mitems[g].end(), by_cost);
+    std::stable_sort(vitems[g].begin(), vitems[g].end(), by_cost);
+    groups[g].first = (int)items.size();
+    for (const auto &m : mitems[g]) items.push_back(m.second);
+    for (const auto &v : vitems[g]) items.push_back(v.second);
+    groups[g].nitems = (int)items.size() - groups[g].first;
+  }
+  const std::vector<int> zeros(ngroups, 0);
   int rc;
   b->args = args;
   b->vargs = vargs;
@@ -1890,30 +1906,45 @@ int gtk_batch_build(gtk_batch *b) {
   if ((rc = batch_upload(b, b->d_fkinds, fkinds.data(), fkinds.size()))) return rc;
   if ((rc = batch_upload(b, b->d_types, types.data(), types.size()))) return rc;
   if ((rc = batch_upload(b, b->d_items, items.data(), items.size()))) return rc;
-  if ((rc = batch_upload(b, b->d_counter, &zero, 1))) return rc;
-  b->nitems = (int)items.size();
-  if (b->lds > 160 * 1024) BFAIL(b, GTK_EUNSUP, "a problem's tile does not fit in LDS");
-  if (b->lds > 64 * 1024)
-    BHIPCHK(b, hipFuncSetAttribute(b->k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds));
-  int per_cu = 0, cus = 0;
-  BHIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->k.fn, b->k.block, b->lds));
+  if ((rc = batch_upload(b, b->d_counter, zeros.data(), zeros.size()))) return rc;
+  int cus = 0;
   BHIPCHK(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-  b->grid = std::max(1, std::min(b->nitems, std::max(1, per_cu) * std::max(1, cus)));
-  // the whole batch is one persistent launch, captured once as a graph
-  gtk::BatchedArgs ba;
-  ba.problems = b->d_problems.as<gtk::MatrixArgs>();
-  ba.loads = b->d_loads.as<gtk::VectorArgs>();
-  ba.types = b->d_types.as<int32_t>();
-  ba.fkinds = b->d_fkinds.as<int32_t>();
-  ba.items = b->d_items.as<int2>();
-  ba.nitems = b->nitems;
-  ba.counter = b->d_counter.as<int>();
-  ba.claim_offset = claim_offset;
-  void *kargs[] = {&ba};
+  b->launches.clear();
+  std::vector<gtk::BatchedArgs> bas;
+  for (int g = 0; g < ngroups; g++) {
+    gtk_batch::Launch &L = groups[g];
+    if (L.nitems == 0) continue;
+    // the claim slots after the largest tile's LDS
+    const int claim_offset = (int)((L.lds + 15) & ~(size_t)15);
+    L.lds = (size_t)claim_offset + 16;
+    if (L.lds > 160 * 1024) BFAIL(b, GTK_EUNSUP, "a problem's tile does not fit in LDS");
+    if (L.lds > 64 * 1024)
+      BHIPCHK(b, hipFuncSetAttribute(L.k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds));
+    int per_cu = 0;
+    BHIPCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L.k.fn, L.k.block, L.lds));
+    L.grid = std::max(1, std::min(L.nitems, std::max(1, per_cu) * std::max(1, cus)));
+    gtk::BatchedArgs ba;
+    ba.problems = b->d_problems.as<gtk::MatrixArgs>();
+    ba.loads = b->d_loads.as<gtk::VectorArgs>();
+    ba.types = b->d_types.as<int32_t>();
+    ba.fkinds = b->d_fkinds.as<int32_t>();
+    ba.items = b->d_items.as<int2>() + L.first;
+    ba.nitems = L.nitems;
+    ba.counter = b->d_counter.as<int>() + g;
+    ba.claim_offset = claim_offset;
+    bas.push_back(ba);
+    b->launches.push_back(L);
+  }
+  // the whole batch is captured once as a graph: the counters' reset, then
+  // the persistent launches in order
   BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(claim_reset_kernel, dim3(1), dim3(64), 0, b->stream, ba.counter);
+  hipLaunchKernelGGL(claim_reset_kernel, dim3(1), dim3(64), 0, b->stream, b->d_counter.as<int>(), ngroups);
   hipError_t le = hipGetLastError();
-  if (le == hipSuccess) le = hipLaunchKernel(b->k.fn, dim3(b->grid), dim3(b->k.block), kargs, b->lds, b->stream);
+  for (size_t i = 0; i < b->launches.size() && le == hipSuccess; i++) {
+    void *kargs[] = {&bas[i]};
+    const gtk_batch::Launch &L = b->launches[i];
+    le = hipLaunchKernel(L.k.fn, dim3(L.grid), dim3(L.k.block), kargs, L.lds, b->stream);
+  }
   hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
   if (le != hipSuccess) BFAIL(b, GTK_EDEVICE, std::string("hipLaunchKernel: ") + hipGetErrorString(le));
   if (ce != hipSuccess) BFAIL(b, GTK_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));

@@ -111,8 +111,11 @@ bool find_vector_kernel(const ElemKey &key, int fkind, KernelInfo &out);
 // hexahedra with (k+1)^3 (types 3, 4), family 2 over scalar P1/P2
 // triangles with degree-2k rules (types 6, 7); batched_type gives a
 // problem's type (-1: none), type / 3 its family and type % 3 its slot in
-// the family's kernel; 8 or 12 waves per workgroup.
+// the family's kernel; 8 or 12 waves per workgroup.  A hexahedral batch
+// runs one launch per element type, on the kernels of "families"
+// kHexQ1Launch and kHexQ2Launch (one tile body each, inlined).
 constexpr int kBatchedTypes = 8;
+constexpr int kHexQ1Launch = 3, kHexQ2Launch = 4;
 int batched_type(const ElemKey &key);
 bool find_batched_kernel(int form, int precision, int waves, int family, KernelInfo &out);
 

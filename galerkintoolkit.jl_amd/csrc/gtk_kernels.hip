@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "gtk_kernels.h"
 
@@ -1449,7 +1450,8 @@ __global__ __launch_bounds__(NW * 64) void matrix_batch_kernel(MatrixArgs args) 
 // layout) are in `problems`, its element type selects the tile body.  The
 // workgroup reloads the reference table when the problem changes.
 template <typename E, int FORM, typename CT, int NW>
-__device__ inline void batched_tile(unsigned char *raw, const MatrixArgs &a, int tile, bool new_problem) {
+__device__ __attribute__((always_inline)) inline void batched_tile_body(unsigned char *raw, const MatrixArgs &a,
+                                                                    int tile, bool new_problem) {
   using BT = BatchTile<E::d, E::nv, E::ns, E::nq, FORM, CT, uint8_t, NW>;
   const typename BT::Lds l = BT::lds(raw, a);
   if (new_problem) {
@@ -1627,7 +1629,8 @@ __global__ __launch_bounds__(kBlock) void vector_tile_kernel(VectorArgs args) {
 }
 
 template <typename E, int BS>
-__device__ inline void batched_vector_tile(const VectorArgs &v, int tile, int fkind, unsigned char *raw) {
+__device__ __attribute__((always_inline)) inline void batched_vector_tile_body(const VectorArgs &v, int tile, int fkind,
+                                                                           unsigned char *raw) {
   if (fkind == kFConst) vector_tile<E::d, E::nv, E::ns, E::nq, E::nc, kFConst, BS>(v, tile, raw);
   else if (fkind == kFCell) vector_tile<E::d, E::nv, E::ns, E::nq, E::nc, kFCell, BS>(v, tile, raw);
   else vector_tile<E::d, E::nv, E::ns, E::nq, E::nc, kFNodal, BS>(v, tile, raw);
@@ -1638,6 +1641,20 @@ __device__ inline void batched_vector_tile(const VectorArgs &v, int tile, int fk
 // the current item), so the workgroups finish together although the items'
 // costs differ by orders of magnitude; the claim is passed to all threads
 // through two LDS slots used in turn.
+// The tile bodies as functions the compiler may keep as calls (kernels over
+// several element types) and, in a kernel of one element type, inlined:
+// there a call's saved registers would go to scratch at every tile (Q1/Q2
+// hexahedra: 164 bytes per lane, about as many bytes written per launch as
+// the values themselves)
+template <typename E, int FORM, typename CT, int NW>
+__device__ inline void batched_tile(unsigned char *raw, const MatrixArgs &a, int tile, bool new_problem) {
+  batched_tile_body<E, FORM, CT, NW>(raw, a, tile, new_problem);
+}
+template <typename E, int BS>
+__device__ inline void batched_vector_tile(const VectorArgs &v, int tile, int fkind, unsigned char *raw) {
+  batched_vector_tile_body<E, BS>(v, tile, fkind, raw);
+}
+
 template <int FORM, typename CT, int NW, typename E0, typename E1, typename E2>
 __global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -1652,10 +1669,14 @@ __global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba)
     const int2 it = ba.items[item];  // (problem, matrix tile t >= 0 or load-vector tile ~t)
     if (it.y < 0) {
       const int fk = ba.fkinds[it.x];
-      switch (ba.types[it.x]) {
-        case 0: batched_vector_tile<E0, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
-        case 1: batched_vector_tile<E1, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
-        default: batched_vector_tile<E2, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+      if constexpr (std::is_same_v<E0, E1> && std::is_same_v<E1, E2>) {
+        batched_vector_tile_body<E0, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw);
+      } else {
+        switch (ba.types[it.x]) {
+          case 0: batched_vector_tile<E0, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+          case 1: batched_vector_tile<E1, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+          default: batched_vector_tile<E2, NW * 64>(ba.loads[it.x], ~it.y, fk, lds_raw); break;
+        }
       }
       prev = -1;  // the reference tables in LDS are overwritten
       item = next;
@@ -1664,10 +1685,14 @@ __global__ __launch_bounds__(NW * 64) void matrix_batched_kernel(BatchedArgs ba)
     const MatrixArgs &a = ba.problems[it.x];
     const bool np = it.x != prev;
     prev = it.x;
-    switch (ba.types[it.x]) {
-      case 0: batched_tile<E0, FORM, CT, NW>(lds_raw, a, it.y, np); break;
-      case 1: batched_tile<E1, FORM, CT, NW>(lds_raw, a, it.y, np); break;
-      default: batched_tile<E2, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+    if constexpr (std::is_same_v<E0, E1> && std::is_same_v<E1, E2>) {
+      batched_tile_body<E0, FORM, CT, NW>(lds_raw, a, it.y, np);  // one element type: the body inlined
+    } else {
+      switch (ba.types[it.x]) {
+        case 0: batched_tile<E0, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+        case 1: batched_tile<E1, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+        default: batched_tile<E2, FORM, CT, NW>(lds_raw, a, it.y, np); break;
+      }
     }
     item = next;
   }
@@ -1952,6 +1977,10 @@ static KernelInfo batched_info() {
     k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, QuadQ1, QuadQ2, QuadQ3>);
   else if constexpr (FAMILY == 1)  // the third slot is unused
     k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, HexQ1, HexQ2, HexQ2>);
+  else if constexpr (FAMILY == kHexQ1Launch)
+    k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, HexQ1, HexQ1, HexQ1>);
+  else if constexpr (FAMILY == kHexQ2Launch)
+    k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, HexQ2, HexQ2, HexQ2>);
   else
     k.fn = reinterpret_cast<const void *>(&matrix_batched_kernel<FORM, CT, NW, TriP1, TriP2, TriP2>);
   k.mfma = true;
@@ -1970,11 +1999,13 @@ template <int FORM>
 static KernelInfo batched_family_info(int precision, int waves, int family) {
   if (family == 0) return batched_info<FORM, 0>(precision, waves);
   if (family == 1) return batched_info<FORM, 1>(precision, waves);
+  if (family == kHexQ1Launch) return batched_info<FORM, kHexQ1Launch>(precision, waves);
+  if (family == kHexQ2Launch) return batched_info<FORM, kHexQ2Launch>(precision, waves);
   return batched_info<FORM, 2>(precision, waves);
 }
 
 bool find_batched_kernel(int form, int precision, int waves, int family, KernelInfo &out) {
-  if ((waves != 8 && waves != 12) || family < 0 || family > 2) return false;
+  if ((waves != 8 && waves != 12) || family < 0 || family > kHexQ2Launch) return false;
   if (form == kLaplace) out = batched_family_info<kLaplace>(precision, waves, family);
   else if (form == kMass) out = batched_family_info<kMass>(precision, waves, family);
   else return false;
