@@ -510,7 +510,7 @@ def bench_batch(GT, args, count, dim=2):
     verrs = np.array([np.linalg.norm(v - r) / np.linalg.norm(r) for v, r in zip(bm.vectors(), vref)])
     orders = [q["order"] for q in probs]
     achieved = nbytes / (kernmx * 1e-3) / 1e9
-    traffic = measured_traffic("batch256", problems=count, seed=args.seed) if dim == 2 else None
+    traffic = measured_traffic("batch256" if dim == 2 else "batch3d64", problems=count, seed=args.seed)
     nmin, nmax, kmax = BATCH_SHAPES[dim]
     shape = "unit-square" if dim == 2 else "unit-cube"
     out = {
