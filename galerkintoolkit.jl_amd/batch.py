@@ -1,7 +1,8 @@
 """Batched assembly: many independent problems of uneven size (meshes,
-orders, coefficient fields) assembled by one persistent kernel launch,
-captured once in a HIP graph, with the tile size of each element type picked
-by an autotuner that caches its choices.
+orders, coefficient fields) assembled by one persistent kernel launch (a
+batch of hexahedra: one per element type), captured once in a HIP graph,
+with the tile size of each element type picked by an autotuner that caches
+its choices.
 
 The reference assembles one problem per GT.assemble_matrix call
 (src/problems.jl:319-350); MatrixBatch stands for a loop of such calls over

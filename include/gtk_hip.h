@@ -329,8 +329,9 @@ int gtk_solve(gtk_ctx *ctx, int precision, const double *b, double *x, double rt
               double inner_rtol, gtk_solve_stats *stats);
 
 /* Batched assembly: many independent problems of uneven size assembled by
- * one persistent kernel launch over a flattened (problem, tile) queue,
- * captured once in a HIP graph.  A problem is a context on the batch's
+ * one persistent kernel launch over a flattened (problem, tile) queue (a
+ * batch of hexahedra: one launch per element type, Q1 then Q2), captured
+ * once in a HIP graph.  A problem is a context on the batch's
  * device with a (GTK_FREE, GTK_FREE) pattern built and its reference tables
  * and coefficients set, on scalar Q1/Q2/Q3 quadrilaterals with (k+1)^2
  * Gauss points, or (another batch) scalar Q1/Q2 hexahedra with (k+1)^3,
