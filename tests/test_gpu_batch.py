@@ -256,6 +256,44 @@ def test_batch_3d_waves_and_kinds(GT, O):
         GT.MatrixBatch(forms[:1] + forms_of(GT, [q2], "laplace"), spaces[:1] + [q2["V"]], tuning=None)
 
 
+def test_batch_3d_one_order_and_replays(GT, O):
+    """A hexahedral batch runs one launch per element type: batches of Q1
+    problems only and of Q2 problems only (one launch empty) match the
+    oracle, and graph replays of a mixed batch (both launches, each with its
+    own claim counter, cleared by the graph's first node) repeat the first
+    run's values bit for bit."""
+    print("case batch 3d one order / replays", flush=True)
+    probs = make_problems_3d(GT, O, 8, seed=61)
+    loads, lref = loads_of(GT, probs, 900)
+    for order in (1, 2):
+        sel = [i for i, q in enumerate(probs) if q["order"] == order]
+        assert len(sel) >= 2
+        sub = [probs[i] for i in sel]
+        b = GT.MatrixBatch(forms_of(GT, sub, "laplace"), [q["V"] for q in sub], precision=GT.MIXED, tuning=None,
+                           loads=[loads[i] for i in sel])
+        b.run()
+        for q, A in zip(sub, b.matrices()):
+            cp, rv, nz = q["p"].assemble_matrix(O.LAPLACE, coef=q["kappa"])
+            assert np.array_equal(A.colptr, cp) and np.array_equal(A.rowval, rv)
+            assert relfro(A.nzval, nz) <= TOL_MIXED, (order, q["m"].cells.shape)
+        for i, v in zip(sel, b.vectors()):
+            fk, fd = lref[i]
+            assert relfro(v, probs[i]["p"].assemble_vector(fk, fd)) <= TOL_FP64, (order, fk)
+        b.close()
+    b = GT.MatrixBatch(forms_of(GT, probs, "laplace"), [q["V"] for q in probs], precision=GT.MIXED, tuning=None,
+                       loads=loads)
+    b.run()
+    first = [A.nzval.copy() for A in b.matrices()]
+    vfirst = [v.copy() for v in b.vectors()]
+    for _ in range(3):
+        b.run()
+        for a, A in zip(first, b.matrices()):
+            assert np.array_equal(a, A.nzval)
+        for a, v in zip(vfirst, b.vectors()):
+            assert np.array_equal(a, v)
+    b.close()
+
+
 def make_problems_tri(GT, O, count, seed, nmin=2, nmax=12):
     u = GT.inputs.uniform(seed, 4 * count)
     out = []
